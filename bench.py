@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""bench.py -- time-to-converge of the HaploADMIXTURE likelihood engine on one MI355X.
+
+A "step" is one complete fit on one synthetic dataset resident in HBM: init_em! (5 EM passes) +
+admixture_qn! (QN-accelerated block SQP, six genotype passes per iteration) until the reference's
+stopping rule (relative log-likelihood change < rtol, default 1e-7), starting from the same seeded
+parameters every step.  Default workload: I=20,000 individuals, J=200,000 SNP blocks (400k SNPs,
+2-bit packed, 2.0 GB), K=6, Fst=0.05, 2% missing, MIXED precision (FP32 per-pair products, FP64
+accumulation / log-likelihood / QP / QN).
+
+Printed JSON (one line): value = seconds per fit (whole job, one GPU).  Extra fields: the plain FP64
+GPU fit of the same data (time, log-likelihood, relative gap), the time at which the MIXED fit's
+log-likelihood first came within 1e-6 relative of the FP64 converged value, iterations/s, the dominant
+kernel's roofline (HIP events on the engine stream) and the CPU baseline (the FP64 C restatement of the
+reference's loops, timed on a J-subset on the host cores and scaled).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "haploadmixture.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+from haplo_amd import HipAdmixData  # noqa: E402
+from haplo_amd.sim import init_params, simulate  # noqa: E402
+
+WORKLOADS = {
+    # name: I, J, K, Fst, missing, data seed, precision
+    "mixed_I20k_J200k_K6": dict(I=20000, J=200000, K=6, fst=0.05, miss=0.02, seed=3, prec="mixed"),
+    "fp64_I5k_J50k_K4": dict(I=5000, J=50000, K=4, fst=0.05, miss=0.01, seed=2, prec="fp64"),
+    "cpu_I500_J2k_K3": dict(I=500, J=2000, K=3, fst=0.10, miss=0.01, seed=1, prec="fp64"),
+}
+PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}  # MI355X FP32 / FP64 vector peaks (dense)
+
+
+def log(msg):
+    sys.stderr.write(f"[bench {time.strftime('%H:%M:%S')}] {msg}\n")
+    sys.stderr.flush()
+
+
+def flops_per_pair(K: int, kind: str) -> int:
+    """Algorithmic flops per (individual, block) pair (FMA = 2, division / log = 1)."""
+    if kind == "update":
+        return 16 * K + 4 * K * (K + 1) + 24
+    if kind == "loglik":
+        return 8 * K + 8
+    return 24 * K + 16  # EM
+
+
+def cpu_baseline(bed, I, J, K, q0, p0, gpu_iters, em_iters, j_sample):
+    """Time the FP64 C restatement (oracle) on the first j_sample blocks, all individuals: one EM
+    iteration and one QN iteration; scale by J / j_sample and the GPU run's iteration counts."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    oracle.lib()
+    bps = (I + 3) // 4
+    sub = np.ascontiguousarray(bed[: 2 * j_sample * bps])
+    qs = np.asfortranarray(q0)
+    ps = np.asfortranarray(p0[:, : 4 * j_sample])
+    t0 = time.perf_counter()
+    oracle.fit(sub, I, j_sample, qs, ps, em_iters=1, n_iter=0)
+    t_em = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.fit(sub, I, j_sample, qs, ps, em_iters=0, n_iter=1, rtol=0.0)
+    t_it = time.perf_counter() - t0
+    scale = J / j_sample
+    est = scale * (t_em * em_iters + t_it * gpu_iters)
+    return dict(value=est, unit="s (time-to-converge, extrapolated)", cores=oracle.threads(), kind="port",
+                sample=(f"I={I}, first {j_sample} of {J} blocks, K={K}: 1 EM iteration {t_em:.2f}s + 1 QN "
+                        f"iteration {t_it:.2f}s, scaled x{scale:.0f} blocks x ({em_iters} EM + {gpu_iters} QN "
+                        "iterations of the GPU run)"),
+                t_em_sample_s=t_em, t_iter_sample_s=t_it)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--workload", default="mixed_I20k_J200k_K6", choices=sorted(WORKLOADS))
+    ap.add_argument("--prec", choices=["mixed", "fp64"], default=None)
+    ap.add_argument("--rtol", type=float, default=1e-7)
+    ap.add_argument("--max-iter", type=int, default=1000)
+    ap.add_argument("--no-fp64-ref", action="store_true", help="skip the plain FP64 GPU fit")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-blocks", type=int, default=1000)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {a.gpus} is not supported by this build "
+              "(the sharded design over J is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    w = dict(WORKLOADS[a.workload])
+    prec = a.prec or w["prec"]
+    I, J, K = w["I"], w["J"], w["K"]
+    log(f"generating {a.workload}: I={I} J={J} K={K} Fst={w['fst']} miss={w['miss']}")
+    t0 = time.perf_counter()
+    data = simulate(I, J, K, fst=w["fst"], miss=w["miss"], seed=w["seed"], truth=False)
+    q0, p0 = init_params(I, J, K, seed=w["seed"] + 100)
+    log(f"generated in {time.perf_counter() - t0:.1f}s; uploading")
+    eng = HipAdmixData(data.bed, I, J, K, precision=prec)
+    log(f"engine ready (q_chunks, p_chunks) = {eng.get_tuning()}")
+
+    def one_fit(profile=0):
+        eng.set_params(q0, p0)
+        return eng.fit(rtol=a.rtol, n_iter=a.max_iter, profile=profile)
+
+    for wi in range(a.warmup):
+        r = one_fit()
+        log(f"warmup {wi}: {r['iters']} iters, {r['seconds']:.2f}s")
+    eng.sync()
+    t0 = time.perf_counter()
+    runs = []
+    for si in range(a.steps):
+        r = one_fit(profile=1)
+        runs.append(r)
+        log(f"step {si}: ll={r['ll']:.10g} iters={r['iters']} converged={r['converged']} {r['seconds']:.2f}s")
+    eng.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / a.steps
+    r = runs[-1]
+
+    # dominant pass kernel (HIP events on the engine stream inside the timed fits)
+    fam = {
+        "update_p pass (k_pside curvature)": (sum(x["t_ppass_ms"] for x in runs), sum(x["n_ppass"] for x in runs), "update"),
+        "update_q pass (k_qside curvature)": (sum(x["t_qpass_ms"] for x in runs), sum(x["n_qpass"] for x in runs), "update"),
+        "log-likelihood pass (k_qside)": (sum(x["t_ll_ms"] for x in runs), sum(x["n_ll"] for x in runs), "loglik"),
+    }
+    name, (tot_ms, nl, kind) = max(fam.items(), key=lambda kv: kv[1][0])
+    avg_ms = tot_ms / max(nl, 1)
+    fl = flops_per_pair(K, kind) * I * J
+    achieved = fl / (avg_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[prec]
+    breakdown = {k: round(v[0] / a.steps, 2) for k, v in fam.items()}
+    breakdown["QP q (ms)"] = round(sum(x["t_qstep_ms"] for x in runs) / a.steps, 2)
+    breakdown["QP p (ms)"] = round(sum(x["t_pstep_ms"] for x in runs) / a.steps, 2)
+    breakdown["EM (ms)"] = round(sum(x["t_em_ms"] for x in runs) / a.steps, 2)
+    breakdown["ZAL+proj (ms)"] = round(sum(x["t_qn_ms"] for x in runs) / a.steps, 2)
+
+    out = {
+        "metric": "time-to-converge (s) at 1e-6 rel loglik, I=20k J=200k K=6; HBM GB/s",
+        "value": round(ms_per_step / 1000.0, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-products/f64-accum" if prec == "mixed" else "f64",
+        "data": "synthetic (seeded haplotype-admixture generator, Balding-Nichols Fst)",
+        "config": {"workload": a.workload, "I": I, "J": J, "K": K, "fst": w["fst"], "missing": w["miss"],
+                   "precision": prec, "rtol": a.rtol, "em_iters": 5, "Q": 3, "parallelism": "single GPU"},
+        "iters": r["iters"],
+        "converged": bool(r["converged"]),
+        "ll": r["ll"],
+        "iters_per_s": round(r["iters"] / r["seconds"], 3),
+        "kernel_ms_per_fit": breakdown,
+        "genotype_GBps": round(0.5 * I * J * (6 * r["iters"] + 15) / (ms_per_step * 1e-3) / 1e9, 1),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": name, "avg_launch_ms": round(avg_ms, 3), "launches": nl,
+                     "flops_per_launch": fl,
+                     "note": "VALU-bound kernel; peak = MI355X vector peak of the per-pair precision"},
+    }
+    if not a.no_fp64_ref and prec == "mixed":
+        eng.close()
+        log("plain FP64 GPU fit (reference precision) on the same data")
+        e64 = HipAdmixData(data.bed, I, J, K, precision="fp64")
+        e64.set_params(q0, p0)
+        r64 = e64.fit(rtol=a.rtol, n_iter=a.max_iter, profile=0)
+        e64.close()
+        gap = abs(r["ll"] - r64["ll"]) / abs(r64["ll"])
+        tr, tt = r["trace"], r["t_trace"]
+        hit = np.nonzero(np.abs(tr - r64["ll"]) <= 1e-6 * abs(r64["ll"]))[0]
+        out["fp64_gpu"] = {"seconds": round(r64["seconds"], 3), "ll": r64["ll"], "iters": r64["iters"],
+                           "rel_ll_gap_mixed_vs_fp64": gap,
+                           "speedup_mixed_over_fp64": round(r64["seconds"] / r["seconds"], 3)}
+        out["time_to_1e-6_of_fp64_ll_s"] = round(float(tt[hit[0]]), 3) if hit.size else None
+        log(f"fp64: ll={r64['ll']:.10g} iters={r64['iters']} {r64['seconds']:.2f}s; gap {gap:.2e}")
+    if not a.no_cpu_baseline:
+        js = min(a.cpu_sample_blocks, J)
+        log(f"CPU baseline: oracle on {js} blocks")
+        out["cpu_baseline"] = cpu_baseline(data.bed, I, J, K, q0, p0, r["iters"], 5, js)
+        out["speedup_vs_cpu"] = round(out["cpu_baseline"]["value"] / out["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,85 @@
+// hk_device.h — gfx950 device helpers shared by the HaploADMIXTURE pass, QP and fit kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HK_MAXK 8          // largest K with compiled pass kernels
+#define HK_QP_NMAX 32      // 4 * HK_MAXK variables in a P-side QP
+#define HK_WAVE 64
+
+// ---------------------------------------------------------------------------------------------
+// Genotype nibble: bits 0-1 = BED code of SNP 2j, bits 2-3 = BED code of SNP 2j+1.
+// BED codes: 0 hom allele-1, 1 missing, 2 het, 3 hom allele-2 (src/loops.jl:63 maps them to
+// g = 0, 3(missing), 1, 2).  Nibble 0b0101 = both SNPs missing ("double missing").
+// ---------------------------------------------------------------------------------------------
+#define HK_NIB_DM 5u
+
+// Compatibility matrix of one SNP code, S = [[s00, s01], [s01, s11]]: S[a][a'] = 1 when alleles
+// (a, a') on the two haplotype copies fit the genotype.  hom allele-1: s00; hom allele-2: s11;
+// het: s01; missing: all ones.  With hi/lo the code bits: s00 = !hi, s11 = lo, s01 = hi ^ lo.
+template <typename T>
+__device__ __forceinline__ void snp_s(unsigned c, T& s00, T& s01, T& s11) {
+  const unsigned hi = c >> 1, lo = c & 1u;
+  s00 = (T)(hi ^ 1u);
+  s11 = (T)lo;
+  s01 = (T)(hi ^ lo);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Reciprocals.  FP32: the hardware v_rcp_f32 (1 ulp).  FP64: v_rcp_f64 seed + two Newton steps.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float hk_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double hk_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Quad (4-lane) DPP broadcasts and sums.  quad_perm control = p0 | p1<<2 | p2<<4 | p3<<6.
+// ---------------------------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)(unsigned)lo) | ((long long)hi << 32));
+}
+template <int LANE>
+__device__ __forceinline__ float quad_bcast(float v) { return dpp_f<LANE * 0x55>(v); }
+template <int LANE>
+__device__ __forceinline__ double quad_bcast(double v) { return dpp_f<LANE * 0x55>(v); }
+
+// Sum over the 4 lanes of a quad; every lane gets the same (bitwise) value.
+template <typename T>
+__device__ __forceinline__ T quad_sum(T v) {
+  v = v + dpp_f<0xB1>(v);  // quad_perm [1,0,3,2]
+  v = v + dpp_f<0x4E>(v);  // quad_perm [2,3,0,1]
+  return v;
+}
+
+// Wave-uniform value (forces SGPR residency of loop-invariant indices).
+__device__ __forceinline__ int hk_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ---------------------------------------------------------------------------------------------
+// Block reduction of doubles (blockDim.x multiple of 64), fixed order: result valid in thread 0.
+// ---------------------------------------------------------------------------------------------
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  // wave-level tree via LDS for determinism and simplicity
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  return sh[0];
+}

@@ -1,0 +1,61 @@
+// hk_internal.h — declarations shared between the kernel translation units and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hk {
+
+enum { MODE_LL = 0, MODE_GRAD = 1, MODE_CURV = 2 };
+enum { PREC_FP64 = 0, PREC_MIXED = 1 };
+
+// Geometry of one fit's genotype passes.
+struct PassGeom {
+  int64_t I, J;    // individuals, SNP blocks
+  int64_t Ip, Jp;  // padded: Ip multiple of 256, Jp multiple of 64
+  int64_t J8, I8;  // genotype words: Jp / 8, Ip / 8
+  const uint32_t* gq;  // [J8][Ip]
+  const uint32_t* gp;  // [I8][Jp]
+  int qchunks, qwpc;   // Q side: number of j chunks, words per chunk
+  int pchunks, pwpc;   // P side: number of i chunks, words per chunk
+};
+
+int nacc(int K, int mode);
+
+hipError_t launch_repack(const uint8_t* raw, int64_t bps, const PassGeom& g, hipStream_t s);
+hipError_t launch_count_dm(const PassGeom& g, int64_t* dm, hipStream_t s);
+hipError_t launch_decode(const PassGeom& g, uint8_t* codes, hipStream_t s);
+hipError_t launch_qside(int K, int prec, int mode, const PassGeom& g, const void* q, const void* p, double* part,
+                        hipStream_t s);
+hipError_t launch_pside(int K, int prec, int mode, const PassGeom& g, const void* q, const void* p, double* out,
+                        hipStream_t s);
+hipError_t launch_reduce_q(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s);
+hipError_t launch_reduce_p(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s);
+hipError_t launch_reduce_ll(const double* part, int nchunk, const PassGeom& g, double* rows, double* total,
+                            hipStream_t s);
+hipError_t launch_expand(const double* res, int K, int NT, int64_t n, double* XtX, double* Xtz, hipStream_t s);
+hipError_t launch_em_expand(const double* res, int K, int NT, int64_t n, const double* par, double* out,
+                            hipStream_t s);
+
+// Row steps, projections, EM normalisation, ZAL (hk_step.hip).
+struct StepOpts {
+  int penalty;  // Q: H + rho 11' without equality; P: rho on the (l, l2) block diagonals
+  double rho;
+  double eps;   // pseudocount of the simplex projections
+};
+hipError_t launch_q_step(int K, int64_t I, const double* res_q, const double* q, double* q_next, const StepOpts& o,
+                         int* fails, hipStream_t s);
+hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
+                         int* fails, hipStream_t s);
+hipError_t launch_project_q(int K, int64_t I, double* q, double eps, hipStream_t s);
+hipError_t launch_project_p(int K, int64_t J, double* p, double eps, hipStream_t s);
+hipError_t launch_em_q(int K, int64_t I, int64_t J, const double* res_q, int NT, const int64_t* dm, const double* q,
+                       double* q_next, double eps, hipStream_t s);
+hipError_t launch_em_p(int K, int64_t J, const double* res_p, int NT, const double* p, double* p_next, double eps,
+                       hipStream_t s);
+hipError_t launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
+hipError_t launch_zal(int64_t N, int Qn, int iter, double* U, double* V, const double* x0, const double* x1,
+                      const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s);
+hipError_t launch_axpy_step(int64_t n, const double* a, const double* d, double f, double* out, hipStream_t s);
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+
+}  // namespace hk

@@ -1,0 +1,647 @@
+// hk_step.hip — device-resident SQP row steps, simplex projections, EM normalisation and the
+// Zhou-Alexander-Lange quasi-Newton extrapolation (gfx950, FP64).
+//
+// Reference behaviour (paths relative to the HaploADMIXTURE.jl repository):
+//   update_q! QP stage      src/algorithms_inner.jl:135-168, step + projection :200-207
+//   update_p! QP stage      src/algorithms_inner.jl:259-312 (block-diagonal assembly :272-283,
+//                           default penalty rho on every (l, l2) block diagonal :295-302), step :344-351
+//   em! normalisation       src/algorithms_inner.jl:66-76
+//   project_p!              src/utils.jl:20-28 (per (k, j) 4-vector, pseudocount 1e-5)
+//   ZAL update_UV!/QN!      src/algorithms_outer.jl:128-133
+// The row QPs (OpenADMIXTURE.create_tableau!/quadratic_program!) are strictly convex box QPs with
+// optional group-sum equalities; their optimum is unique.  Here each QP is solved by a primal
+// active-set method on ONE wavefront: lane r owns row r of the (masked) curvature matrix in
+// registers, pivots and columns are broadcast with v_readlane, and the two triangular solves use
+// one LDS transpose.  No global atomics; every reduction runs in a fixed order.
+#include "hk_device.h"
+#include "hk_internal.h"
+
+#include <type_traits>
+
+namespace hk {
+
+// ------------------------------------------------------------------------------------------------
+// Wave utilities
+// ------------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ double rl(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)(unsigned)lo) | ((long long)hi << 32));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+// (value, index) minimum with ties to the smaller index; identical result on every lane.
+__device__ __forceinline__ void wave_argmin(double& v, int& idx) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(v, off, 64);
+    const int oi = __shfl_xor(idx, off, 64);
+    if (ov < v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+}
+__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(v, off, 64);
+    const int oi = __shfl_xor(idx, off, 64);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Simplex projection with pseudocount: argmin ||x - v|| s.t. x >= eps, sum x = 1.
+// Sort (descending) by an odd-even transposition network, then the sorted-prefix threshold; the
+// arithmetic sequence is the same as the CPU restatement's, so results agree bitwise.
+// ------------------------------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void proj_simplex(double (&v)[N], double eps) {
+  double u[N], s[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) { u[t] = v[t] - eps; s[t] = u[t]; }
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int t = r & 1; t + 1 < N; t += 2) {
+      const double a = s[t], b = s[t + 1];
+      s[t] = a < b ? b : a;
+      s[t + 1] = a < b ? a : b;
+    }
+  const double target = 1.0 - N * eps;
+  double cum = 0.0, tau = 0.0;
+#pragma unroll
+  for (int t = 0; t < N; ++t) {
+    cum += s[t];
+    const double tt = (cum - target) / (double)(t + 1);
+    if (s[t] - tt > 0.0) tau = tt;
+  }
+#pragma unroll
+  for (int t = 0; t < N; ++t) {
+    const double y = u[t] - tau;
+    v[t] = (y > 0.0 ? y : 0.0) + eps;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// One-wave active-set QP:  minimise 0.5 x'Hx + g'x  s.t.  lo <= x <= hi,  sum_{grp(i)=c} x_i = 0.
+// Lane r < N holds Hrow = H[r][:], g, lo, hi and its group (0..M-1).  x = 0 must be feasible.
+// Returns the iteration count (>= 0) or a negative code; x holds the solution on lanes < N.
+// lds must hold N*N doubles.
+// ------------------------------------------------------------------------------------------------
+template <int N, int M>
+__device__ int wave_qp(const double (&Hrow)[N], double g, double lo, double hi, int grp, double& x_out,
+                       double* lds) {
+  const int lane = threadIdx.x & 63;
+  const bool act = lane < N;
+  double x = 0.0;
+  int st = act ? (lo >= 0.0 ? -1 : (hi <= 0.0 ? 1 : 0)) : -1;
+  double hm = 0.0;
+#pragma unroll
+  for (int c = 0; c < N; ++c) hm = fmax(hm, fabs(Hrow[c]));
+  const double gs = wave_max(act ? fabs(g) : 0.0);
+  const double bm = wave_max(act ? fmax(fabs(lo), fabs(hi)) : 0.0);
+  hm = wave_max(act ? hm : 0.0);
+  const double tol = 1e-11 * (gs + hm * bm) + 1e-300;
+  double dmax = 0.0;
+#pragma unroll
+  for (int c = 0; c < N; ++c) dmax = (lane == c) ? fabs(Hrow[c]) : dmax;
+  dmax = wave_max(act ? dmax : 0.0);
+
+  const int maxit = 4 * N + 20;
+  int it = 0;
+  int code = 0;
+  for (; it < maxit; ++it) {
+    const bool fr = act && st == 0;
+    const unsigned long long fmask = __ballot(fr);
+    // Masked curvature: H on free x free, identity elsewhere.  Up to one ridge retry.
+    double R[N], dinv = 1.0;
+    bool ok = true;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      const double ridge = attempt ? 1e-12 * dmax + 1e-300 : 0.0;
+#pragma unroll
+      for (int c = 0; c < N; ++c) {
+        const bool fc = (fmask >> c) & 1ull;
+        R[c] = (fr && fc) ? Hrow[c] + (lane == c ? ridge : 0.0) : (lane == c ? 1.0 : 0.0);
+      }
+      ok = true;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const double dkk = rl(R[k], k);
+        if (!(dkk > 0.0)) { ok = false; break; }
+        const double sq = sqrt(dkk);
+        const double isq = 1.0 / sq;
+        R[k] = (lane == k) ? sq : (lane > k ? R[k] * isq : R[k]);
+        dinv = (lane == k) ? isq : dinv;
+#pragma unroll
+        for (int c = k + 1; c < N; ++c) {
+          const double lck = rl(R[k], c);
+          R[c] = __builtin_fma(-R[k], lck, R[c]);
+        }
+      }
+      if (ok) break;
+    }
+    if (!ok) { code = -2; break; }
+    // Transpose L through LDS so lane r also holds column r (LT[k] = L[k][r]).
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int c = 0; c < N; ++c)
+        if (c <= lane) lds[lane * N + c] = R[c];
+    }
+    __syncthreads();
+    double LT[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) LT[k] = (act && k >= lane) ? lds[k * N + lane] : 0.0;
+
+    // Solve M y = b for b held one element per lane.
+    auto solve = [&](double b) -> double {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const double yk = rl(b, k) * rl(dinv, k);
+        b = (lane == k) ? yk : (lane > k ? __builtin_fma(-R[k], yk, b) : b);
+      }
+#pragma unroll
+      for (int k = N - 1; k >= 0; --k) {
+        const double xk = rl(b, k) * rl(dinv, k);
+        b = (lane == k) ? xk : (lane < k ? __builtin_fma(-LT[k], xk, b) : b);
+      }
+      return b;
+    };
+
+    // Gradient at x: r = Hx + g.
+    double r = g;
+#pragma unroll
+    for (int c = 0; c < N; ++c) r = __builtin_fma(Hrow[c], rl(x, c), r);
+    double z = solve(fr ? r : 0.0);
+    double lam_me = 0.0;  // multiplier of this lane's group
+    if constexpr (M > 0) {
+      double Y[M];
+#pragma unroll
+      for (int c = 0; c < M; ++c) Y[c] = solve((fr && grp == c) ? 1.0 : 0.0);
+      double S[M][M], rhs[M];
+      bool has[M];
+#pragma unroll
+      for (int c1 = 0; c1 < M; ++c1) {
+        const bool mine = fr && grp == c1;
+        rhs[c1] = -wave_sum(mine ? z : 0.0);
+        has[c1] = __ballot(mine) != 0ull;
+#pragma unroll
+        for (int c2 = 0; c2 < M; ++c2) S[c1][c2] = wave_sum(mine ? Y[c2] : 0.0);
+      }
+      // Constraints without a free member are already satisfied: identity rows.
+#pragma unroll
+      for (int c1 = 0; c1 < M; ++c1)
+        if (!has[c1]) {
+#pragma unroll
+          for (int c2 = 0; c2 < M; ++c2) { S[c1][c2] = (c1 == c2) ? 1.0 : 0.0; S[c2][c1] = (c1 == c2) ? 1.0 : 0.0; }
+          rhs[c1] = 0.0;
+        }
+      // Cholesky solve of the small SPD system (uniform on every lane).
+      bool sok = true;
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        double d = S[k][k];
+#pragma unroll
+        for (int t = 0; t < k; ++t) d -= S[k][t] * S[k][t];
+        if (!(d > 0.0)) sok = false;
+        d = sqrt(d);
+        S[k][k] = d;
+#pragma unroll
+        for (int rr = k + 1; rr < M; ++rr) {
+          double s = S[rr][k];
+#pragma unroll
+          for (int t = 0; t < k; ++t) s -= S[rr][t] * S[k][t];
+          S[rr][k] = s / d;
+        }
+      }
+      if (!sok) { code = -3; break; }
+#pragma unroll
+      for (int rr = 0; rr < M; ++rr) {
+        double s = rhs[rr];
+#pragma unroll
+        for (int t = 0; t < rr; ++t) s -= S[rr][t] * rhs[t];
+        rhs[rr] = s / S[rr][rr];
+      }
+#pragma unroll
+      for (int rr = M - 1; rr >= 0; --rr) {
+        double s = rhs[rr];
+#pragma unroll
+        for (int t = rr + 1; t < M; ++t) s -= S[t][rr] * rhs[t];
+        rhs[rr] = s / S[rr][rr];
+      }
+#pragma unroll
+      for (int c = 0; c < M; ++c) {
+        z = __builtin_fma(Y[c], rhs[c], z);
+        lam_me = (grp == c) ? rhs[c] : lam_me;
+      }
+    }
+    const double d = fr ? -z : 0.0;
+    // Ratio test.
+    double tc = 2.0;
+    int side = 0;
+    if (fr) {
+      if (d < 0.0 && x + d < lo) { tc = (lo - x) / d; side = -1; }
+      else if (d > 0.0 && x + d > hi) { tc = (hi - x) / d; side = 1; }
+    }
+    double amin = tc;
+    int blk = lane;
+    wave_argmin(amin, blk);
+    const bool blocked = amin < 1.0;
+    const double alpha = blocked ? fmax(amin, 0.0) : 1.0;
+    if (fr) {
+      const double v = x + alpha * d;
+      x = v < lo ? lo : (v > hi ? hi : v);
+    }
+    const int bside = __shfl(side, blk, 64);
+    if (blocked) {
+      if (lane == blk) { x = bside < 0 ? lo : hi; st = bside; }
+      continue;
+    }
+    // At the minimiser of the working set: bound multipliers.
+    double s = g;
+#pragma unroll
+    for (int c = 0; c < N; ++c) s = __builtin_fma(Hrow[c], rl(x, c), s);
+    s += lam_me;
+    double viol = (act && st != 0) ? (st < 0 ? -s : s) : -1.0;
+    int worst = lane;
+    wave_argmax(viol, worst);
+    if (!(viol > tol)) break;
+    if (lane == worst) st = 0;
+  }
+  x_out = x;
+  return code < 0 ? code : it;
+}
+
+// ------------------------------------------------------------------------------------------------
+// update_q! row step: one wavefront per individual.
+// res_q[i][NT]: Xtz (K) then packed lower XtX (K(K+1)/2), as produced by the pass (Xtz not negated).
+// ------------------------------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(64) void k_q_step(int64_t I, const double* __restrict__ res, const double* __restrict__ q,
+                                               double* __restrict__ q_next, int penalty, double rho, double eps,
+                                               int* __restrict__ fails) {
+  constexpr int NT = K + K * (K + 1) / 2;
+  __shared__ double lds[K * K > 64 ? K * K : 64];
+  const int64_t i = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double* r = res + i * NT;
+  const int rr = lane < K ? lane : 0;
+  // Without the penalty the sum(dq) = 0 equality makes sigma 11' vanish on the feasible set; adding
+  // it (sigma = largest diagonal) keeps the optimum and makes the curvature definite on free sets.
+  const double sigma = wave_max(lane < K ? r[K + rr * (rr + 1) / 2 + rr] : 0.0);
+  const double add = penalty ? rho : sigma;
+  double Hrow[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    const int a = rr > c ? rr : c, b = rr > c ? c : rr;
+    Hrow[c] = (lane < K) ? r[K + a * (a + 1) / 2 + b] + add : 0.0;
+  }
+  const double qi = lane < K ? q[rr + (int64_t)K * i] : 0.0;
+  const double g = lane < K ? -r[rr] : 0.0;
+  const double lo = lane < K ? 0.0 - qi : 0.0, hi = lane < K ? 1.0 - qi : 0.0;
+  double x = 0.0;
+  int it;
+  if (penalty) it = wave_qp<K, 0>(Hrow, g, lo, hi, 0, x, lds);
+  else it = wave_qp<K, 1>(Hrow, g, lo, hi, 0, x, lds);
+  if (it < 0) x = 0.0;
+  if (it < 0 && lane == 0) atomicAdd(fails, 1);
+  // q_next = q + dq, then project_q! (lane 0 gathers the K values).
+  double v[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) v[c] = rl(qi + x, c);
+  if (lane == 0) {
+    proj_simplex<K>(v, eps);
+#pragma unroll
+    for (int c = 0; c < K; ++c) q_next[c + (int64_t)K * i] = v[c];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// update_p! row step: one wavefront per SNP block j; variable (l, k) at index l*K + k.
+// ------------------------------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(64) void k_p_step(int64_t J, const double* __restrict__ res, const double* __restrict__ p,
+                                               double* __restrict__ p_next, int penalty, double rho, double eps,
+                                               int* __restrict__ fails) {
+  constexpr int NT = K + K * (K + 1) / 2;
+  constexpr int N = 4 * K;
+  __shared__ double lds[N * N];
+  const int64_t j = blockIdx.x;
+  const int lane = threadIdx.x;
+  const bool act = lane < N;
+  const int l = act ? lane / K : 0, k = act ? lane % K : 0;
+  const double* r = res + (4 * j + l) * NT;
+  // penalty: rho on every (l, l2) block diagonal.  Without it the K equalities sum_l dp[l,k] = 0 make
+  // sigma (11' x I_K) vanish on the feasible set; it is added for definiteness (sigma = largest diagonal).
+  const double sigma = wave_max(act ? r[K + k * (k + 1) / 2 + k] : 0.0);
+  const double add = penalty ? rho : sigma;
+  double Hrow[N];
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    const int l2 = c / K, k2 = c % K;
+    double h = 0.0;
+    if (act && l2 == l) {
+      const int a = k > k2 ? k : k2, b = k > k2 ? k2 : k;
+      h = r[K + a * (a + 1) / 2 + b];
+    }
+    if (act && k2 == k) h += add;
+    Hrow[c] = h;
+  }
+  const double pv = act ? p[k + (int64_t)K * (4 * j + l)] : 0.0;
+  const double g = act ? -r[k] : 0.0;
+  const double lo = act ? 0.0 - pv : 0.0, hi = act ? 1.0 - pv : 0.0;
+  double x = 0.0;
+  int it;
+  if (penalty) it = wave_qp<N, 0>(Hrow, g, lo, hi, k, x, lds);
+  else it = wave_qp<N, K>(Hrow, g, lo, hi, k, x, lds);
+  if (it < 0) x = 0.0;
+  if (it < 0 && lane == 0) atomicAdd(fails, 1);
+  // p_next = p + dp, then project_p! on each (k, j) 4-vector: lane k gathers haplotypes 0..3.
+  const double nv = pv + x;
+  double v[4];
+#pragma unroll
+  for (int h = 0; h < 4; ++h) v[h] = __shfl(nv, h * K + (lane < K ? lane : 0), 64);
+  if (lane < K) {
+    proj_simplex<4>(v, eps);
+#pragma unroll
+    for (int h = 0; h < 4; ++h) p_next[lane + (int64_t)K * (4 * j + h)] = v[h];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Projections and EM normalisation (thread per column / per (k, j)).
+// ------------------------------------------------------------------------------------------------
+template <int K>
+__global__ void k_project_q(int64_t I, double* __restrict__ q, double eps) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I) return;
+  double v[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) v[c] = q[c + (int64_t)K * i];
+  proj_simplex<K>(v, eps);
+#pragma unroll
+  for (int c = 0; c < K; ++c) q[c + (int64_t)K * i] = v[c];
+}
+
+__global__ void k_project_p(int K, int64_t J, double* __restrict__ p, double eps) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)K * J) return;
+  const int64_t j = idx / K;
+  const int k = (int)(idx - j * K);
+  double v[4];
+#pragma unroll
+  for (int h = 0; h < 4; ++h) v[h] = p[k + (int64_t)K * (4 * j + h)];
+  proj_simplex<4>(v, eps);
+#pragma unroll
+  for (int h = 0; h < 4; ++h) p[k + (int64_t)K * (4 * j + h)] = v[h];
+}
+
+// em! (algorithms_inner.jl:66,75): q_next = q .* grad / (2 (J - dm_i)), then project_q!.
+template <int K>
+__global__ void k_em_q(int64_t I, int64_t J, const double* __restrict__ res, int NT, const int64_t* __restrict__ dm,
+                       const double* __restrict__ q, double* __restrict__ q_next, double eps) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I) return;
+  const double den = (double)(2 * (J - dm[i]));
+  double v[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) v[c] = q[c + (int64_t)K * i] * res[i * NT + c] / den;
+  proj_simplex<K>(v, eps);
+#pragma unroll
+  for (int c = 0; c < K; ++c) q_next[c + (int64_t)K * i] = v[c];
+}
+
+// em! (algorithms_inner.jl:67-76): p_next = p .* grad normalised over the 4 haplotypes, then project.
+__global__ void k_em_p(int K, int64_t J, const double* __restrict__ res, int NT, const double* __restrict__ p,
+                       double* __restrict__ p_next, double eps) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)K * J) return;
+  const int64_t j = idx / K;
+  const int k = (int)(idx - j * K);
+  double v[4];
+#pragma unroll
+  for (int h = 0; h < 4; ++h) v[h] = p[k + (int64_t)K * (4 * j + h)] * res[(4 * j + h) * NT + k];
+  const double s = v[0] + v[1] + v[2] + v[3];
+#pragma unroll
+  for (int h = 0; h < 4; ++h) v[h] /= s;
+  proj_simplex<4>(v, eps);
+#pragma unroll
+  for (int h = 0; h < 4; ++h) p_next[k + (int64_t)K * (4 * j + h)] = v[h];
+}
+
+__global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+__global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_axpy(int64_t n, const double* __restrict__ a, const double* __restrict__ d, double f,
+                       double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + f * d[i];
+}
+
+// ------------------------------------------------------------------------------------------------
+// ZAL quasi-Newton: U[:,col] = x1 - x0, V[:,col] = x2 - x1; dots U'U, U'V, U'(x0 - x1);
+// x_out = x1 - V (U'U - U'V)^{-1} U'(x0 - x1).  Qn <= 4.
+// ------------------------------------------------------------------------------------------------
+#define ZAL_QMAX 4
+#define ZAL_NDOT (ZAL_QMAX * ZAL_QMAX * 2 + ZAL_QMAX)
+#define ZAL_NB 512
+
+__global__ __launch_bounds__(256) void k_zal_dots(int64_t N, int col, int ncols, double* __restrict__ U,
+                                                  double* __restrict__ V, const double* __restrict__ x0,
+                                                  const double* __restrict__ x1, const double* __restrict__ x2,
+                                                  double* __restrict__ part) {
+  __shared__ double sh[256];
+  double acc[ZAL_NDOT];
+#pragma unroll
+  for (int d = 0; d < ZAL_NDOT; ++d) acc[d] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += stride) {
+    const double a0 = x0[t], a1 = x1[t], a2 = x2[t];
+    const double u = a1 - a0, v = a2 - a1, rr = a0 - a1;
+    U[t + N * col] = u;
+    V[t + N * col] = v;
+    double ua[ZAL_QMAX], va[ZAL_QMAX];
+#pragma unroll
+    for (int a = 0; a < ZAL_QMAX; ++a) {
+      ua[a] = (a < ncols) ? (a == col ? u : U[t + N * a]) : 0.0;
+      va[a] = (a < ncols) ? (a == col ? v : V[t + N * a]) : 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < ZAL_QMAX; ++a) {
+#pragma unroll
+      for (int b = 0; b < ZAL_QMAX; ++b) {
+        acc[a * ZAL_QMAX + b] = __builtin_fma(ua[a], ua[b], acc[a * ZAL_QMAX + b]);
+        acc[ZAL_QMAX * ZAL_QMAX + a * ZAL_QMAX + b] = __builtin_fma(ua[a], va[b], acc[ZAL_QMAX * ZAL_QMAX + a * ZAL_QMAX + b]);
+      }
+      acc[2 * ZAL_QMAX * ZAL_QMAX + a] = __builtin_fma(ua[a], rr, acc[2 * ZAL_QMAX * ZAL_QMAX + a]);
+    }
+  }
+  for (int d = 0; d < ZAL_NDOT; ++d) {
+    const double s = block_sum<256>(acc[d], sh);
+    if (threadIdx.x == 0) part[blockIdx.x * ZAL_NDOT + d] = s;
+    __syncthreads();
+  }
+}
+
+__global__ void k_zal_solve(int nb, int ncols, const double* __restrict__ part, double* __restrict__ coef) {
+  __shared__ double dots[ZAL_NDOT];
+  const int d = threadIdx.x;
+  if (d < ZAL_NDOT) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[b * ZAL_NDOT + d];
+    dots[d] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const int n = ncols;
+  double A[ZAL_QMAX * ZAL_QMAX], y[ZAL_QMAX];
+  for (int a = 0; a < n; ++a) {
+    y[a] = dots[2 * ZAL_QMAX * ZAL_QMAX + a];
+    for (int b = 0; b < n; ++b) A[a + n * b] = dots[a * ZAL_QMAX + b] - dots[ZAL_QMAX * ZAL_QMAX + a * ZAL_QMAX + b];
+  }
+  int rc = 0;
+  // LU with partial pivoting (same elimination order as the CPU restatement)
+  for (int c = 0; c < n && rc == 0; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (fabs(A[r + n * c]) > fabs(A[piv + n * c])) piv = r;
+    if (A[piv + n * c] == 0.0) { rc = -1; break; }
+    if (piv != c) {
+      for (int t = 0; t < n; ++t) { const double tmp = A[c + n * t]; A[c + n * t] = A[piv + n * t]; A[piv + n * t] = tmp; }
+      const double tmp = y[c]; y[c] = y[piv]; y[piv] = tmp;
+    }
+    for (int r = c + 1; r < n; ++r) {
+      const double f = A[r + n * c] / A[c + n * c];
+      for (int t = c; t < n; ++t) A[r + n * t] -= f * A[c + n * t];
+      y[r] -= f * y[c];
+    }
+  }
+  if (rc == 0)
+    for (int r = n - 1; r >= 0; --r) {
+      double s = y[r];
+      for (int t = r + 1; t < n; ++t) s -= A[r + n * t] * y[t];
+      y[r] = s / A[r + n * r];
+    }
+  for (int a = 0; a < ZAL_QMAX; ++a) coef[a] = (rc == 0 && a < n) ? y[a] : 0.0;
+  coef[ZAL_QMAX] = (double)rc;
+}
+
+__global__ void k_zal_apply(int64_t N, int ncols, const double* __restrict__ V, const double* __restrict__ x1,
+                            const double* __restrict__ coef, double* __restrict__ x_out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  double s = x1[t];
+  if (coef[ZAL_QMAX] == 0.0)
+    for (int a = 0; a < ncols; ++a) s -= V[t + N * a] * coef[a];
+  x_out[t] = s;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------------------------------
+// Calls f(std::integral_constant<int, K>) for the runtime K in 1..HK_MAXK.
+template <int KK = 1, typename F>
+static hipError_t with_k(int K, F&& f) {
+  if constexpr (KK > HK_MAXK) {
+    return hipErrorInvalidValue;
+  } else {
+    if (K == KK) {
+      f(std::integral_constant<int, KK>{});
+      return hipGetLastError();
+    }
+    return with_k<KK + 1>(K, f);
+  }
+}
+
+hipError_t launch_q_step(int K, int64_t I, const double* res_q, const double* q, double* q_next, const StepOpts& o,
+                         int* fails, hipStream_t s) {
+  return with_k(K, [&](auto kc) {
+    constexpr int KT = decltype(kc)::value;
+    hipLaunchKernelGGL((k_q_step<KT>), dim3((unsigned)I), dim3(64), 0, s, I, res_q, q, q_next, o.penalty, o.rho,
+                       o.eps, fails);
+  });
+}
+
+hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
+                         int* fails, hipStream_t s) {
+  return with_k(K, [&](auto kc) {
+    constexpr int KT = decltype(kc)::value;
+    hipLaunchKernelGGL((k_p_step<KT>), dim3((unsigned)J), dim3(64), 0, s, J, res_p, p, p_next, o.penalty, o.rho,
+                       o.eps, fails);
+  });
+}
+
+hipError_t launch_project_q(int K, int64_t I, double* q, double eps, hipStream_t s) {
+  return with_k(K, [&](auto kc) {
+    constexpr int KT = decltype(kc)::value;
+    hipLaunchKernelGGL((k_project_q<KT>), dim3((unsigned)((I + 255) / 256)), dim3(256), 0, s, I, q, eps);
+  });
+}
+
+hipError_t launch_project_p(int K, int64_t J, double* p, double eps, hipStream_t s) {
+  const int64_t n = (int64_t)K * J;
+  hipLaunchKernelGGL(k_project_p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, K, J, p, eps);
+  return hipGetLastError();
+}
+
+hipError_t launch_em_q(int K, int64_t I, int64_t J, const double* res_q, int NT, const int64_t* dm, const double* q,
+                       double* q_next, double eps, hipStream_t s) {
+  return with_k(K, [&](auto kc) {
+    constexpr int KT = decltype(kc)::value;
+    hipLaunchKernelGGL((k_em_q<KT>), dim3((unsigned)((I + 255) / 256)), dim3(256), 0, s, I, J, res_q, NT, dm, q,
+                       q_next, eps);
+  });
+}
+
+hipError_t launch_em_p(int K, int64_t J, const double* res_p, int NT, const double* p, double* p_next, double eps,
+                       hipStream_t s) {
+  const int64_t n = (int64_t)K * J;
+  hipLaunchKernelGGL(k_em_p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, K, J, res_p, NT, p, p_next, eps);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy_step(int64_t n, const double* a, const double* d, double f, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, d, f, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_zal(int64_t N, int Qn, int iter, double* U, double* V, const double* x0, const double* x1,
+                      const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s) {
+  if (Qn < 1 || Qn > ZAL_QMAX) return hipErrorInvalidValue;
+  const int col = (iter - 1) % Qn;
+  const int ncols = iter < Qn ? iter : Qn;
+  hipLaunchKernelGGL(k_zal_dots, dim3(ZAL_NB), dim3(256), 0, s, N, col, ncols, U, V, x0, x1, x2, dots_part);
+  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(64), 0, s, ZAL_NB, ncols, dots_part, coef);
+  hipLaunchKernelGGL(k_zal_apply, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, ncols, V, x1, coef, x_out);
+  return hipGetLastError();
+}
+
+}  // namespace hk

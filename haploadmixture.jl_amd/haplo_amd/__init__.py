@@ -1,0 +1,13 @@
+"""haplo_amd -- MI355X-native likelihood engine for HaploADMIXTURE (PLINK BED, haplotype admixture).
+
+Host-side mirror of the reference's driver / data-layout API on top of the C ABI of libhaplo_hip.so.
+"""
+from ._native import HipError, NativeLibraryError, header_symbols
+from .admix import AdmixData2, _admixture_base, run_admixture
+from .engine import HipAdmixData
+from .sim import SimData, init_params, simulate
+
+__all__ = [
+    "AdmixData2", "HipAdmixData", "HipError", "NativeLibraryError", "SimData", "_admixture_base",
+    "header_symbols", "init_params", "run_admixture", "simulate",
+]

@@ -1,0 +1,103 @@
+"""Reference-shaped entry points: ``run_admixture``, ``_admixture_base``, ``AdmixData2``.
+
+Mirrors src/driver.jl:38-201 (keyword names, defaults and control flow) with the likelihood engine on
+the MI355X.  Differences, each forced by the build:
+  * ``rng`` is an integer seed of the build's counter RNG (the reference takes a Julia AbstractRNG);
+  * ``use_gpu=False`` is rejected: the CPU path of the reference is not part of this library;
+  * ``sparsity`` (SKFR AIM pre-selection, driver.jl:84-131) is not provided;
+  * ``T2=np.float32`` selects the MIXED engine (FP32 per-pair products, FP64 accumulation), the
+    analogue of the reference's ``T2=Float32`` GPU arrays (driver.jl:149, cuda/structs.jl:19-30).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import plink
+from .engine import HipAdmixData
+from .sim import init_params
+
+
+class AdmixData2:
+    """Host view of the fit state (src/structs.jl:14-139): q (K x I), p (K x 4J), x = [q p], lls."""
+
+    def __init__(self, I, J, K, Q=3, rng=1):
+        self.I, self.J, self.K, self.Q = int(I), int(J), int(K), int(Q)
+        self.q, self.p = init_params(self.I, self.J, self.K, seed=int(rng))
+        self.doublemissing = None
+        self.ll_prev = math.nan
+        self.ll_new = math.nan
+        self.ll_tmp = math.nan
+        self.trace = np.zeros(0)
+        self.iters = 0
+        self.converged = False
+
+    @property
+    def x(self):
+        return np.concatenate([self.q, self.p], axis=1)
+
+
+def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iters=5, T=np.float64,
+                    T2=np.float64, Q=3, use_gpu=True, verbose=False, fix_q=False, fix_p=False,
+                    init_q=None, init_p=None, engine_opts=None):
+    """src/driver.jl:133-201."""
+    if not use_gpu:
+        raise NotImplementedError("this library is the GPU engine; use_gpu=False has no implementation here")
+    if T is not np.float64:
+        raise NotImplementedError("parameters and QP run in FP64 (T=Float64)")
+    precision = "mixed" if T2 is np.float32 else "fp64"
+    print("Loading genotype data...")
+    payload, n_ind, n_snps, bps = plink.read_bed(filename)
+    if I > n_ind or 2 * J > n_snps:
+        raise ValueError(f"file has {n_ind} individuals and {n_snps} SNPs; asked for I={I}, 2J={2 * J}")
+    print(f"Loaded {I} samples and {2 * J} SNPs")
+    d = AdmixData2(I, J, K, Q, rng=rng)
+    eng = HipAdmixData(np.asarray(payload[: 2 * J * bps]), I, J, K, bytes_per_snp=bps, n_snp=2 * J,
+                       precision=precision)
+    try:
+        d.doublemissing = eng.double_missing()
+        eng.set_params(d.q, d.p)
+        base = dict(Qn=Q, fix_q=int(fix_q), fix_p=int(fix_p), verbose=int(verbose))
+        if engine_opts:
+            base.update(engine_opts)
+        ll_start = math.nan
+        # init_em! unless both init_q and init_p are given (driver.jl:181-183); it runs without fix_q/fix_p.
+        if init_q is None or init_p is None:
+            r = eng.fit(eng.default_opts(**{**base, "em_iters": em_iters, "n_iter": 0, "fix_q": 0, "fix_p": 0}))
+            ll_start = r["ll"]
+        if init_q is not None or init_p is not None:
+            q, p = eng.get_params()
+            if init_q is not None:
+                q = np.asfortranarray(init_q, dtype=np.float64)
+            if init_p is not None:
+                p = np.asfortranarray(init_p, dtype=np.float64)
+            eng.set_params(q, p)
+        # admixture_qn! (driver.jl:195-197): starts from d.ll_new as left by init_em! (NaN -> evaluated).
+        r = eng.fit(eng.default_opts(**{**base, "em_iters": 0, "n_iter": n_iter, "rtol": rtol}), ll_start=ll_start)
+        d.q, d.p = eng.get_params()
+        d.ll_new = r["ll"]
+        d.trace = r["trace"]
+        d.iters = r["iters"]
+        d.converged = bool(r["converged"])
+        d.ll_prev = d.trace[-2] if d.trace.size > 1 else r["ll_init"]
+        d.fit_info = r
+    finally:
+        eng.close()
+    return d
+
+
+def run_admixture(filename, I, J, K, *, rng=1, sparsity=None, prefix=None, skfr_tries=1, skfr_max_inner_iter=50,
+                  skfr_mode="global", admix_n_iter=1000, admix_rtol=1e-7, admix_n_em_iters=5, T=np.float64,
+                  T2=np.float64, Q=3, use_gpu=True, verbose=False, progress_bar=False, fix_q=False, fix_p=False,
+                  init_q=None, init_p=None):
+    """src/driver.jl:38-79.  Returns (d, clusters, aims); clusters/aims are None without SKFR."""
+    if not str(filename).endswith(".bed"):
+        raise ValueError("filename should end with .bed")
+    print(f"Using {filename} as input.")
+    if sparsity is not None:
+        raise NotImplementedError("SKFR AIM pre-selection (sparsity=...) is not part of this engine")
+    d = _admixture_base(filename, I, J, K, n_iter=admix_n_iter, rtol=admix_rtol, rng=rng,
+                        em_iters=admix_n_em_iters, T=T, T2=T2, Q=Q, use_gpu=use_gpu, verbose=verbose,
+                        fix_q=fix_q, fix_p=fix_p, init_q=init_q, init_p=init_p)
+    return d, None, None

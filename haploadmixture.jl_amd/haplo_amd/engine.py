@@ -1,0 +1,175 @@
+"""``HipAdmixData``: the device context of the MI355X engine (libhaplo_hip.so through its C ABI).
+
+It plays the role of the reference's ``CuAdmixData`` + ``g_cu`` pair (src/cuda/structs.jl:1-44) and of
+the runner functions in src/cuda/runners.jl: ``loglikelihood``, ``em!``/``em_p!``, ``update_q_cuda!``,
+``update_p_cuda!``.  Arrays cross the boundary in the reference's column-major layouts (K x I q,
+K x 4J p, K x K x n curvature).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _native
+from ._native import FitOpts, FitResult, HipError, ptr
+
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+def _F(a, shape):
+    a = np.asfortranarray(a, dtype=np.float64)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+class HipAdmixData:
+    def __init__(self, bed: np.ndarray, I: int, J: int, K: int, bytes_per_snp: int | None = None,
+                 n_snp: int | None = None, precision: str = "fp64"):
+        self._L = _native.hip()
+        self.I, self.J, self.K = int(I), int(J), int(K)
+        bps = (self.I + 3) // 4 if bytes_per_snp is None else int(bytes_per_snp)
+        bed = np.ascontiguousarray(bed, dtype=np.uint8)
+        nsnp = bed.size // bps if n_snp is None else int(n_snp)
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {list(PRECISIONS)}")
+        self.precision = precision
+        self._bed_keepalive = bed
+        h = ctypes.c_void_p()
+        rc = self._L.hk_ctx_create(ctypes.byref(h), self.I, self.J, self.K, ptr(bed), bps, nsnp,
+                                   PRECISIONS[precision])
+        if rc != 0:
+            raise HipError(rc, self._L.hk_create_error().decode())
+        self._h = h
+        del self._bed_keepalive
+
+    # -- plumbing ---------------------------------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            raise HipError(rc, self._L.hk_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.hk_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- parameters -------------------------------------------------------------------------------
+    def set_params(self, q, p):
+        q = _F(q, (self.K, self.I))
+        p = _F(p, (self.K, 4 * self.J))
+        self._check(self._L.hk_set_params(self._h, ptr(q), ptr(p)))
+
+    def get_params(self):
+        q = np.zeros((self.K, self.I), order="F")
+        p = np.zeros((self.K, 4 * self.J), order="F")
+        self._check(self._L.hk_get_params(self._h, ptr(q), ptr(p)))
+        return q, p
+
+    # -- reference runner entry points ------------------------------------------------------------
+    def loglikelihood(self, q=None, p=None, rows=False):
+        """runners.jl:2-16 on (q, p) (or the current parameters)."""
+        if q is not None or p is not None:
+            self.set_params(q, p)
+        ll = ctypes.c_double()
+        r = np.zeros(self.I) if rows else None
+        self._check(self._L.hk_loglik(self._h, ctypes.byref(ll), ptr(r)))
+        return (ll.value, r) if rows else ll.value
+
+    def em(self, fix_q=False):
+        """em!(d_cu, g_cu) / em_p! (runners.jl:18-55): unnormalised (q_next, p_next)."""
+        qn = None if fix_q else np.zeros((self.K, self.I), order="F")
+        pn = np.zeros((self.K, 4 * self.J), order="F")
+        self._check(self._L.hk_em(self._h, int(bool(fix_q)), ptr(qn), ptr(pn)))
+        return qn, pn
+
+    def update_q(self):
+        """update_q_cuda! (runners.jl:58-72): (XtX_q K x K x I, Xtz_q K x I)."""
+        XtX = np.zeros((self.K, self.K, self.I), order="F")
+        Xtz = np.zeros((self.K, self.I), order="F")
+        self._check(self._L.hk_update_q(self._h, ptr(XtX), ptr(Xtz)))
+        return XtX, Xtz
+
+    def update_p(self):
+        """update_p_cuda! (runners.jl:76-92): (XtX_p K x K x 4J, Xtz_p K x 4J)."""
+        XtX = np.zeros((self.K, self.K, 4 * self.J), order="F")
+        Xtz = np.zeros((self.K, 4 * self.J), order="F")
+        self._check(self._L.hk_update_p(self._h, ptr(XtX), ptr(Xtz)))
+        return XtX, Xtz
+
+    def double_missing(self):
+        dm = np.zeros(self.I, dtype=np.int64)
+        self._check(self._L.hk_double_missing(self._h, ptr(dm)))
+        return dm
+
+    def decode(self):
+        """BED codes held by the device stream, shape (2J, I)."""
+        codes = np.zeros((2 * self.J, self.I), dtype=np.uint8)
+        self._check(self._L.hk_decode(self._h, ptr(codes)))
+        return codes
+
+    # -- device-resident row steps ----------------------------------------------------------------
+    def q_step(self, penalty=False, rho=1e7):
+        qn = np.zeros((self.K, self.I), order="F")
+        self._check(self._L.hk_q_step(self._h, int(penalty), float(rho), ptr(qn)))
+        return qn
+
+    def p_step(self, q_pass=None, penalty=True, rho=1e7):
+        pn = np.zeros((self.K, 4 * self.J), order="F")
+        qp = None if q_pass is None else _F(q_pass, (self.K, self.I))
+        self._check(self._L.hk_p_step(self._h, ptr(qp), int(penalty), float(rho), ptr(pn)))
+        return pn
+
+    # -- whole fit ----------------------------------------------------------------------------------
+    @staticmethod
+    def default_opts(**kw) -> FitOpts:
+        o = FitOpts()
+        _native.hip().hk_fit_opts_default(ctypes.byref(o))
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise TypeError(f"unknown fit option {k}")
+            setattr(o, k, v)
+        return o
+
+    def fit(self, opts: FitOpts | None = None, ll_start: float = math.nan, trace: bool = True, **kw):
+        o = opts if opts is not None else self.default_opts(**kw)
+        tr = np.full(max(int(o.n_iter), 1), np.nan) if trace else None
+        tt = np.full(max(int(o.n_iter), 1), np.nan) if trace else None
+        res = FitResult()
+        self._check(self._L.hk_fit_ex(self._h, ctypes.byref(o), float(ll_start), ptr(tr), ptr(tt),
+                                      ctypes.byref(res)))
+        out = res.as_dict()
+        if trace:
+            out["trace"] = tr[: res.iters].copy()
+            out["t_trace"] = tt[: res.iters].copy()
+        return out
+
+    # -- tuning / timing ----------------------------------------------------------------------------
+    def set_tuning(self, q_chunks=0, p_chunks=0):
+        self._check(self._L.hk_set_tuning(self._h, int(q_chunks), int(p_chunks)))
+
+    def get_tuning(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.hk_get_tuning(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def time_pass(self, kind: int, reps: int = 5) -> float:
+        ms = ctypes.c_double()
+        self._check(self._L.hk_time_pass(self._h, int(kind), int(reps), ctypes.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        self._check(self._L.hk_sync(self._h))

