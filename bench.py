@@ -110,7 +110,7 @@ def main():
 
     def one_fit(profile=0):
         eng.set_params(q0, p0)
-        return eng.fit(rtol=a.rtol, n_iter=a.max_iter, profile=profile)
+        return eng.fit(rtol=a.rtol, n_iter=a.max_iter, profile=profile, verbose=1)
 
     for wi in range(a.warmup):
         r = one_fit()
@@ -176,7 +176,7 @@ def main():
         log("plain FP64 GPU fit (reference precision) on the same data")
         e64 = HipAdmixData(data.bed, I, J, K, precision="fp64")
         e64.set_params(q0, p0)
-        r64 = e64.fit(rtol=a.rtol, n_iter=a.max_iter, profile=0)
+        r64 = e64.fit(rtol=a.rtol, n_iter=a.max_iter, profile=0, verbose=1)
         e64.close()
         gap = abs(r["ll"] - r64["ll"]) / abs(r64["ll"])
         tr, tt = r["trace"], r["t_trace"]

@@ -561,12 +561,12 @@ extern "C" int hk_fit_ex(hk_ctx* c, const hk_fit_opts* o, double ll_start, doubl
     CHK(fit_loglik(c, tm, x, &ll_new));
     HIPC(c, launch_project_p(K, J, pof(c, x), o->eps, c->s));
     HIPC(c, launch_project_q(K, I, qof(c, x), o->eps, c->s));
-    if (o->verbose) printf("EM iter %d, ll: %.17g\n", it + 1, ll_new), fflush(stdout);
+    if (o->verbose) fprintf(stderr, "EM iter %d, ll: %.17g\n", it + 1, ll_new);
   }
   if (o->em_iters == 0) ll_new = ll_start;
   if (isnan(ll_new)) CHK(fit_loglik(c, tm, x, &ll_new));
   res->ll_init = ll_new;
-  if (o->verbose) printf("initial ll: %.17g\n", ll_new), fflush(stdout);
+  if (o->verbose) fprintf(stderr, "initial ll: %.17g\n", ll_new);
   const int64_t nq = (int64_t)K * I, np = (int64_t)K * 4 * J;
   int iters = 0, conv = 0;
   for (int i = 1; i <= o->n_iter; ++i) {
@@ -601,9 +601,8 @@ extern "C" int hk_fit_ex(hk_ctx* c, const hk_fit_opts* o, double ll_start, doubl
     if (t_trace) t_trace[i - 1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const double reldiff = fabs((ll_new - ll_prev) / ll_prev);
     if (o->verbose)
-      printf("Iteration %d: ll=%.17g, reldiff = %.6g, ll_basic=%.17g, ll_qn=%.17g\n", i, ll_new, reldiff, ll_basic,
-             ll_qn),
-          fflush(stdout);
+      fprintf(stderr, "Iteration %d: ll=%.17g, reldiff = %.6g, ll_basic=%.17g, ll_qn=%.17g\n", i, ll_new, reldiff,
+              ll_basic, ll_qn);
     if (!isfinite(ll_new)) return fail(c, HK_ENAN, "log-likelihood became non-finite at iteration %d", i);
     if (reldiff < o->rtol) {
       conv = 1;

@@ -123,11 +123,12 @@ __device__ int wave_qp(const double (&Hrow)[N], double g, double lo, double hi, 
   for (; it < maxit; ++it) {
     const bool fr = act && st == 0;
     const unsigned long long fmask = __ballot(fr);
-    // Masked curvature: H on free x free, identity elsewhere.  Up to one ridge retry.
+    // Masked curvature: H on free x free, identity elsewhere.  A curvature that is semidefinite up to
+    // round-off gets an escalating ridge (1e-12, 1e-9, 1e-6 of the largest diagonal).
     double R[N], dinv = 1.0;
     bool ok = true;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-      const double ridge = attempt ? 1e-12 * dmax + 1e-300 : 0.0;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+      const double ridge = attempt == 0 ? 0.0 : (attempt == 1 ? 1e-12 : attempt == 2 ? 1e-9 : 1e-6) * dmax + 1e-300;
 #pragma unroll
       for (int c = 0; c < N; ++c) {
         const bool fc = (fmask >> c) & 1ull;

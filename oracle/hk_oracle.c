@@ -438,12 +438,18 @@ int hko_qp_solve(int n, const double* H, const double* g, const double* lo, cons
     for (int a = 0; a < nf; ++a)
       for (int b = 0; b < nf; ++b) M[a + nf * b] = H[fidx[a] + n * fidx[b]];
     if (nf > 0 && chol_inplace(M, nf, nf) != 0) {
-      /* Singular curvature on the free set: regularise by a relative ridge and retry. */
+      /* Curvature semidefinite up to round-off on the free set: escalating relative ridge. */
       double dmax = 0.0;
-      for (int a = 0; a < nf; ++a) dmax = fmax(dmax, fabs(H[fidx[a] + n * fidx[a]]));
-      for (int a = 0; a < nf; ++a)
-        for (int b = 0; b < nf; ++b) M[a + nf * b] = H[fidx[a] + n * fidx[b]] + (a == b ? 1e-12 * dmax + 1e-300 : 0.0);
-      if (chol_inplace(M, nf, nf) != 0) return -2;
+      for (int i = 0; i < n; ++i) dmax = fmax(dmax, fabs(H[i + n * i]));
+      int ok = 0;
+      const double ridges[3] = {1e-12, 1e-9, 1e-6};
+      for (int t = 0; t < 3 && !ok; ++t) {
+        for (int a = 0; a < nf; ++a)
+          for (int b = 0; b < nf; ++b)
+            M[a + nf * b] = H[fidx[a] + n * fidx[b]] + (a == b ? ridges[t] * dmax + 1e-300 : 0.0);
+        ok = chol_inplace(M, nf, nf) == 0;
+      }
+      if (!ok) return -2;
     }
     double z[HKO_QP_MAXN];
     for (int a = 0; a < nf; ++a) z[a] = r[fidx[a]];
