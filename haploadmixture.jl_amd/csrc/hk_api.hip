@@ -22,6 +22,8 @@ struct hk_ctx {
   hipStream_t s = nullptr;
   uint32_t* gq = nullptr;
   uint32_t* gp = nullptr;
+  void* qq = nullptr;
+  void* pp = nullptr;
   int64_t* dm = nullptr;
   double* xb[4] = {nullptr, nullptr, nullptr, nullptr};  // parameter buffers [q | p | pad]
   int64_t xlen = 0;                                      // K * (I + 4 Jp)
@@ -185,6 +187,7 @@ static int run_qpass(hk_ctx* c, int mode, const double* q, const double* p, Time
   HIPC(c, launch_qside(c->K, c->prec, mode, c->g, qq, pp, c->part_q, c->s));
   if (tm) tm->end();
   if (mode == MODE_LL) HIPC(c, launch_reduce_ll(c->part_q, c->g.qchunks, c->g, c->ll_rows, c->ll_dev, c->s));
+  else if (mode == MODE_CURV) HIPC(c, launch_reduce_q_curv(c->part_q, c->g.qchunks, c->K, c->g, q, c->res_q, c->s));
   else HIPC(c, launch_reduce_q(c->part_q, c->g.qchunks, nacc(c->K, mode), c->g, c->res_q, c->s));
   return HK_OK;
 }
@@ -204,6 +207,7 @@ static int run_ppass(hk_ctx* c, int mode, const double* q, const double* p, Time
     if (tm) tm->end();
     HIPC(c, launch_reduce_p(c->part_p, c->g.pchunks, NT, c->g, c->res_p, c->s));
   }
+  if (mode == MODE_CURV) HIPC(c, launch_finish_p(c->K, c->g, p, c->res_p, c->s));
   return HK_OK;
 }
 
@@ -227,7 +231,7 @@ const char* hk_last_error(const hk_ctx* c) { return c ? c->err : g_create_err; }
 int hk_ctx_destroy(hk_ctx* c) {
   if (!c) return HK_OK;
   if (c->s) (void)hipStreamSynchronize(c->s);
-  void* ptrs[] = {c->gq, c->gp, c->dm, c->xb[0], c->xb[1], c->xb[2], c->xb[3], c->x32, c->res_q, c->res_p,
+  void* ptrs[] = {c->gq, c->gp, c->qq, c->pp, c->dm, c->xb[0], c->xb[1], c->xb[2], c->xb[3], c->x32, c->res_q, c->res_p,
                   c->part_q, c->part_p, c->ll_dev, c->ll_rows, c->U, c->V, c->zal_part, c->zal_coef, c->fails,
                   c->full_a, c->full_b};
   for (void* p : ptrs)
@@ -280,6 +284,15 @@ int hk_ctx_create(hk_ctx** out, int64_t I, int64_t J, int K, const uint8_t* bed,
   g.I8 = g.Ip / 8;
   CREATE_HIP(hipMalloc(&c->gq, sizeof(uint32_t) * g.J8 * g.Ip));
   CREATE_HIP(hipMalloc(&c->gp, sizeof(uint32_t) * g.I8 * g.Jp));
+  {
+    const int qs = 16 * ((K * (K + 1) / 2 + 15) / 16);
+    const size_t el = precision == HK_PREC_FP64 ? sizeof(double) : sizeof(float);
+    CREATE_HIP(hipMalloc(&c->qq, el * g.Ip * qs));
+    CREATE_HIP(hipMalloc(&c->pp, el * 4 * g.Jp * qs));
+  }
+  g.qq = c->qq;
+  g.pp = c->pp;
+  g.use_mfma = 1;
   g.gq = c->gq;
   g.gp = c->gp;
   // Upload the first 2J SNP columns (each bps bytes) and repack into the two nibble streams.

@@ -25,6 +25,10 @@ __device__ __forceinline__ void snp_s(unsigned c, T& s00, T& s01, T& s11) {
   s01 = (T)(hi ^ lo);
 }
 
+// Type-generic fused multiply-add (__builtin_fma is the double-precision builtin).
+__device__ __forceinline__ float hk_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double hk_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // ---------------------------------------------------------------------------------------------
 // Reciprocals.  FP32: the hardware v_rcp_f32 (1 ulp).  FP64: v_rcp_f64 seed + two Newton steps.
 // ---------------------------------------------------------------------------------------------

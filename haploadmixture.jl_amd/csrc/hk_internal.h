@@ -17,6 +17,9 @@ struct PassGeom {
   const uint32_t* gp;  // [I8][Jp]
   int qchunks, qwpc;   // Q side: number of j chunks, words per chunk
   int pchunks, pwpc;   // P side: number of i chunks, words per chunk
+  int use_mfma;        // curvature passes on the matrix cores (1) or on the VALU (0)
+  void* qq;            // [Ip][16 * ceil(K(K+1)/2 / 16)] q_k q_k' products (TP) for the P-side MFMA pass
+  void* pp;            // [4Jp][16 * ceil(K(K+1)/2 / 16)] p_k p_k' products (TP) for the Q-side MFMA pass
 };
 
 int nacc(int K, int mode);
@@ -29,6 +32,9 @@ hipError_t launch_qside(int K, int prec, int mode, const PassGeom& g, const void
 hipError_t launch_pside(int K, int prec, int mode, const PassGeom& g, const void* q, const void* p, double* out,
                         hipStream_t s);
 hipError_t launch_reduce_q(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s);
+hipError_t launch_reduce_q_curv(const double* part, int nchunk, int K, const PassGeom& g, const double* q, double* res,
+                               hipStream_t s);
+hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* res, hipStream_t s);
 hipError_t launch_reduce_p(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s);
 hipError_t launch_reduce_ll(const double* part, int nchunk, const PassGeom& g, double* rows, double* total,
                             hipStream_t s);

@@ -30,6 +30,11 @@
 //         q[:, i] wave-uniform (scalar loads); optional i chunks reduced in order.
 // Accumulators are FP64.  In MIXED mode the per-pair arithmetic is FP32 and the 8 pairs of a
 // genotype word are summed in FP32 before being added to the FP64 accumulators.
+//
+// The curvature passes accumulate XtX only: the reference's gradient follows from it exactly,
+//     sum_k' XtX_q[k,k',i] q[k',i] = sum_{j,h} v_h p_kh X_h = sum_{j,h} w_h p_kh = Xtz_q[k,i]
+// (likewise Xtz_p[:, jh] = XtX_p[:, :, jh] p[:, jh]), formed once per row by the reduce / finish
+// kernels from the FP64 parameters.  All terms are non-negative, so no cancellation enters.
 #include "hk_device.h"
 #include "hk_internal.h"
 
@@ -132,8 +137,11 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
   double acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = 0.0;
-  double llm = 1.0;  // FP64 log-likelihood: running product mantissa ...
-  int lle = 0;       // ... and binary exponent
+  // Log-likelihood: the per-pair likelihoods (>= ~1e-20) are multiplied into an FP64 mantissa that is
+  // renormalised once per genotype word (8 pairs, >= ~1e-160: no underflow) with an integer exponent;
+  // one log per lane at the end.  Rounding grows like sqrt(J) ulps instead of J logs' worth.
+  double llm = 1.0;
+  int lle = 0;
 
   for (int jw = jw0; jw < jw1; ++jw) {
     const uint32_t word = gq[(int64_t)jw * Ip + i];
@@ -157,7 +165,7 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
       for (int h = 0; h < 4; ++h) {
         TP s = (TP)0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) s = __builtin_fma(qi[k], P[h][k], s);
+        for (int k = 0; k < K; ++k) s = hk_fma(qi[k], P[h][k], s);
         X[h] = s;
       }
       TP s1_00, s1_01, s1_11, s2_00, s2_01, s2_11;
@@ -177,25 +185,21 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
         const bool het = (c1 == 2u) | (c2 == 2u);
         TP Lh = het ? (TP)0.5 * L : L;
         Lh = dm ? (TP)1 : Lh;
-        if constexpr (F32) {
-          pa[0] += __builtin_amdgcn_logf(Lh);  // log2
-        } else {
-          llm *= Lh;
-        }
+        llm *= (double)Lh;  // FP64 running product; renormalised by frexp once per genotype word
       } else {
         const TP rl = dm ? (TP)0 : (TP)2 * hk_rcp(L);
         TP w[4];
 #pragma unroll
         for (int h = 0; h < 4; ++h) w[h] = Y[h] * rl;
-        TP* dst = nullptr;
-        (void)dst;
+        if constexpr (MODE == MODE_GRAD) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          TP g = w[0] * P[0][k];
-          g = __builtin_fma(w[1], P[1][k], g);
-          g = __builtin_fma(w[2], P[2][k], g);
-          g = __builtin_fma(w[3], P[3][k], g);
-          if constexpr (F32) pa[k] += g; else acc[k] += g;
+          for (int k = 0; k < K; ++k) {
+            TP g = w[0] * P[0][k];
+            g = hk_fma(w[1], P[1][k], g);
+            g = hk_fma(w[2], P[2][k], g);
+            g = hk_fma(w[3], P[3][k], g);
+            if constexpr (F32) pa[k] += g; else acc[k] += g;
+          }
         }
         if constexpr (MODE == MODE_CURV) {
           TP v[4];
@@ -209,17 +213,17 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
               const int t = K + k * (k + 1) / 2 + k2;
               if constexpr (F32) {
                 TP s = pa[t];
-                s = __builtin_fma(u0, P[0][k2], s);
-                s = __builtin_fma(u1, P[1][k2], s);
-                s = __builtin_fma(u2, P[2][k2], s);
-                s = __builtin_fma(u3, P[3][k2], s);
+                s = hk_fma(u0, P[0][k2], s);
+                s = hk_fma(u1, P[1][k2], s);
+                s = hk_fma(u2, P[2][k2], s);
+                s = hk_fma(u3, P[3][k2], s);
                 pa[t] = s;
               } else {
                 double s = acc[t];
-                s = __builtin_fma(u0, P[0][k2], s);
-                s = __builtin_fma(u1, P[1][k2], s);
-                s = __builtin_fma(u2, P[2][k2], s);
-                s = __builtin_fma(u3, P[3][k2], s);
+                s = hk_fma(u0, P[0][k2], s);
+                s = hk_fma(u1, P[1][k2], s);
+                s = hk_fma(u2, P[2][k2], s);
+                s = hk_fma(u3, P[3][k2], s);
                 acc[t] = s;
               }
             }
@@ -227,19 +231,16 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
         }
       }
     }
-    if constexpr (F32) {
-#pragma unroll
-      for (int t = 0; t < NT; ++t) acc[t] += (double)pa[t];
-    } else if constexpr (MODE == MODE_LL) {
+    if constexpr (MODE == MODE_LL) {
       int e;
       llm = frexp(llm, &e);
       lle += e;
+    } else if constexpr (F32) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] += (double)pa[t];
     }
   }
-  if constexpr (MODE == MODE_LL) {
-    if constexpr (F32) acc[0] *= 0.69314718055994530942;  // log2 -> natural log
-    else acc[0] = log(llm) + (double)lle * 0.69314718055994530942;
-  }
+  if constexpr (MODE == MODE_LL) acc[0] = log(llm) + (double)lle * 0.69314718055994530942;
 #pragma unroll
   for (int t = 0; t < NT; ++t) part[((int64_t)chunk * NT + t) * Ip + i] = acc[t];
 }
@@ -288,7 +289,7 @@ __global__ __launch_bounds__(256) void k_pside(const uint32_t* __restrict__ gp, 
       const unsigned c1 = nib & 3u, c2 = nib >> 2;
       TP X = (TP)0;
 #pragma unroll
-      for (int k = 0; k < K; ++k) X = __builtin_fma(Q[k], Pw[k], X);
+      for (int k = 0; k < K; ++k) X = hk_fma(Q[k], Pw[k], X);
       const TP X0 = quad_bcast<0>(X), X1 = quad_bcast<1>(X), X2 = quad_bcast<2>(X), X3 = quad_bcast<3>(X);
       TP s1_00, s1_01, s1_11, s2_00, s2_01, s2_11;
       snp_s<TP>(c1, s1_00, s1_01, s1_11);
@@ -299,10 +300,12 @@ __global__ __launch_bounds__(256) void k_pside(const uint32_t* __restrict__ gp, 
       const TP L = quad_sum<TP>(X * Y);
       const bool dm = nib == HK_NIB_DM;
       const TP w = dm ? (TP)0 : (TP)2 * Y * hk_rcp(L);
+      if constexpr (MODE == MODE_GRAD) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        if constexpr (F32) pa[k] = __builtin_fma(w, Q[k], pa[k]);
-        else acc[k] = __builtin_fma(w, Q[k], acc[k]);
+        for (int k = 0; k < K; ++k) {
+          if constexpr (F32) pa[k] = hk_fma(w, Q[k], pa[k]);
+          else acc[k] = hk_fma(w, Q[k], acc[k]);
+        }
       }
       if constexpr (MODE == MODE_CURV) {
         const TP v = w * hk_rcp(X);
@@ -312,8 +315,8 @@ __global__ __launch_bounds__(256) void k_pside(const uint32_t* __restrict__ gp, 
 #pragma unroll
           for (int k2 = 0; k2 <= k; ++k2) {
             const int t = K + k * (k + 1) / 2 + k2;
-            if constexpr (F32) pa[t] = __builtin_fma(u, Q[k2], pa[t]);
-            else acc[t] = __builtin_fma(u, Q[k2], acc[t]);
+            if constexpr (F32) pa[t] = hk_fma(u, Q[k2], pa[t]);
+            else acc[t] = hk_fma(u, Q[k2], acc[t]);
           }
         }
       }
@@ -329,6 +332,293 @@ __global__ __launch_bounds__(256) void k_pside(const uint32_t* __restrict__ gp, 
 }
 
 // -----------------------------------------------------------------------------------------------
+// Curvature passes on the matrix cores
+//
+// XtX is a GEMM with the per-pair weights as one operand:
+//     XtX_p[jh, n] = sum_i  v[i, jh] * QQ[i, n]      QQ[i, n] = q[k,i] q[k',i], n = k(k+1)/2 + k'
+//     XtX_q[i, n]  = sum_jh v[i, jh] * PP[jh, n]     PP[jh, n] = p[k,jh] p[k',jh]
+// One lane computes one (i, j) pair with all four haplotype weights v_h (no redundant per-pair work);
+// v_mfma_{f32,f64}_16x16x4 takes A = v_h (16 rows x 4), B = QQ or PP (4 x 16 columns), four MFMAs per
+// step (one per haplotype) and ceil(K(K+1)/2 / 16) column tiles.
+//   A operand: lane l holds A[l % 16][l / 16];  B operand: lane l holds B[l / 16][l % 16];
+//   result: lane l, register v holds D[row][l % 16], row = 4 (l/16) + v (f32) or l/16 + 4 v (f64).
+// MIXED: FP32 products and MFMA accumulation over 64 individuals (P side) / 64 blocks (Q side), then
+// added to FP64 accumulators.  FP64: the MFMA accumulates in FP64 directly.
+// -----------------------------------------------------------------------------------------------
+
+template <typename TP>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int v) { return 4 * g + v; }
+};
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int v) { return g + 4 * v; }
+};
+
+// Per-pair gradient / curvature weights of all four haplotypes (see the header of this file).
+template <typename TP>
+__device__ __forceinline__ void pair_weights(const TP (&X)[4], unsigned nib, TP (&v)[4]) {
+  const unsigned c1 = nib & 3u, c2 = nib >> 2;
+  TP s1_00, s1_01, s1_11, s2_00, s2_01, s2_11;
+  snp_s<TP>(c1, s1_00, s1_01, s1_11);
+  snp_s<TP>(c2, s2_00, s2_01, s2_11);
+  const TP z00 = X[0] * s2_00 + X[1] * s2_01, z01 = X[0] * s2_01 + X[1] * s2_11;
+  const TP z10 = X[2] * s2_00 + X[3] * s2_01, z11 = X[2] * s2_01 + X[3] * s2_11;
+  TP Y[4];
+  Y[0] = s1_00 * z00 + s1_01 * z10;
+  Y[1] = s1_00 * z01 + s1_01 * z11;
+  Y[2] = s1_01 * z00 + s1_11 * z10;
+  Y[3] = s1_01 * z01 + s1_11 * z11;
+  const TP L = X[0] * Y[0] + X[1] * Y[1] + X[2] * Y[2] + X[3] * Y[3];
+  const TP rl = nib == HK_NIB_DM ? (TP)0 : (TP)2 * hk_rcp(L);
+#pragma unroll
+  for (int h = 0; h < 4; ++h) v[h] = Y[h] * rl * hk_rcp(X[h]);
+}
+
+// Packed outer products of parameter columns: out[c * stride + n] = x[k,c] x[k',c] for
+// n = k(k+1)/2 + k' (k' <= k), 0 for n >= K(K+1)/2; columns c >= ncol repeat column ncol-1.
+template <int K, typename TP>
+__global__ void k_make_outer(const TP* __restrict__ x, int64_t ncol, int64_t ncol_pad, int stride,
+                             TP* __restrict__ out) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncol_pad * stride) return;
+  const int64_t c = idx / stride;
+  const int n = (int)(idx - c * stride);
+  const int64_t cc = c < ncol ? c : ncol - 1;
+  TP v = (TP)0;
+  if (n < K * (K + 1) / 2) {
+    int k = 0;
+    while ((k + 1) * (k + 2) / 2 <= n) ++k;
+    const int k2 = n - k * (k + 1) / 2;
+    v = x[k + (int64_t)K * cc] * x[k2 + (int64_t)K * cc];
+  }
+  out[idx] = v;
+}
+
+// P side: a wave owns 16 blocks (A rows); each MFMA step takes 4 individuals (A columns / B rows).
+// The workgroup (64 blocks) stages q and the q q' products of 32 individuals (4 genotype words) in LDS.
+template <int K, typename TP>
+__global__ __launch_bounds__(256) void k_pside_mm(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8,
+                                                  int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
+                                                  const TP* __restrict__ qq, double* __restrict__ out) {
+  constexpr int NKK = K * (K + 1) / 2, NT = K + NKK, NTILE = (NKK + 15) / 16, QS = 16 * NTILE;
+  constexpr int WPS = 4;  // genotype words per LDS stage
+  constexpr bool F32 = sizeof(TP) == 4;
+  using M = Mfma<TP>;
+  using V4 = typename M::v4;
+  __shared__ TP qs[8 * WPS][K];
+  __shared__ TP qqs[8 * WPS][QS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int64_t jbase = ((int64_t)blockIdx.x * 4 + wv) * 16;
+  const int64_t j = jbase + m;
+  const int chunk = blockIdx.y;
+  const int iw0 = chunk * wpc, iw1 = min(I8, iw0 + wpc);
+  TP P[4][K];
+#pragma unroll
+  for (int h = 0; h < 4; ++h)
+#pragma unroll
+    for (int k = 0; k < K; ++k) P[h][k] = p[k + (int64_t)K * (4 * j + h)];
+  V4 C[4][NTILE];
+  double D[4][NTILE][4];
+#pragma unroll
+  for (int h = 0; h < 4; ++h)
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+      C[h][t] = V4{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) D[h][t][v] = 0.0;
+    }
+  int nword = 0;
+  for (int iws = iw0; iws < iw1; iws += WPS) {
+    const int nw = min(WPS, iw1 - iws);
+    __syncthreads();
+    {
+      const int64_t i0 = (int64_t)iws * 8;  // individuals i0 .. i0 + 8 WPS - 1 (clamped to I - 1)
+      for (int e = threadIdx.x; e < 8 * WPS * K; e += 256) {
+        const int r = e / K, k = e - r * K;
+        int64_t ii = i0 + r;
+        ii = ii < I ? ii : I - 1;
+        qs[r][k] = q[k + (int64_t)K * ii];
+      }
+      for (int e = threadIdx.x; e < 8 * WPS * QS; e += 256) {
+        const int r = e / QS, n = e - r * QS;
+        int64_t ii = i0 + r;
+        ii = ii < I ? ii : I - 1;
+        qqs[r][n] = qq[ii * QS + n];
+      }
+    }
+    __syncthreads();
+    for (int w = 0; w < nw; ++w) {
+      const uint32_t word = gp[(int64_t)(iws + w) * Jp + j];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int r = 8 * w + 4 * s + g;  // staged individual of this lane's A column
+        TP X[4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          TP x = (TP)0;
+#pragma unroll
+          for (int k = 0; k < K; ++k) x = hk_fma(qs[r][k], P[h][k], x);
+          X[h] = x;
+        }
+        TP v[4];
+        pair_weights<TP>(X, (word >> (4 * (4 * s + g))) & 15u, v);
+        TP b[NTILE];
+#pragma unroll
+        for (int t = 0; t < NTILE; ++t) b[t] = qqs[r][16 * t + m];
+#pragma unroll
+        for (int h = 0; h < 4; ++h)
+#pragma unroll
+          for (int t = 0; t < NTILE; ++t) C[h][t] = M::mma(v[h], b[t], C[h][t]);
+      }
+      if constexpr (F32) {
+        if (++nword == 8) {
+          nword = 0;
+#pragma unroll
+          for (int h = 0; h < 4; ++h)
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) D[h][t][v] += (double)C[h][t][v];
+              C[h][t] = V4{0, 0, 0, 0};
+            }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 4; ++h)
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double val = F32 ? D[h][t][v] + (double)C[h][t][v] : (double)C[h][t][v];
+        const int n = 16 * t + m;
+        const int64_t row = jbase + M::row(g, v);  // block of this result element
+        double* o = out + ((int64_t)chunk * 4 * Jp + 4 * row + h) * NT;
+        if (n < NKK) o[K + n] = val;
+        if (t == 0 && m < K) o[m] = 0.0;  // gradient slots are formed by the finish kernel
+      }
+}
+
+// Q side: a wave owns 2 x 16 individuals (two A row tiles); each MFMA step takes 4 blocks (A columns /
+// B rows).  The workgroup (128 individuals) stages p and the p p' products of 16 blocks (2 words) in LDS.
+template <int K, typename TP>
+__global__ __launch_bounds__(256) void k_qside_mm(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
+                                                  int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
+                                                  const TP* __restrict__ pp, double* __restrict__ part) {
+  constexpr int NKK = K * (K + 1) / 2, NT = K + NKK, NTILE = (NKK + 15) / 16, QS = 16 * NTILE;
+  constexpr int WPS = 2, R = 2;
+  constexpr bool F32 = sizeof(TP) == 4;
+  using M = Mfma<TP>;
+  using V4 = typename M::v4;
+  __shared__ TP Pl[8 * WPS][4 * K];
+  __shared__ TP PPl[8 * WPS][4][QS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int64_t ibase = ((int64_t)blockIdx.x * 4 + wv) * 16 * R;
+  const int chunk = blockIdx.y;
+  const int jw0 = chunk * wpc, jw1 = min(J8, jw0 + wpc);
+  TP qi[R][K];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    int64_t ic = ibase + 16 * r + m;
+    ic = ic < I ? ic : I - 1;
+#pragma unroll
+    for (int k = 0; k < K; ++k) qi[r][k] = q[k + (int64_t)K * ic];
+  }
+  V4 C[R][NTILE];
+  double D[R][NTILE][4];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+      C[r][t] = V4{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) D[r][t][v] = 0.0;
+    }
+  int nword = 0;
+  for (int jws = jw0; jws < jw1; jws += WPS) {
+    const int nw = min(WPS, jw1 - jws);
+    __syncthreads();
+    {
+      const TP* ps = p + (int64_t)K * 4 * 8 * jws;
+      const TP* pps = pp + (int64_t)4 * 8 * jws * QS;
+      // exactly nw words: the chunk's last stage may hold one word, and p / pp end at block Jp
+      for (int e = threadIdx.x; e < 8 * nw * 4 * K; e += 256) (&Pl[0][0])[e] = ps[e];
+      for (int e = threadIdx.x; e < 8 * nw * 4 * QS; e += 256) (&PPl[0][0][0])[e] = pps[e];
+    }
+    __syncthreads();
+    for (int w = 0; w < nw; ++w) {
+      uint32_t word[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) word[r] = gq[(int64_t)(jws + w) * Ip + ibase + 16 * r + m];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int blk = 8 * w + 4 * s + g;  // staged block of this lane's A column
+        TP b[4][NTILE];
+#pragma unroll
+        for (int h = 0; h < 4; ++h)
+#pragma unroll
+          for (int t = 0; t < NTILE; ++t) b[h][t] = PPl[blk][h][16 * t + m];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          TP X[4];
+#pragma unroll
+          for (int h = 0; h < 4; ++h) {
+            TP x = (TP)0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) x = hk_fma(qi[r][k], Pl[blk][h * K + k], x);
+            X[h] = x;
+          }
+          TP v[4];
+          pair_weights<TP>(X, (word[r] >> (4 * (4 * s + g))) & 15u, v);
+#pragma unroll
+          for (int h = 0; h < 4; ++h)
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) C[r][t] = M::mma(v[h], b[h][t], C[r][t]);
+        }
+      }
+      if constexpr (F32) {
+        if (++nword == 8) {
+          nword = 0;
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) D[r][t][v] += (double)C[r][t][v];
+              C[r][t] = V4{0, 0, 0, 0};
+            }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double val = F32 ? D[r][t][v] + (double)C[r][t][v] : (double)C[r][t][v];
+        const int n = 16 * t + m;
+        const int64_t row = ibase + 16 * r + M::row(g, v);  // individual of this result element
+        if (n < NKK) part[((int64_t)chunk * NT + K + n) * Ip + row] = val;
+      }
+}
+
+// -----------------------------------------------------------------------------------------------
 // Reductions over chunks (fixed order)
 // -----------------------------------------------------------------------------------------------
 
@@ -341,6 +631,44 @@ __global__ void k_reduce_q(const double* __restrict__ part, int nchunk, int NT, 
     double s = 0.0;
     for (int c = 0; c < nchunk; ++c) s += part[((int64_t)c * NT + t) * Ip + i];
     res[i * NT + t] = s;
+  }
+}
+
+// Curvature pass: part[c][t][Ip] (XtX slots t >= K) -> res[i][NT] with Xtz = XtX q (FP64 q).
+__global__ void k_reduce_q_curv(const double* __restrict__ part, int nchunk, int K, int64_t Ip, int64_t I,
+                                const double* __restrict__ q, double* __restrict__ res) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I) return;
+  const int NT = K + K * (K + 1) / 2;
+  double* r = res + i * NT;
+  for (int t = K; t < NT; ++t) {
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[((int64_t)c * NT + t) * Ip + i];
+    r[t] = s;
+  }
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int k2 = 0; k2 < K; ++k2) {
+      const int a = k > k2 ? k : k2, b = k > k2 ? k2 : k;
+      s += r[K + a * (a + 1) / 2 + b] * q[k2 + (int64_t)K * i];
+    }
+    r[k] = s;
+  }
+}
+
+// Curvature pass, P side: Xtz_p[:, jh] = XtX_p[:, :, jh] p[:, jh] into slots 0..K-1 of res[jh][NT].
+__global__ void k_finish_p(int K, int64_t J, const double* __restrict__ p, double* __restrict__ res) {
+  const int64_t jh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (jh >= 4 * J) return;
+  const int NT = K + K * (K + 1) / 2;
+  double* r = res + jh * NT;
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int k2 = 0; k2 < K; ++k2) {
+      const int a = k > k2 ? k : k2, b = k > k2 ? k2 : k;
+      s += r[K + a * (a + 1) / 2 + b] * p[k2 + (int64_t)K * jh];
+    }
+    r[k] = s;
   }
 }
 
@@ -411,6 +739,13 @@ static hipError_t qside_k(int mode, const PassGeom& g, const void* q, const void
     hipLaunchKernelGGL((k_qside<K, TP, MODE_LL>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part);
   else if (mode == MODE_GRAD)
     hipLaunchKernelGGL((k_qside<K, TP, MODE_GRAD>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part);
+  else if (g.use_mfma) {
+    constexpr int QS = 16 * ((K * (K + 1) / 2 + 15) / 16);
+    hipLaunchKernelGGL((k_make_outer<K, TP>), dim3((unsigned)((4 * g.Jp * QS + 255) / 256)), dim3(256), 0, s, pp,
+                       4 * g.Jp, 4 * g.Jp, QS, (TP*)g.pp);
+    hipLaunchKernelGGL((k_qside_mm<K, TP>), dim3((unsigned)(g.Ip / 128), (unsigned)g.qchunks), block, 0, s, g.gq, g.Ip,
+                       g.I, (int)g.J8, g.qwpc, qq, pp, (const TP*)g.pp, part);
+  }
   else
     hipLaunchKernelGGL((k_qside<K, TP, MODE_CURV>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part);
   return hipGetLastError();
@@ -424,7 +759,13 @@ static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void
   const TP* pp = (const TP*)p;
   if (mode == MODE_GRAD)
     hipLaunchKernelGGL((k_pside<K, TP, MODE_GRAD>), grid, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out);
-  else
+  else if (g.use_mfma) {
+    constexpr int QS = 16 * ((K * (K + 1) / 2 + 15) / 16);
+    hipLaunchKernelGGL((k_make_outer<K, TP>), dim3((unsigned)((g.Ip * QS + 255) / 256)), dim3(256), 0, s, qq, g.I,
+                       g.Ip, QS, (TP*)g.qq);
+    hipLaunchKernelGGL((k_pside_mm<K, TP>), dim3((unsigned)(g.Jp / 64), (unsigned)g.pchunks), block, 0, s, g.gp, g.Jp,
+                       g.I, (int)g.I8, g.pwpc, qq, pp, (const TP*)g.qq, out);
+  } else
     hipLaunchKernelGGL((k_pside<K, TP, MODE_CURV>), grid, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out);
   return hipGetLastError();
 }
@@ -478,6 +819,18 @@ hipError_t launch_decode(const PassGeom& g, uint8_t* codes, hipStream_t s) {
 
 hipError_t launch_reduce_q(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s) {
   hipLaunchKernelGGL(k_reduce_q, dim3((unsigned)((g.I + 255) / 256)), dim3(256), 0, s, part, nchunk, NT, g.Ip, g.I, res);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_q_curv(const double* part, int nchunk, int K, const PassGeom& g, const double* q, double* res,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_q_curv, dim3((unsigned)((g.I + 255) / 256)), dim3(256), 0, s, part, nchunk, K, g.Ip, g.I,
+                     q, res);
+  return hipGetLastError();
+}
+
+hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* res, hipStream_t s) {
+  hipLaunchKernelGGL(k_finish_p, dim3((unsigned)((4 * g.J + 255) / 256)), dim3(256), 0, s, K, g.J, p, res);
   return hipGetLastError();
 }
 

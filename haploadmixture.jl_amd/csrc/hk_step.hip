@@ -12,7 +12,7 @@
 // optional group-sum equalities; their optimum is unique.  Here each QP is solved by a primal
 // active-set method on ONE wavefront: lane r owns row r of the (masked) curvature matrix in
 // registers, pivots and columns are broadcast with v_readlane, and the two triangular solves use
-// one LDS transpose.  No global atomics; every reduction runs in a fixed order.
+// one LDS transpose.  Every reduction runs in a fixed order; the only atomic is the failure counter.
 #include "hk_device.h"
 #include "hk_internal.h"
 
@@ -137,8 +137,9 @@ __device__ int wave_qp(const double (&Hrow)[N], double g, double lo, double hi, 
       ok = true;
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        const double dkk = rl(R[k], k);
-        if (!(dkk > 0.0)) { ok = false; break; }
+        double dkk = rl(R[k], k);
+        ok = ok && (dkk > 0.0);  // no early exit: the loop must stay fully unrolled (R in registers)
+        dkk = ok ? dkk : 1.0;
         const double sq = sqrt(dkk);
         const double isq = 1.0 / sq;
         R[k] = (lane == k) ? sq : (lane > k ? R[k] * isq : R[k]);
@@ -287,10 +288,11 @@ __device__ int wave_qp(const double (&Hrow)[N], double g, double lo, double hi, 
 // update_q! row step: one wavefront per individual.
 // res_q[i][NT]: Xtz (K) then packed lower XtX (K(K+1)/2), as produced by the pass (Xtz not negated).
 // ------------------------------------------------------------------------------------------------
-template <int K>
+template <int K, bool PEN>
 __global__ __launch_bounds__(64) void k_q_step(int64_t I, const double* __restrict__ res, const double* __restrict__ q,
-                                               double* __restrict__ q_next, int penalty, double rho, double eps,
+                                               double* __restrict__ q_next, double rho, double eps,
                                                int* __restrict__ fails) {
+  constexpr bool penalty = PEN;
   constexpr int NT = K + K * (K + 1) / 2;
   __shared__ double lds[K * K > 64 ? K * K : 64];
   const int64_t i = blockIdx.x;
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(64) void k_q_step(int64_t I, const double* __restri
   const double lo = lane < K ? 0.0 - qi : 0.0, hi = lane < K ? 1.0 - qi : 0.0;
   double x = 0.0;
   int it;
-  if (penalty) it = wave_qp<K, 0>(Hrow, g, lo, hi, 0, x, lds);
+  if constexpr (PEN) it = wave_qp<K, 0>(Hrow, g, lo, hi, 0, x, lds);
   else it = wave_qp<K, 1>(Hrow, g, lo, hi, 0, x, lds);
   if (it < 0) x = 0.0;
   if (it < 0 && lane == 0) atomicAdd(fails, 1);
@@ -330,10 +332,11 @@ __global__ __launch_bounds__(64) void k_q_step(int64_t I, const double* __restri
 // ------------------------------------------------------------------------------------------------
 // update_p! row step: one wavefront per SNP block j; variable (l, k) at index l*K + k.
 // ------------------------------------------------------------------------------------------------
-template <int K>
+template <int K, bool PEN>
 __global__ __launch_bounds__(64) void k_p_step(int64_t J, const double* __restrict__ res, const double* __restrict__ p,
-                                               double* __restrict__ p_next, int penalty, double rho, double eps,
+                                               double* __restrict__ p_next, double rho, double eps,
                                                int* __restrict__ fails) {
+  constexpr bool penalty = PEN;
   constexpr int NT = K + K * (K + 1) / 2;
   constexpr int N = 4 * K;
   __shared__ double lds[N * N];
@@ -363,7 +366,7 @@ __global__ __launch_bounds__(64) void k_p_step(int64_t J, const double* __restri
   const double lo = act ? 0.0 - pv : 0.0, hi = act ? 1.0 - pv : 0.0;
   double x = 0.0;
   int it;
-  if (penalty) it = wave_qp<N, 0>(Hrow, g, lo, hi, k, x, lds);
+  if constexpr (PEN) it = wave_qp<N, 0>(Hrow, g, lo, hi, k, x, lds);
   else it = wave_qp<N, K>(Hrow, g, lo, hi, k, x, lds);
   if (it < 0) x = 0.0;
   if (it < 0 && lane == 0) atomicAdd(fails, 1);
@@ -575,8 +578,12 @@ hipError_t launch_q_step(int K, int64_t I, const double* res_q, const double* q,
                          int* fails, hipStream_t s) {
   return with_k(K, [&](auto kc) {
     constexpr int KT = decltype(kc)::value;
-    hipLaunchKernelGGL((k_q_step<KT>), dim3((unsigned)I), dim3(64), 0, s, I, res_q, q, q_next, o.penalty, o.rho,
-                       o.eps, fails);
+    if (o.penalty)
+      hipLaunchKernelGGL((k_q_step<KT, true>), dim3((unsigned)I), dim3(64), 0, s, I, res_q, q, q_next, o.rho, o.eps,
+                         fails);
+    else
+      hipLaunchKernelGGL((k_q_step<KT, false>), dim3((unsigned)I), dim3(64), 0, s, I, res_q, q, q_next, o.rho, o.eps,
+                         fails);
   });
 }
 
@@ -584,8 +591,12 @@ hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p,
                          int* fails, hipStream_t s) {
   return with_k(K, [&](auto kc) {
     constexpr int KT = decltype(kc)::value;
-    hipLaunchKernelGGL((k_p_step<KT>), dim3((unsigned)J), dim3(64), 0, s, J, res_p, p, p_next, o.penalty, o.rho,
-                       o.eps, fails);
+    if (o.penalty)
+      hipLaunchKernelGGL((k_p_step<KT, true>), dim3((unsigned)J), dim3(64), 0, s, J, res_p, p, p_next, o.rho, o.eps,
+                         fails);
+    else
+      hipLaunchKernelGGL((k_p_step<KT, false>), dim3((unsigned)J), dim3(64), 0, s, J, res_p, p, p_next, o.rho, o.eps,
+                         fails);
   });
 }
 

@@ -100,9 +100,11 @@ def test_fit_matches_oracle_fit(prec, oracle):
         assert r["qp_failures"] == 0
         tol = 1e-9 if prec == "fp64" else 1e-6
         assert abs(r["ll"] - oll) <= tol * abs(oll), (r["ll"], oll, r["iters"], oit)
-        # the log-likelihood never decreases across accepted iterations beyond round-off
+        # the log-likelihood never decreases across accepted iterations beyond round-off (MIXED: the
+        # FP32 per-pair likelihoods put ~1e-8 relative noise on each evaluation)
         tr = r["trace"]
-        assert np.all(np.diff(tr) >= -1e-9 * abs(tr[-1]))
+        noise = 1e-9 if prec == "fp64" else 1e-7
+        assert np.all(np.diff(tr) >= -noise * abs(tr[-1]))
 
 
 def test_fit_first_iterations_track_oracle(oracle):
