@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--no-fp64-ref", action="store_true", help="skip the plain FP64 GPU fit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-blocks", type=int, default=1000)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="after the timed fits, run a plain copy kernel of this size (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {a.gpus} is not supported by this build "
@@ -144,6 +146,12 @@ def main():
     breakdown["EM (ms)"] = round(sum(x["t_em_ms"] for x in runs) / a.steps, 2)
     breakdown["ZAL+proj (ms)"] = round(sum(x["t_qn_ms"] for x in runs) / a.steps, 2)
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath)).get(a.workload)
+        if t and prec == "mixed" and "pside" in name:
+            traffic = t["hbm_bytes_per_launch"]
     out = {
         "metric": "time-to-converge (s) at 1e-6 rel loglik, I=20k J=200k K=6; HBM GB/s",
         "value": round(ms_per_step / 1000.0, 4),
@@ -166,11 +174,14 @@ def main():
         "kernel_ms_per_fit": breakdown,
         "genotype_GBps": round(0.5 * I * J * (6 * r["iters"] + 15) / (ms_per_step * 1e-3) / 1e9, 1),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "frac": round(achieved / peak, 4), "traffic": traffic,
                      "kernel": name, "avg_launch_ms": round(avg_ms, 3), "launches": nl,
                      "flops_per_launch": fl,
                      "note": "VALU-bound kernel; peak = MI355X vector peak of the per-pair precision"},
     }
+    if a.calib_copy_mb:
+        out["copy_calibration"] = {"bytes": a.calib_copy_mb << 20, "reps": 3,
+                                   "GBps": round(eng.copy_bench(a.calib_copy_mb << 20, 3), 1)}
     if not a.no_fp64_ref and prec == "mixed":
         eng.close()
         log("plain FP64 GPU fit (reference precision) on the same data")

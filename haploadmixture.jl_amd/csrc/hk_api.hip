@@ -680,3 +680,29 @@ extern "C" int hk_time_pass(hk_ctx* c, int kind, int reps, double* ms) {
   if (rc == HK_OK) *ms = total / reps;
   return rc;
 }
+
+extern "C" int hk_copy_bench(hk_ctx* c, int64_t bytes, int reps, double* gbps) {
+  if (!c || bytes < 16 || reps < 1 || !gbps) return fail(c, HK_EINVAL, "invalid arguments");
+  const int64_t n16 = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  HIPC(c, hipMalloc(&a, n16 * 16));
+  hipError_t e = hipMalloc(&b, n16 * 16);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (e == hipSuccess) e = hipMemsetAsync(a, 0, n16 * 16, c->s);
+  if (e == hipSuccess) e = hipEventCreate(&t0);
+  if (e == hipSuccess) e = hipEventCreate(&t1);
+  if (e == hipSuccess) e = launch_copy16(a, b, n16, c->s);  // warm
+  if (e == hipSuccess) e = hipEventRecord(t0, c->s);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy16(a, b, n16, c->s);
+  if (e == hipSuccess) e = hipEventRecord(t1, c->s);
+  if (e == hipSuccess) e = hipEventSynchronize(t1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  HIPC(c, e);
+  *gbps = 2.0 * (double)(n16 * 16) * reps / (ms * 1e-3) / 1e9;
+  return HK_OK;
+}

@@ -63,5 +63,6 @@ hipError_t launch_zal(int64_t N, int Qn, int iter, double* U, double* V, const d
                       const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s);
 hipError_t launch_axpy_step(int64_t n, const double* a, const double* d, double f, double* out, hipStream_t s);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+hipError_t launch_copy16(const void* src, void* dst, int64_t n16, hipStream_t s);
 
 }  // namespace hk

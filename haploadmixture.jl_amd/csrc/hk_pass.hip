@@ -662,11 +662,14 @@ __global__ void k_finish_p(int K, int64_t J, const double* __restrict__ p, doubl
   if (jh >= 4 * J) return;
   const int NT = K + K * (K + 1) / 2;
   double* r = res + jh * NT;
+  double a[HK_MAXK * (HK_MAXK + 1) / 2], pv[HK_MAXK];
+  for (int t = 0; t < K * (K + 1) / 2; ++t) a[t] = r[K + t];
+  for (int k = 0; k < K; ++k) pv[k] = p[k + (int64_t)K * jh];
   for (int k = 0; k < K; ++k) {
     double s = 0.0;
     for (int k2 = 0; k2 < K; ++k2) {
-      const int a = k > k2 ? k : k2, b = k > k2 ? k2 : k;
-      s += r[K + a * (a + 1) / 2 + b] * p[k2 + (int64_t)K * jh];
+      const int hi = k > k2 ? k : k2, lo = k > k2 ? k2 : k;
+      s += a[hi * (hi + 1) / 2 + lo] * pv[k2];
     }
     r[k] = s;
   }

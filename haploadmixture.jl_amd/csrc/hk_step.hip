@@ -453,6 +453,12 @@ __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
   if (i < n) p[i] = v;
 }
 
+// Plain streaming copy (16 B per lane per iteration): HBM bandwidth and counter calibration.
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void k_axpy(int64_t n, const double* __restrict__ a, const double* __restrict__ d, double f,
                        double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -637,6 +643,11 @@ hipError_t launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy16(const void* src, void* dst, int64_t n16, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n16);
   return hipGetLastError();
 }
 

@@ -105,6 +105,7 @@ def hip():
     L.hk_get_tuning.argtypes = [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int)]
     L.hk_sync.argtypes = [_vp]
     L.hk_time_pass.argtypes = [_vp, _int, _int, ctypes.POINTER(_dbl)]
+    L.hk_copy_bench.argtypes = [_vp, _i64, _int, ctypes.POINTER(_dbl)]
     _hip = L
     return L
 

@@ -171,5 +171,11 @@ class HipAdmixData:
         self._check(self._L.hk_time_pass(self._h, int(kind), int(reps), ctypes.byref(ms)))
         return ms.value
 
+    def copy_bench(self, nbytes: int, reps: int = 3) -> float:
+        """GB/s of a plain device copy of nbytes (read + write counted)."""
+        g = ctypes.c_double()
+        self._check(self._L.hk_copy_bench(self._h, int(nbytes), int(reps), ctypes.byref(g)))
+        return g.value
+
     def sync(self):
         self._check(self._L.hk_sync(self._h))

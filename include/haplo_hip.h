@@ -139,6 +139,10 @@ int hk_sync(hk_ctx* ctx);
  * ms_per_launch receives the mean device time of the pass's main kernel. */
 int hk_time_pass(hk_ctx* ctx, int kind, int reps, double* ms_per_launch);
 
+/* Calibration: copy `bytes` device-to-device `reps` times with a plain streaming kernel (k_copy16, 16 B per
+ * lane) on the context stream; gbps = (read + write bytes) / time.  Used to scale HBM byte counters. */
+int hk_copy_bench(hk_ctx* ctx, int64_t bytes, int reps, double* gbps);
+
 #ifdef __cplusplus
 }
 #endif
