@@ -36,6 +36,8 @@ WORKLOADS = {
     "mixed_I20k_J200k_K6": dict(I=20000, J=200000, K=6, fst=0.05, miss=0.02, seed=3, prec="mixed"),
     "fp64_I5k_J50k_K4": dict(I=5000, J=50000, K=4, fst=0.05, miss=0.01, seed=2, prec="fp64"),
     "cpu_I500_J2k_K3": dict(I=500, J=2000, K=3, fst=0.10, miss=0.01, seed=1, prec="fp64"),
+    # batched model selection: K = 2..8 x 4 restarts on data simulated with K = 5
+    "select_I3k_J20k_K2-8x4": dict(I=3000, J=20000, K=5, fst=0.05, miss=0.01, seed=4, prec="mixed"),
 }
 PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}  # MI355X FP32 / FP64 vector peaks (dense)
 
@@ -80,6 +82,47 @@ def cpu_baseline(bed, I, J, K, q0, p0, gpu_iters, em_iters, j_sample):
                 t_em_sample_s=t_em, t_iter_sample_s=t_it)
 
 
+def select_bench(a, w, prec):
+    """28 fits (K = 2..8 x 4 restarts) of one dataset: lockstep batch (shared genotype, per-K autotuned
+    chunk shapes, one captured graph per fit per iteration) against the same 28 fits run one at a time."""
+    from haplo_amd.select import select_k
+
+    I, J = w["I"], w["J"]
+    log(f"generating {a.workload}: I={I} J={J} true K={w['K']}")
+    data = simulate(I, J, w["K"], fst=w["fst"], miss=w["miss"], seed=w["seed"], truth=False)
+    Ks = list(range(2, 9))
+    log("warm-up (kernel loading, graph capture)")
+    select_k(data.bed, I, J, Ks=Ks, restarts=1, precision=prec, rtol=a.rtol, n_iter=3, autotune=False)
+    log("lockstep batch")
+    b = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, lockstep=True)
+    log(f"batch wall {b.wall_s:.2f}s best K={b.best_k}")
+    sq = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, lockstep=False)
+    log(f"sequential wall {sq.wall_s:.2f}s best K={sq.best_k}")
+    lat_b = np.array([t["finished_at_s"] for t in b.table])
+    lat_s = np.array([t["seconds"] for t in sq.table])
+    out = {
+        "metric": "model selection wall time (s), K=2..8 x 4 restarts, I=3k J=20k",
+        "value": round(b.wall_s, 3), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(1000 * b.wall_s, 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-products/f64-accum" if prec == "mixed" else "f64",
+        "data": "synthetic (seeded generator, true K=5)",
+        "config": {"workload": a.workload, "I": I, "J": J, "Ks": Ks, "restarts": 4, "fits": len(b.table),
+                   "precision": prec, "rtol": a.rtol},
+        "sequential_wall_s": round(sq.wall_s, 3),
+        "speedup_batch_over_sequential": round(sq.wall_s / b.wall_s, 3),
+        "per_fit_latency_batch_s": {"p50": round(float(np.percentile(lat_b, 50)), 3),
+                                    "p95": round(float(np.percentile(lat_b, 95)), 3)},
+        "per_fit_latency_sequential_s": {"p50": round(float(np.percentile(lat_s, 50)), 3),
+                                         "p95": round(float(np.percentile(lat_s, 95)), 3)},
+        "aic_by_k": {str(K): round(b.best_by_k[K]["aic"], 1) for K in Ks},
+        "ll_by_k": {str(K): b.best_by_k[K]["ll"] for K in Ks},
+        "best_k": b.best_k,
+        "autotuned_chunks_by_k": {str(K): list(v) for K, v in b.tuning.items()},
+        "iters_by_fit": [t["iters"] for t in b.table],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +144,8 @@ def main():
         sys.exit(2)
     w = dict(WORKLOADS[a.workload])
     prec = a.prec or w["prec"]
+    if a.workload.startswith("select"):
+        return select_bench(a, w, prec)
     I, J, K = w["I"], w["J"], w["K"]
     log(f"generating {a.workload}: I={I} J={J} K={K} Fst={w['fst']} miss={w['miss']}")
     t0 = time.perf_counter()

@@ -44,6 +44,14 @@ struct hk_ctx {
   double* full_b = nullptr;
   int64_t full_len = 0;
   int qchunks_req = 0, pchunks_req = 0;
+  // fit state for the sync-free iteration: ll[0] ll_basic, ll[1] ll_qn, ll[2] ll_new, ll[3] scratch
+  double* d_ll = nullptr;
+  int* d_iter = nullptr;
+  double* h_ll = nullptr;  // pinned mirror of d_ll
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  unsigned long long graph_key = 0;
+  bool owns_geno = true;
   int64_t part_q_len = 0, part_p_len = 0;
   char err[512] = {0};
 };
@@ -180,13 +188,15 @@ enum { T_QPASS, T_PPASS, T_LL, T_QSTEP, T_PSTEP, T_EM, T_QN, T_NCAT };
 }  // namespace
 
 // Q-side pass (individual-parallel): MODE_LL -> ll_dev / ll_rows, else res_q.
-static int run_qpass(hk_ctx* c, int mode, const double* q, const double* p, Timer* tm = nullptr, int cat = 0) {
+static int run_qpass(hk_ctx* c, int mode, const double* q, const double* p, Timer* tm = nullptr, int cat = 0,
+                     double* ll_out = nullptr) {
   const void *qq, *pp;
   CHK(pass_params(c, q, p, &qq, &pp));
   if (tm) tm->begin(cat);
   HIPC(c, launch_qside(c->K, c->prec, mode, c->g, qq, pp, c->part_q, c->s));
   if (tm) tm->end();
-  if (mode == MODE_LL) HIPC(c, launch_reduce_ll(c->part_q, c->g.qchunks, c->g, c->ll_rows, c->ll_dev, c->s));
+  if (mode == MODE_LL)
+    HIPC(c, launch_reduce_ll(c->part_q, c->g.qchunks, c->g, c->ll_rows, ll_out ? ll_out : c->ll_dev, c->s));
   else if (mode == MODE_CURV) HIPC(c, launch_reduce_q_curv(c->part_q, c->g.qchunks, c->K, c->g, q, c->res_q, c->s));
   else HIPC(c, launch_reduce_q(c->part_q, c->g.qchunks, nacc(c->K, mode), c->g, c->res_q, c->s));
   return HK_OK;
@@ -231,13 +241,59 @@ const char* hk_last_error(const hk_ctx* c) { return c ? c->err : g_create_err; }
 int hk_ctx_destroy(hk_ctx* c) {
   if (!c) return HK_OK;
   if (c->s) (void)hipStreamSynchronize(c->s);
-  void* ptrs[] = {c->gq, c->gp, c->qq, c->pp, c->dm, c->xb[0], c->xb[1], c->xb[2], c->xb[3], c->x32, c->res_q, c->res_p,
-                  c->part_q, c->part_p, c->ll_dev, c->ll_rows, c->U, c->V, c->zal_part, c->zal_coef, c->fails,
-                  c->full_a, c->full_b};
-  for (void* p : ptrs)
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  void* owned[] = {c->qq, c->pp, c->xb[0], c->xb[1], c->xb[2], c->xb[3], c->x32, c->res_q, c->res_p,
+                   c->part_q, c->part_p, c->ll_dev, c->ll_rows, c->U, c->V, c->zal_part, c->zal_coef, c->fails,
+                   c->full_a, c->full_b, c->d_ll, c->d_iter};
+  for (void* p : owned)
     if (p) (void)hipFree(p);
+  if (c->owns_geno) {
+    void* geno[] = {c->gq, c->gp, c->dm};
+    for (void* p : geno)
+      if (p) (void)hipFree(p);
+  }
+  if (c->h_ll) (void)hipHostFree(c->h_ll);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
+  return HK_OK;
+}
+
+// Per-fit device buffers (everything except the genotype streams and dm).
+static int init_fit_buffers(hk_ctx* c) {
+  const int K = c->K;
+  const int64_t I = c->I;
+  PassGeom& g = c->g;
+  {
+    const int qs = 16 * ((K * (K + 1) / 2 + 15) / 16);
+    const size_t el = c->prec == HK_PREC_FP64 ? sizeof(double) : sizeof(float);
+    HIPC(c, hipMalloc(&c->qq, el * g.Ip * qs));
+    HIPC(c, hipMalloc(&c->pp, el * 4 * g.Jp * qs));
+  }
+  g.qq = c->qq;
+  g.pp = c->pp;
+  g.use_mfma = 1;
+  c->xlen = (int64_t)K * (I + 4 * g.Jp);
+  for (int b = 0; b < 4; ++b) {
+    HIPC(c, hipMalloc(&c->xb[b], sizeof(double) * c->xlen));
+    HIPC(c, launch_fill(c->xb[b], (int64_t)K * I, 1.0 / K, c->s));
+    HIPC(c, launch_fill(c->xb[b] + (int64_t)K * I, (int64_t)K * 4 * g.Jp, 0.25, c->s));
+  }
+  if (c->prec == HK_PREC_MIXED) HIPC(c, hipMalloc(&c->x32, sizeof(float) * c->xlen));
+  const int NT = ntq(K);
+  HIPC(c, hipMalloc(&c->res_q, sizeof(double) * I * NT));
+  HIPC(c, hipMalloc(&c->res_p, sizeof(double) * 4 * g.Jp * NT));
+  HIPC(c, hipMalloc(&c->ll_dev, sizeof(double) * 4));
+  HIPC(c, hipMalloc(&c->ll_rows, sizeof(double) * I));
+  HIPC(c, hipMalloc(&c->fails, sizeof(int)));
+  HIPC(c, hipMalloc(&c->d_ll, sizeof(double) * 8));
+  HIPC(c, hipMalloc(&c->d_iter, sizeof(int)));
+  HIPC(c, hipHostMalloc((void**)&c->h_ll, sizeof(double) * 8, hipHostMallocDefault));
+  HIPC(c, hipMalloc(&c->zal_part, sizeof(double) * 512 * 64));
+  HIPC(c, hipMalloc(&c->zal_coef, sizeof(double) * 8));
+  plan_chunks(c);
+  CHK(alloc_parts(c));
+  HIPC(c, hipStreamSynchronize(c->s));
   return HK_OK;
 }
 
@@ -258,23 +314,9 @@ int hk_ctx_create(hk_ctx** out, int64_t I, int64_t J, int K, const uint8_t* bed,
   c->J = J;
   c->K = K;
   c->prec = precision;
+  int rc = HK_OK;
   hipError_t e = hipGetDevice(&c->dev);
-  if (e != hipSuccess) {
-    fail(nullptr, HK_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
-    delete c;
-    return HK_EHIP;
-  }
-#define CREATE_HIP(call)                                                                   \
-  do {                                                                                     \
-    hipError_t e_ = (call);                                                                \
-    if (e_ != hipSuccess) {                                                                \
-      fail(nullptr, e_ == hipErrorOutOfMemory ? HK_ENOMEM : HK_EHIP, "%s failed: %s", #call, \
-           hipGetErrorString(e_));                                                         \
-      hk_ctx_destroy(c);                                                                   \
-      return e_ == hipErrorOutOfMemory ? HK_ENOMEM : HK_EHIP;                              \
-    }                                                                                      \
-  } while (0)
-  CREATE_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
   PassGeom& g = c->g;
   g.I = I;
   g.J = J;
@@ -282,61 +324,76 @@ int hk_ctx_create(hk_ctx** out, int64_t I, int64_t J, int K, const uint8_t* bed,
   g.Jp = round_up(J, 64);
   g.J8 = g.Jp / 8;
   g.I8 = g.Ip / 8;
-  CREATE_HIP(hipMalloc(&c->gq, sizeof(uint32_t) * g.J8 * g.Ip));
-  CREATE_HIP(hipMalloc(&c->gp, sizeof(uint32_t) * g.I8 * g.Jp));
-  {
-    const int qs = 16 * ((K * (K + 1) / 2 + 15) / 16);
-    const size_t el = precision == HK_PREC_FP64 ? sizeof(double) : sizeof(float);
-    CREATE_HIP(hipMalloc(&c->qq, el * g.Ip * qs));
-    CREATE_HIP(hipMalloc(&c->pp, el * 4 * g.Jp * qs));
-  }
-  g.qq = c->qq;
-  g.pp = c->pp;
-  g.use_mfma = 1;
+  if (e == hipSuccess) e = hipMalloc(&c->gq, sizeof(uint32_t) * g.J8 * g.Ip);
+  if (e == hipSuccess) e = hipMalloc(&c->gp, sizeof(uint32_t) * g.I8 * g.Jp);
   g.gq = c->gq;
   g.gp = c->gp;
   // Upload the first 2J SNP columns (each bps bytes) and repack into the two nibble streams.
-  {
+  if (e == hipSuccess) {
     uint8_t* raw = nullptr;
     const int64_t nbytes = 2 * J * bps;
-    CREATE_HIP(hipMalloc(&raw, (size_t)nbytes));
-    e = hipMemcpyAsync(raw, bed, (size_t)nbytes, hipMemcpyHostToDevice, c->s);
+    e = hipMalloc(&raw, (size_t)nbytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(raw, bed, (size_t)nbytes, hipMemcpyHostToDevice, c->s);
     if (e == hipSuccess) e = launch_repack(raw, bps, g, c->s);
     if (e == hipSuccess) e = hipStreamSynchronize(c->s);
-    (void)hipFree(raw);
-    CREATE_HIP(e);
+    if (raw) (void)hipFree(raw);
   }
-  CREATE_HIP(hipMalloc(&c->dm, sizeof(int64_t) * I));
-  CREATE_HIP(launch_count_dm(g, c->dm, c->s));
-  c->xlen = (int64_t)K * (I + 4 * g.Jp);
-  for (int b = 0; b < 4; ++b) {
-    CREATE_HIP(hipMalloc(&c->xb[b], sizeof(double) * c->xlen));
-    CREATE_HIP(launch_fill(c->xb[b], (int64_t)K * I, 1.0 / K, c->s));
-    CREATE_HIP(launch_fill(c->xb[b] + (int64_t)K * I, (int64_t)K * 4 * g.Jp, 0.25, c->s));
-  }
-  if (precision == HK_PREC_MIXED) CREATE_HIP(hipMalloc(&c->x32, sizeof(float) * c->xlen));
-  const int NT = ntq(K);
-  CREATE_HIP(hipMalloc(&c->res_q, sizeof(double) * I * NT));
-  CREATE_HIP(hipMalloc(&c->res_p, sizeof(double) * 4 * g.Jp * NT));
-  CREATE_HIP(hipMalloc(&c->ll_dev, sizeof(double) * 4));
-  CREATE_HIP(hipMalloc(&c->ll_rows, sizeof(double) * I));
-  CREATE_HIP(hipMalloc(&c->fails, sizeof(int)));
-  CREATE_HIP(hipMalloc(&c->zal_part, sizeof(double) * 512 * 64));
-  CREATE_HIP(hipMalloc(&c->zal_coef, sizeof(double) * 8));
-  plan_chunks(c);
-  if (alloc_parts(c) != HK_OK) {
+  if (e == hipSuccess) e = hipMalloc(&c->dm, sizeof(int64_t) * I);
+  if (e == hipSuccess) e = launch_count_dm(g, c->dm, c->s);
+  if (e != hipSuccess) {
+    rc = e == hipErrorOutOfMemory ? HK_ENOMEM : HK_EHIP;
+    fail(nullptr, rc, "context setup failed: %s", hipGetErrorString(e));
+  } else if ((rc = init_fit_buffers(c)) != HK_OK) {
     snprintf(g_create_err, sizeof g_create_err, "%s", c->err);
-    hk_ctx_destroy(c);
-    return HK_ENOMEM;
   }
-  CREATE_HIP(hipStreamSynchronize(c->s));
-#undef CREATE_HIP
+  if (rc != HK_OK) {
+    hk_ctx_destroy(c);
+    return rc;
+  }
   *out = c;
   return HK_OK;
 }
 
+int hk_ctx_create_shared(hk_ctx** out, hk_ctx* base, int K, int precision) {
+  if (!out || !base) return fail(nullptr, HK_EINVAL, "null pointer");
+  *out = nullptr;
+  if (K < 1 || K > HK_MAX_K || (precision != HK_PREC_FP64 && precision != HK_PREC_MIXED))
+    return fail(nullptr, HK_EINVAL, "invalid K=%d or precision=%d", K, precision);
+  hipError_t e = hipSetDevice(base->dev);
+  if (e != hipSuccess) return fail(nullptr, HK_EHIP, "hipSetDevice: %s", hipGetErrorString(e));
+  hk_ctx* c = new hk_ctx();
+  c->I = base->I;
+  c->J = base->J;
+  c->K = K;
+  c->prec = precision;
+  c->dev = base->dev;
+  c->owns_geno = false;
+  c->g = base->g;
+  c->gq = base->gq;
+  c->gp = base->gp;
+  c->dm = base->dm;
+  c->qchunks_req = c->pchunks_req = 0;
+  int rc = HK_OK;
+  e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    rc = HK_EHIP;
+    fail(nullptr, rc, "hipStreamCreate: %s", hipGetErrorString(e));
+  } else if ((rc = init_fit_buffers(c)) != HK_OK) {
+    snprintf(g_create_err, sizeof g_create_err, "%s", c->err);
+  }
+  if (rc != HK_OK) {
+    hk_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return HK_OK;
+}
+
+static void drop_graph(hk_ctx* c);
+
 int hk_set_tuning(hk_ctx* c, int qc, int pc) {
   if (!c || qc < 0 || pc < 0) return fail(c, HK_EINVAL, "invalid tuning");
+  drop_graph(c);
   c->qchunks_req = qc;
   c->pchunks_req = pc;
   plan_chunks(c);
@@ -455,7 +512,7 @@ int hk_q_step(hk_ctx* c, int penalty, double rho, double* q_next) {
   double* xn = c->xb[1];
   CHK(run_qpass(c, MODE_CURV, qof(c, x), pof(c, x)));
   StepOpts so{penalty, rho, 1e-5};
-  HIPC(c, hipMemsetAsync(c->fails, 0, sizeof(int), c->s));
+  HIPC(c, launch_set_int(c->fails, 0, c->s));
   HIPC(c, launch_q_step(c->K, c->I, c->res_q, qof(c, x), qof(c, xn), so, c->fails, c->s));
   int nf = 0;
   HIPC(c, hipMemcpyAsync(&nf, c->fails, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -476,7 +533,7 @@ int hk_p_step(hk_ctx* c, const double* q_pass, int penalty, double rho, double* 
   }
   CHK(run_ppass(c, MODE_CURV, qp, pof(c, x)));
   StepOpts so{penalty, rho, 1e-5};
-  HIPC(c, hipMemsetAsync(c->fails, 0, sizeof(int), c->s));
+  HIPC(c, launch_set_int(c->fails, 0, c->s));
   HIPC(c, launch_p_step(c->K, c->J, c->res_p, pof(c, x), pof(c, xn), so, c->fails, c->s));
   int nf = 0;
   HIPC(c, hipMemcpyAsync(&nf, c->fails, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -500,6 +557,7 @@ void hk_fit_opts_default(hk_fit_opts* o) {
   o->eps = 1e-5;
   o->verbose = 0;
   o->profile = 0;
+  o->use_graph = 1;
 }
 
 }  // extern "C"
@@ -508,145 +566,358 @@ void hk_fit_opts_default(hk_fit_opts* o) {
 // Device-resident fit
 // ---------------------------------------------------------------------------------------------------
 
-static int fit_loglik(hk_ctx* c, Timer& tm, double* x, double* out) {
-  CHK(run_qpass(c, MODE_LL, qof(c, x), pof(c, x), &tm, T_LL));
-  HIPC(c, hipMemcpyAsync(out, c->ll_dev, sizeof(double), hipMemcpyDeviceToHost, c->s));
+// ---------------------------------------------------------------------------------------------------
+// One QN iteration of admixture_qn! (algorithms_outer.jl:65-171) enqueued without host synchronisation:
+//   F(x) -> x_next, F(F(x)) -> x_next2, ll_basic, ZAL -> x_tmp, projections, ll_qn, accept into x.
+// Fixed buffers (x = xb[0], x_next = xb[1], x_next2 = xb[2], x_tmp = xb[3]) and device-resident
+// iteration counter / log-likelihoods make the sequence replayable from a captured hipGraph.
+// ---------------------------------------------------------------------------------------------------
+static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm) {
+  const int K = c->K;
+  const int64_t I = c->I, J = c->J;
+  const int64_t N = (int64_t)K * (I + 4 * J), nq = (int64_t)K * I, np = (int64_t)K * 4 * J;
+  double* x = c->xb[0];
+  double* xn = c->xb[1];
+  double* xn2 = c->xb[2];
+  double* xt = c->xb[3];
+  const StepOpts qso{o->q_penalty, o->rho, o->eps};
+  const StepOpts pso{o->p_penalty, o->rho, o->eps};
+  auto qstep = [&](const double* q, const double* p, double* q_out) -> int {
+    CHK(run_qpass(c, MODE_CURV, q, p, tm, T_QPASS));
+    if (tm) tm->begin(T_QSTEP);
+    HIPC(c, launch_q_step(K, I, c->res_q, q, q_out, qso, c->fails, c->s));
+    if (tm) tm->end();
+    return HK_OK;
+  };
+  auto pstep = [&](const double* q, const double* p, double* p_out) -> int {
+    CHK(run_ppass(c, MODE_CURV, q, p, tm, T_PPASS));
+    if (tm) tm->begin(T_PSTEP);
+    HIPC(c, launch_p_step(K, J, c->res_p, p, p_out, pso, c->fails, c->s));
+    if (tm) tm->end();
+    return HK_OK;
+  };
+  // F(x): q_next from (q, p), p_next from (q_next, p)  (update_q!, update_p!; Gauss-Seidel order)
+  if (!o->fix_q) CHK(qstep(qof(c, x), pof(c, x), qof(c, xn)));
+  else HIPC(c, launch_copy_f64(nq, qof(c, x), qof(c, xn), c->s));
+  if (!o->fix_p) CHK(pstep(qof(c, xn), pof(c, x), pof(c, xn)));
+  else HIPC(c, launch_copy_f64(np, pof(c, x), pof(c, xn), c->s));
+  // F(F(x)): q_next2 from (q_next, p_next), p_next2 from (q_next2, p_next)
+  if (!o->fix_q) CHK(qstep(qof(c, xn), pof(c, xn), qof(c, xn2)));
+  else HIPC(c, launch_copy_f64(nq, qof(c, x), qof(c, xn2), c->s));
+  if (!o->fix_p) CHK(pstep(qof(c, xn2), pof(c, xn), pof(c, xn2)));
+  else HIPC(c, launch_copy_f64(np, pof(c, x), pof(c, xn2), c->s));
+  CHK(run_qpass(c, MODE_LL, qof(c, xn2), pof(c, xn2), tm, T_LL, c->d_ll + 0));
+  if (tm) tm->begin(T_QN);
+  HIPC(c, launch_zal(N, o->Qn, c->d_iter, c->U, c->V, x, xn, xn2, xt, c->zal_part, c->zal_coef, c->s));
+  HIPC(c, launch_project_p(K, J, pof(c, xt), o->eps, c->s));
+  HIPC(c, launch_project_q(K, I, qof(c, xt), o->eps, c->s));
+  if (tm) tm->end();
+  CHK(run_qpass(c, MODE_LL, qof(c, xt), pof(c, xt), tm, T_LL, c->d_ll + 1));
+  HIPC(c, launch_accept(N, c->d_ll, xn2, xt, x, c->d_iter, c->s));
+  return HK_OK;
+}
+
+// Options that change the enqueued sequence (graph cache key).
+static unsigned long long graph_key(const hk_ctx* c, const hk_fit_opts* o) {
+  unsigned long long k = 1469598103934665603ull;
+  auto mix = [&](unsigned long long v) { k = (k ^ v) * 1099511628211ull; };
+  mix((unsigned)o->fix_q); mix((unsigned)o->fix_p); mix((unsigned)o->q_penalty); mix((unsigned)o->p_penalty);
+  mix((unsigned)o->Qn); mix(*(const unsigned long long*)&o->rho); mix(*(const unsigned long long*)&o->eps);
+  mix((unsigned)c->g.qchunks); mix((unsigned)c->g.pchunks);
+  return k;
+}
+
+static void drop_graph(hk_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_key = 0;
+}
+
+// Capture one iteration into a graph (once per option set) and launch it; eager enqueue otherwise.
+static int launch_iteration(hk_ctx* c, const hk_fit_opts* o, bool use_graph) {
+  if (!use_graph) return enqueue_iteration(c, o, nullptr);
+  const unsigned long long key = graph_key(c, o);
+  if (!c->graph_exec || c->graph_key != key) {
+    drop_graph(c);
+    HIPC(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_iteration(c, o, nullptr);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(c->s, &gr);
+    if (rc != HK_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    HIPC(c, e);
+    c->graph = gr;
+    HIPC(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    c->graph_key = key;
+  }
+  HIPC(c, hipGraphLaunch(c->graph_exec, c->s));
+  return HK_OK;
+}
+
+static int ensure_fit_buffers(hk_ctx* c, int Qn) {
+  const int64_t N = (int64_t)c->K * (c->I + 4 * c->J);
+  if (c->Qn_alloc < Qn) {
+    drop_graph(c);
+    if (c->U) HIPC(c, hipFree(c->U));
+    if (c->V) HIPC(c, hipFree(c->V));
+    c->U = c->V = nullptr;
+    HIPC(c, hipMalloc(&c->U, sizeof(double) * N * Qn));
+    HIPC(c, hipMalloc(&c->V, sizeof(double) * N * Qn));
+    c->Qn_alloc = Qn;
+  }
+  return HK_OK;
+}
+
+static int read_ll(hk_ctx* c, int n) {
+  HIPC(c, hipMemcpyAsync(c->h_ll, c->d_ll, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
   HIPC(c, hipStreamSynchronize(c->s));
   return HK_OK;
 }
 
-// F-map half sweeps (update_q! / update_p!, src/algorithms_inner.jl:97-355).
-static int fit_qstep(hk_ctx* c, Timer& tm, const double* q, const double* p, double* q_out, const StepOpts& so) {
-  CHK(run_qpass(c, MODE_CURV, q, p, &tm, T_QPASS));
-  tm.begin(T_QSTEP);
-  HIPC(c, launch_q_step(c->K, c->I, c->res_q, q, q_out, so, c->fails, c->s));
-  tm.end();
+// init_em! pass (algorithms_outer.jl:4-16): EM update into x, log-likelihood -> d_ll[3], projections.
+static int enqueue_em_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm) {
+  const int K = c->K;
+  double* x = c->xb[0];
+  double* xn = c->xb[1];
+  const int64_t N = (int64_t)K * (c->I + 4 * c->J);
+  if (tm) tm->begin(T_EM);
+  CHK(run_qpass(c, MODE_GRAD, qof(c, x), pof(c, x)));
+  HIPC(c, launch_em_q(K, c->I, c->J, c->res_q, K, c->dm, qof(c, x), qof(c, xn), o->eps, c->s));
+  CHK(run_ppass(c, MODE_GRAD, qof(c, x), pof(c, x)));
+  HIPC(c, launch_em_p(K, c->J, c->res_p, K, pof(c, x), pof(c, xn), o->eps, c->s));
+  if (tm) tm->end();
+  HIPC(c, launch_copy_f64(N, xn, x, c->s));  // d.p .= d.p_next; d.q .= d.q_next
+  CHK(run_qpass(c, MODE_LL, qof(c, x), pof(c, x), tm, T_LL, c->d_ll + 3));
+  HIPC(c, launch_project_p(K, c->J, pof(c, x), o->eps, c->s));
+  HIPC(c, launch_project_q(K, c->I, qof(c, x), o->eps, c->s));
   return HK_OK;
 }
 
-static int fit_pstep(hk_ctx* c, Timer& tm, const double* q, const double* p, double* p_out, const StepOpts& so) {
-  CHK(run_ppass(c, MODE_CURV, q, p, &tm, T_PPASS));
-  tm.begin(T_PSTEP);
-  HIPC(c, launch_p_step(c->K, c->J, c->res_p, p, p_out, so, c->fails, c->s));
-  tm.end();
+namespace {
+// Per-fit host state of the QN loop.
+struct FitRun {
+  hk_ctx* c;
+  const hk_fit_opts* o;
+  double* ll_trace;
+  double* t_trace;
+  hk_fit_result* res;
+  Timer tm;
+  double ll_new = NAN;
+  int iter = 0;
+  bool active = true;
+  std::chrono::steady_clock::time_point t0;
+};
+}  // namespace
+
+static int fit_prepare(FitRun& f, double ll_start) {
+  hk_ctx* c = f.c;
+  const hk_fit_opts* o = f.o;
+  if (o->Qn < 1 || o->Qn > 4 || o->n_iter < 0 || o->em_iters < 0)
+    return fail(c, HK_EINVAL, "invalid fit options (Qn in 1..4)");
+  memset(f.res, 0, sizeof(*f.res));
+  f.t0 = std::chrono::steady_clock::now();
+  f.tm.on = o->profile != 0;
+  f.tm.s = c->s;
+  CHK(ensure_fit_buffers(c, o->Qn));
+  HIPC(c, launch_set_int(c->fails, 0, c->s));
+  HIPC(c, launch_set_int(c->d_iter, 1, c->s));
+  return HK_OK;
+}
+
+static int fit_em(FitRun& f, double ll_start) {
+  hk_ctx* c = f.c;
+  const hk_fit_opts* o = f.o;
+  double ll = NAN;
+  for (int it = 0; it < o->em_iters; ++it) {
+    CHK(enqueue_em_iteration(c, o, f.tm.on ? &f.tm : nullptr));
+    CHK(read_ll(c, 4));
+    ll = c->h_ll[3];
+    if (o->verbose) fprintf(stderr, "EM iter %d, ll: %.17g\n", it + 1, ll);
+  }
+  if (o->em_iters == 0) ll = ll_start;
+  if (isnan(ll)) {
+    CHK(run_qpass(c, MODE_LL, qof(c, c->xb[0]), pof(c, c->xb[0]), f.tm.on ? &f.tm : nullptr, T_LL, c->d_ll + 3));
+    CHK(read_ll(c, 4));
+    ll = c->h_ll[3];
+  }
+  f.ll_new = ll;
+  f.res->ll_init = ll;
+  if (o->verbose) fprintf(stderr, "initial ll: %.17g\n", ll);
+  if (o->n_iter == 0) f.active = false;
+  return HK_OK;
+}
+
+// Host side of one iteration after its work completed: trace, accept, stopping rule.
+static int fit_after_iteration(FitRun& f) {
+  hk_ctx* c = f.c;
+  const hk_fit_opts* o = f.o;
+  const double ll_prev = f.ll_new;
+  const double ll_basic = c->h_ll[0], ll_qn = c->h_ll[1];
+  f.ll_new = c->h_ll[2];
+  ++f.iter;
+  if (f.ll_trace) f.ll_trace[f.iter - 1] = f.ll_new;
+  if (f.t_trace)
+    f.t_trace[f.iter - 1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - f.t0).count();
+  const double reldiff = fabs((f.ll_new - ll_prev) / ll_prev);
+  if (o->verbose)
+    fprintf(stderr, "Iteration %d: ll=%.17g, reldiff = %.6g, ll_basic=%.17g, ll_qn=%.17g\n", f.iter, f.ll_new, reldiff,
+            ll_basic, ll_qn);
+  if (!isfinite(f.ll_new)) return fail(c, HK_ENAN, "log-likelihood became non-finite at iteration %d", f.iter);
+  if (reldiff < o->rtol) {
+    f.res->converged = 1;
+    f.active = false;
+  } else if (f.iter >= o->n_iter) {
+    f.active = false;
+  }
+  return HK_OK;
+}
+
+static int fit_finish(FitRun& f) {
+  hk_ctx* c = f.c;
+  int nf = 0;
+  HIPC(c, hipMemcpyAsync(&nf, c->fails, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  HIPC(c, hipStreamSynchronize(c->s));
+  hk_fit_result* r = f.res;
+  r->ll = f.ll_new;
+  r->iters = f.iter;
+  r->qp_failures = nf;
+  if (f.tm.on) {
+    double t[T_NCAT] = {0};
+    int n[T_NCAT] = {0};
+    f.tm.collect(t, n);
+    r->t_qpass_ms = t[T_QPASS]; r->n_qpass = n[T_QPASS];
+    r->t_ppass_ms = t[T_PPASS]; r->n_ppass = n[T_PPASS];
+    r->t_ll_ms = t[T_LL]; r->n_ll = n[T_LL];
+    r->t_qstep_ms = t[T_QSTEP]; r->n_qstep = n[T_QSTEP];
+    r->t_pstep_ms = t[T_PSTEP]; r->n_pstep = n[T_PSTEP];
+    r->t_em_ms = t[T_EM]; r->n_em = n[T_EM];
+    r->t_qn_ms = t[T_QN]; r->n_qn = n[T_QN];
+  }
+  r->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - f.t0).count();
   return HK_OK;
 }
 
 extern "C" int hk_fit_ex(hk_ctx* c, const hk_fit_opts* o, double ll_start, double* ll_trace, double* t_trace,
                          hk_fit_result* res) {
   if (!c || !o || !res) return fail(c, HK_EINVAL, "null pointer");
-  if (o->Qn < 1 || o->Qn > 4 || o->n_iter < 0 || o->em_iters < 0)
-    return fail(c, HK_EINVAL, "invalid fit options (Qn in 1..4)");
-  const auto t0 = std::chrono::steady_clock::now();
-  memset(res, 0, sizeof(*res));
-  const int K = c->K;
-  const int64_t I = c->I, J = c->J;
-  const int64_t N = (int64_t)K * (I + 4 * J);
-  if (c->Qn_alloc < o->Qn) {
-    if (c->U) HIPC(c, hipFree(c->U));
-    if (c->V) HIPC(c, hipFree(c->V));
-    c->U = c->V = nullptr;
-    HIPC(c, hipMalloc(&c->U, sizeof(double) * N * o->Qn));
-    HIPC(c, hipMalloc(&c->V, sizeof(double) * N * o->Qn));
-    c->Qn_alloc = o->Qn;
+  FitRun f;
+  f.c = c;
+  f.o = o;
+  f.ll_trace = ll_trace;
+  f.t_trace = t_trace;
+  f.res = res;
+  CHK(fit_prepare(f, ll_start));
+  CHK(fit_em(f, ll_start));
+  const bool use_graph = o->use_graph && !o->profile;  // per-kernel events need the eager path
+  while (f.active) {
+    if (use_graph) CHK(launch_iteration(c, o, true));
+    else CHK(enqueue_iteration(c, o, f.tm.on ? &f.tm : nullptr));
+    CHK(read_ll(c, 3));
+    CHK(fit_after_iteration(f));
   }
-  Timer tm;
-  tm.on = o->profile != 0;
-  tm.s = c->s;
-  HIPC(c, hipMemsetAsync(c->fails, 0, sizeof(int), c->s));
-  double*& x = c->xb[0];
-  double*& xn = c->xb[1];
-  double*& xn2 = c->xb[2];
-  double*& xt = c->xb[3];
-  const StepOpts qso{o->q_penalty, o->rho, o->eps};
-  const StepOpts pso{o->p_penalty, o->rho, o->eps};
-  const int NTg = K;  // gradient-only accumulator count
-  double ll_new = NAN;
-  // init_em! (src/algorithms_outer.jl:1-22); driver.jl:182 runs it without fix_q/fix_p.
-  for (int it = 0; it < o->em_iters; ++it) {
-    tm.begin(T_EM);
-    CHK(run_qpass(c, MODE_GRAD, qof(c, x), pof(c, x)));
-    HIPC(c, launch_em_q(K, I, J, c->res_q, NTg, c->dm, qof(c, x), qof(c, xn), o->eps, c->s));
-    CHK(run_ppass(c, MODE_GRAD, qof(c, x), pof(c, x)));
-    HIPC(c, launch_em_p(K, J, c->res_p, NTg, pof(c, x), pof(c, xn), o->eps, c->s));
-    tm.end();
-    std::swap(x, xn);  // d.p .= d.p_next; d.q .= d.q_next
-    CHK(fit_loglik(c, tm, x, &ll_new));
-    HIPC(c, launch_project_p(K, J, pof(c, x), o->eps, c->s));
-    HIPC(c, launch_project_q(K, I, qof(c, x), o->eps, c->s));
-    if (o->verbose) fprintf(stderr, "EM iter %d, ll: %.17g\n", it + 1, ll_new);
-  }
-  if (o->em_iters == 0) ll_new = ll_start;
-  if (isnan(ll_new)) CHK(fit_loglik(c, tm, x, &ll_new));
-  res->ll_init = ll_new;
-  if (o->verbose) fprintf(stderr, "initial ll: %.17g\n", ll_new);
-  const int64_t nq = (int64_t)K * I, np = (int64_t)K * 4 * J;
-  int iters = 0, conv = 0;
-  for (int i = 1; i <= o->n_iter; ++i) {
-    iters = i;
-    const double ll_prev = ll_new;
-    // F(x): q_next from (q, p); p_next from (q_next, p)
-    if (!o->fix_q) CHK(fit_qstep(c, tm, qof(c, x), pof(c, x), qof(c, xn), qso));
-    else HIPC(c, hipMemcpyAsync(qof(c, xn), qof(c, x), sizeof(double) * nq, hipMemcpyDeviceToDevice, c->s));
-    if (!o->fix_p) CHK(fit_pstep(c, tm, qof(c, xn), pof(c, x), pof(c, xn), pso));
-    else HIPC(c, hipMemcpyAsync(pof(c, xn), pof(c, x), sizeof(double) * np, hipMemcpyDeviceToDevice, c->s));
-    // F(F(x)): q_next2 from (q_next, p_next); p_next2 from (q_next2, p_next)
-    if (!o->fix_q) CHK(fit_qstep(c, tm, qof(c, xn), pof(c, xn), qof(c, xn2), qso));
-    else HIPC(c, hipMemcpyAsync(qof(c, xn2), qof(c, x), sizeof(double) * nq, hipMemcpyDeviceToDevice, c->s));
-    if (!o->fix_p) CHK(fit_pstep(c, tm, qof(c, xn2), pof(c, xn), pof(c, xn2), pso));
-    else HIPC(c, hipMemcpyAsync(pof(c, xn2), pof(c, x), sizeof(double) * np, hipMemcpyDeviceToDevice, c->s));
-    double ll_basic, ll_qn;
-    CHK(fit_loglik(c, tm, xn2, &ll_basic));
-    tm.begin(T_QN);
-    HIPC(c, launch_zal(N, o->Qn, i, c->U, c->V, x, xn, xn2, xt, c->zal_part, c->zal_coef, c->s));
-    HIPC(c, launch_project_p(K, J, pof(c, xt), o->eps, c->s));
-    HIPC(c, launch_project_q(K, I, qof(c, xt), o->eps, c->s));
-    tm.end();
-    CHK(fit_loglik(c, tm, xt, &ll_qn));
-    if (ll_basic < ll_qn) {
-      std::swap(x, xt);
-      ll_new = ll_qn;
-    } else {
-      std::swap(x, xn2);
-      ll_new = ll_basic;
-    }
-    if (ll_trace) ll_trace[i - 1] = ll_new;
-    if (t_trace) t_trace[i - 1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    const double reldiff = fabs((ll_new - ll_prev) / ll_prev);
-    if (o->verbose)
-      fprintf(stderr, "Iteration %d: ll=%.17g, reldiff = %.6g, ll_basic=%.17g, ll_qn=%.17g\n", i, ll_new, reldiff,
-              ll_basic, ll_qn);
-    if (!isfinite(ll_new)) return fail(c, HK_ENAN, "log-likelihood became non-finite at iteration %d", i);
-    if (reldiff < o->rtol) {
-      conv = 1;
-      break;
-    }
-  }
-  int nf = 0;
-  HIPC(c, hipMemcpyAsync(&nf, c->fails, sizeof(int), hipMemcpyDeviceToHost, c->s));
-  HIPC(c, hipStreamSynchronize(c->s));
-  res->ll = ll_new;
-  res->iters = iters;
-  res->converged = conv;
-  res->qp_failures = nf;
-  if (tm.on) {
-    double t[T_NCAT] = {0};
-    int n[T_NCAT] = {0};
-    tm.collect(t, n);
-    res->t_qpass_ms = t[T_QPASS]; res->n_qpass = n[T_QPASS];
-    res->t_ppass_ms = t[T_PPASS]; res->n_ppass = n[T_PPASS];
-    res->t_ll_ms = t[T_LL]; res->n_ll = n[T_LL];
-    res->t_qstep_ms = t[T_QSTEP]; res->n_qstep = n[T_QSTEP];
-    res->t_pstep_ms = t[T_PSTEP]; res->n_pstep = n[T_PSTEP];
-    res->t_em_ms = t[T_EM]; res->n_em = n[T_EM];
-    res->t_qn_ms = t[T_QN]; res->n_qn = n[T_QN];
-  }
-  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  return HK_OK;
+  return fit_finish(f);
 }
 
 extern "C" int hk_fit(hk_ctx* c, const hk_fit_opts* o, double ll_start, double* ll_trace, hk_fit_result* res) {
   return hk_fit_ex(c, o, ll_start, ll_trace, nullptr, res);
+}
+
+// Several fits in lockstep, each on its own context / stream: every round enqueues one iteration of each
+// active fit (captured graphs), then collects their log-likelihoods.  Contexts created with
+// hk_ctx_create_shared() read one genotype upload.
+extern "C" int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const double* ll_start,
+                            hk_fit_result* res, double* seconds_at_finish) {
+  if (!ctxs || n < 1 || !opts || !res) return fail(nullptr, HK_EINVAL, "hk_fit_batch: invalid arguments");
+  std::vector<FitRun> runs(n);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int b = 0; b < n; ++b) {
+    runs[b].c = ctxs[b];
+    runs[b].o = &opts[b];
+    runs[b].ll_trace = nullptr;
+    runs[b].t_trace = nullptr;
+    runs[b].res = &res[b];
+    CHK(fit_prepare(runs[b], ll_start ? ll_start[b] : NAN));
+  }
+  // EM warm-up, lockstep over EM iterations.
+  int max_em = 0;
+  for (int b = 0; b < n; ++b) max_em = opts[b].em_iters > max_em ? opts[b].em_iters : max_em;
+  std::vector<double> ll_em(n, NAN);
+  for (int it = 0; it < max_em; ++it) {
+    for (int b = 0; b < n; ++b)
+      if (it < opts[b].em_iters) CHK(enqueue_em_iteration(ctxs[b], &opts[b], nullptr));
+    for (int b = 0; b < n; ++b)
+      if (it < opts[b].em_iters) {
+        CHK(read_ll(ctxs[b], 4));
+        ll_em[b] = ctxs[b]->h_ll[3];
+      }
+  }
+  for (int b = 0; b < n; ++b) {
+    hk_fit_opts oz = opts[b];
+    oz.em_iters = 0;
+    runs[b].o = &oz;
+    CHK(fit_em(runs[b], opts[b].em_iters > 0 ? ll_em[b] : (ll_start ? ll_start[b] : NAN)));
+    runs[b].o = &opts[b];
+  }
+  int active = 0;
+  for (int b = 0; b < n; ++b) active += runs[b].active;
+  while (active > 0) {
+    for (int b = 0; b < n; ++b)
+      if (runs[b].active) CHK(launch_iteration(ctxs[b], &opts[b], opts[b].use_graph != 0));
+    for (int b = 0; b < n; ++b)
+      if (runs[b].active) {
+        CHK(read_ll(ctxs[b], 3));
+        CHK(fit_after_iteration(runs[b]));
+        if (!runs[b].active) {
+          --active;
+          if (seconds_at_finish)
+            seconds_at_finish[b] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+      }
+  }
+  for (int b = 0; b < n; ++b) CHK(fit_finish(runs[b]));
+  return HK_OK;
+}
+
+// Autotuner: time the chunk (tile) shapes of the individual-parallel passes (log-likelihood + update_q)
+// and of the block-parallel update_p pass on the context's data, keep the fastest.
+extern "C" int hk_autotune(hk_ctx* c, int reps, int* q_chunks, int* p_chunks) {
+  if (!c || reps < 1) return fail(c, HK_EINVAL, "invalid arguments");
+  drop_graph(c);
+  const int q0 = c->g.qchunks, p0 = c->g.pchunks;
+  const int qreq0 = c->qchunks_req, preq0 = c->pchunks_req;
+  int qbest = q0, pbest = p0;
+  double tq_best = 1e300, tp_best = 1e300;
+  const int mult[] = {1, 2, 4, 8, 16};
+  const int div[] = {4, 2};
+  std::vector<int> qc, pc;
+  for (int d : div) if (q0 / d >= 1) qc.push_back(q0 / d);
+  for (int m : mult) qc.push_back(q0 * m);
+  for (int d : div) if (p0 / d >= 1) pc.push_back(p0 / d);
+  for (int m : mult) pc.push_back(p0 * m);
+  for (int qv : qc) {
+    if (qv > c->g.J8) continue;
+    CHK(hk_set_tuning(c, qv, preq0));
+    double tl = 0, tu = 0;
+    CHK(hk_time_pass(c, 0, reps, &tl));
+    CHK(hk_time_pass(c, 1, reps, &tu));
+    if (tl + tu < tq_best) { tq_best = tl + tu; qbest = c->g.qchunks; }
+  }
+  for (int pv : pc) {
+    if (pv > c->g.I8) continue;
+    CHK(hk_set_tuning(c, qbest, pv));
+    double tp = 0;
+    CHK(hk_time_pass(c, 2, reps, &tp));
+    if (tp < tp_best) { tp_best = tp; pbest = c->g.pchunks; }
+  }
+  (void)qreq0;
+  CHK(hk_set_tuning(c, qbest, pbest));
+  if (q_chunks) *q_chunks = qbest;
+  if (p_chunks) *p_chunks = pbest;
+  return HK_OK;
 }
 
 extern "C" int hk_time_pass(hk_ctx* c, int kind, int reps, double* ms) {

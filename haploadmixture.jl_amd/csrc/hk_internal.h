@@ -59,8 +59,12 @@ hipError_t launch_em_q(int K, int64_t I, int64_t J, const double* res_q, int NT,
 hipError_t launch_em_p(int K, int64_t J, const double* res_p, int NT, const double* p, double* p_next, double eps,
                        hipStream_t s);
 hipError_t launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
-hipError_t launch_zal(int64_t N, int Qn, int iter, double* U, double* V, const double* x0, const double* x1,
+hipError_t launch_zal(int64_t N, int Qn, const int* d_iter, double* U, double* V, const double* x0, const double* x1,
                       const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s);
+hipError_t launch_accept(int64_t N, double* ll, const double* x_basic, const double* x_qn, double* x, int* d_iter,
+                         hipStream_t s);
+hipError_t launch_copy_f64(int64_t n, const double* src, double* dst, hipStream_t s);
+hipError_t launch_set_int(int* p, int v, hipStream_t s);
 hipError_t launch_axpy_step(int64_t n, const double* a, const double* d, double f, double* out, hipStream_t s);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy16(const void* src, void* dst, int64_t n16, hipStream_t s);

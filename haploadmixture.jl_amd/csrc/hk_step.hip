@@ -473,11 +473,21 @@ __global__ void k_axpy(int64_t n, const double* __restrict__ a, const double* __
 #define ZAL_NDOT (ZAL_QMAX * ZAL_QMAX * 2 + ZAL_QMAX)
 #define ZAL_NB 512
 
-__global__ __launch_bounds__(256) void k_zal_dots(int64_t N, int col, int ncols, double* __restrict__ U,
-                                                  double* __restrict__ V, const double* __restrict__ x0,
-                                                  const double* __restrict__ x1, const double* __restrict__ x2,
-                                                  double* __restrict__ part) {
+// The QN iteration i (1-based) is read from device memory so that an iteration can be replayed from a
+// captured graph: column (i-1) mod Qn receives the new secant pair, min(i, Qn) columns are used.
+__device__ __forceinline__ void zal_cols(const int* it, int Qn, int& col, int& ncols) {
+  const int i = *it;
+  col = (i - 1) % Qn;
+  ncols = i < Qn ? i : Qn;
+}
+
+__global__ __launch_bounds__(256) void k_zal_dots(int64_t N, const int* __restrict__ d_iter, int Qn,
+                                                  double* __restrict__ U, double* __restrict__ V,
+                                                  const double* __restrict__ x0, const double* __restrict__ x1,
+                                                  const double* __restrict__ x2, double* __restrict__ part) {
   __shared__ double sh[256];
+  int col, ncols;
+  zal_cols(d_iter, Qn, col, ncols);
   double acc[ZAL_NDOT];
 #pragma unroll
   for (int d = 0; d < ZAL_NDOT; ++d) acc[d] = 0.0;
@@ -510,8 +520,12 @@ __global__ __launch_bounds__(256) void k_zal_dots(int64_t N, int col, int ncols,
   }
 }
 
-__global__ void k_zal_solve(int nb, int ncols, const double* __restrict__ part, double* __restrict__ coef) {
+__global__ void k_zal_solve(int nb, const int* __restrict__ d_iter, int Qn, const double* __restrict__ part,
+                            double* __restrict__ coef) {
   __shared__ double dots[ZAL_NDOT];
+  int col, ncols;
+  zal_cols(d_iter, Qn, col, ncols);
+  (void)col;
   const int d = threadIdx.x;
   if (d < ZAL_NDOT) {
     double s = 0.0;
@@ -553,14 +567,43 @@ __global__ void k_zal_solve(int nb, int ncols, const double* __restrict__ part, 
   coef[ZAL_QMAX] = (double)rc;
 }
 
-__global__ void k_zal_apply(int64_t N, int ncols, const double* __restrict__ V, const double* __restrict__ x1,
-                            const double* __restrict__ coef, double* __restrict__ x_out) {
+__global__ void k_zal_apply(int64_t N, const int* __restrict__ d_iter, int Qn, const double* __restrict__ V,
+                            const double* __restrict__ x1, const double* __restrict__ coef, double* __restrict__ x_out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N) return;
+  int col, ncols;
+  zal_cols(d_iter, Qn, col, ncols);
+  (void)col;
   double s = x1[t];
   if (coef[ZAL_QMAX] == 0.0)
     for (int a = 0; a < ncols; ++a) s -= V[t + N * a] * coef[a];
   x_out[t] = s;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Device-side accept rule of admixture_qn! (algorithms_outer.jl:154-160): keep the QN point when its
+// log-likelihood beats the plain double sweep.  ll[0] = ll_basic, ll[1] = ll_qn -> ll[2] = ll_new; the
+// winning parameter vector is copied into x; block 0 / thread 0 then advances the iteration counter.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_accept(int64_t N, double* __restrict__ ll, const double* __restrict__ x_basic,
+                         const double* __restrict__ x_qn, double* __restrict__ x, int* __restrict__ d_iter) {
+  const bool take_qn = ll[0] < ll[1];
+  const double* src = take_qn ? x_qn : x_basic;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += stride) x[t] = src[t];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ll[2] = take_qn ? ll[1] : ll[0];
+    *d_iter += 1;
+  }
+}
+
+__global__ void k_copy_f64(int64_t n, const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) dst[t] = src[t];
+}
+
+__global__ void k_set_int(int* __restrict__ p, int v) {
+  if (threadIdx.x == 0) *p = v;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -646,6 +689,23 @@ hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t launch_accept(int64_t N, double* ll, const double* x_basic, const double* x_qn, double* x, int* d_iter,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_accept, dim3(1024), dim3(256), 0, s, N, ll, x_basic, x_qn, x, d_iter);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_f64(int64_t n, const double* src, double* dst, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_f64, dim3(1024), dim3(256), 0, s, n, src, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_int(int* p, int v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, p, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy16(const void* src, void* dst, int64_t n16, hipStream_t s) {
   hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n16);
   return hipGetLastError();
@@ -656,14 +716,12 @@ hipError_t launch_axpy_step(int64_t n, const double* a, const double* d, double 
   return hipGetLastError();
 }
 
-hipError_t launch_zal(int64_t N, int Qn, int iter, double* U, double* V, const double* x0, const double* x1,
+hipError_t launch_zal(int64_t N, int Qn, const int* d_iter, double* U, double* V, const double* x0, const double* x1,
                       const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s) {
   if (Qn < 1 || Qn > ZAL_QMAX) return hipErrorInvalidValue;
-  const int col = (iter - 1) % Qn;
-  const int ncols = iter < Qn ? iter : Qn;
-  hipLaunchKernelGGL(k_zal_dots, dim3(ZAL_NB), dim3(256), 0, s, N, col, ncols, U, V, x0, x1, x2, dots_part);
-  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(64), 0, s, ZAL_NB, ncols, dots_part, coef);
-  hipLaunchKernelGGL(k_zal_apply, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, ncols, V, x1, coef, x_out);
+  hipLaunchKernelGGL(k_zal_dots, dim3(ZAL_NB), dim3(256), 0, s, N, d_iter, Qn, U, V, x0, x1, x2, dots_part);
+  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(64), 0, s, ZAL_NB, d_iter, Qn, dots_part, coef);
+  hipLaunchKernelGGL(k_zal_apply, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, d_iter, Qn, V, x1, coef, x_out);
   return hipGetLastError();
 }
 

@@ -41,7 +41,7 @@ class FitOpts(ctypes.Structure):
     _fields_ = [
         ("em_iters", _int), ("n_iter", _int), ("rtol", _dbl), ("Qn", _int),
         ("fix_q", _int), ("fix_p", _int), ("q_penalty", _int), ("p_penalty", _int),
-        ("rho", _dbl), ("eps", _dbl), ("verbose", _int), ("profile", _int),
+        ("rho", _dbl), ("eps", _dbl), ("verbose", _int), ("profile", _int), ("use_graph", _int),
     ]
 
 
@@ -84,6 +84,9 @@ def hip():
     L = ctypes.CDLL(HIP_LIB)
     L.hk_ctx_create.argtypes = [ctypes.POINTER(_vp), _i64, _i64, _int, _vp, _i64, _i64, _int]
     L.hk_ctx_destroy.argtypes = [_vp]
+    L.hk_ctx_create_shared.argtypes = [ctypes.POINTER(_vp), _vp, _int, _int]
+    L.hk_fit_batch.argtypes = [ctypes.POINTER(_vp), _int, ctypes.POINTER(FitOpts), _vp, ctypes.POINTER(FitResult), _vp]
+    L.hk_autotune.argtypes = [_vp, _int, ctypes.POINTER(_int), ctypes.POINTER(_int)]
     L.hk_last_error.argtypes = [_vp]
     L.hk_last_error.restype = ctypes.c_char_p
     L.hk_create_error.restype = ctypes.c_char_p

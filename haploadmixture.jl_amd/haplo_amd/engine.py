@@ -45,6 +45,23 @@ class HipAdmixData:
         self._h = h
         del self._bed_keepalive
 
+    @classmethod
+    def shared(cls, base: "HipAdmixData", K: int, precision: str | None = None) -> "HipAdmixData":
+        """A fit context over base's genotype upload (hk_ctx_create_shared); base must stay open."""
+        self = cls.__new__(cls)
+        self._L = _native.hip()
+        self.I, self.J, self.K = base.I, base.J, int(K)
+        self.precision = precision or base.precision
+        if self.precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {list(PRECISIONS)}")
+        h = ctypes.c_void_p()
+        rc = self._L.hk_ctx_create_shared(ctypes.byref(h), base._h, self.K, PRECISIONS[self.precision])
+        if rc != 0:
+            raise HipError(rc, self._L.hk_create_error().decode())
+        self._h = h
+        self._base = base
+        return self
+
     # -- plumbing ---------------------------------------------------------------------------------
     def _check(self, rc):
         if rc != 0:
@@ -177,5 +194,27 @@ class HipAdmixData:
         self._check(self._L.hk_copy_bench(self._h, int(nbytes), int(reps), ctypes.byref(g)))
         return g.value
 
+    def autotune(self, reps: int = 3):
+        """Time candidate chunk shapes of the passes on this data and keep the fastest (hk_autotune)."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.hk_autotune(self._h, int(reps), ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def sync(self):
         self._check(self._L.hk_sync(self._h))
+
+
+def fit_batch(engines, opts_list, ll_start=None):
+    """hk_fit_batch: fits of several contexts in lockstep; returns (results, seconds_at_finish)."""
+    n = len(engines)
+    L = _native.hip()
+    handles = (ctypes.c_void_p * n)(*[e._h for e in engines])
+    opts = (FitOpts * n)(*opts_list)
+    res = (FitResult * n)()
+    fin = np.zeros(n)
+    lls = None if ll_start is None else np.ascontiguousarray(ll_start, dtype=np.float64)
+    rc = L.hk_fit_batch(handles, n, opts, ptr(lls), res, ptr(fin))
+    if rc != 0:
+        msgs = "; ".join(L.hk_last_error(e._h).decode() for e in engines if L.hk_last_error(e._h))
+        raise HipError(rc, msgs or L.hk_create_error().decode())
+    return [r.as_dict() for r in res], fin

@@ -49,6 +49,9 @@ enum hk_precision {
  * n_snp >= 2J SNPs, and allocate the parameter and pass buffers.  1 <= K <= HK_MAX_K. */
 int hk_ctx_create(hk_ctx** out, int64_t I, int64_t J, int K, const uint8_t* bed, int64_t bytes_per_snp,
                   int64_t n_snp, int precision);
+/* A further fit context (own K, precision, parameters, buffers and stream) over the genotype upload of
+ * `base`, which must outlive it.  Used for several fits of one dataset (grid of K x restarts). */
+int hk_ctx_create_shared(hk_ctx** out, hk_ctx* base, int K, int precision);
 int hk_ctx_destroy(hk_ctx* ctx);
 const char* hk_last_error(const hk_ctx* ctx);
 /* Library-level error text for failures before a context exists (hk_ctx_create). */
@@ -100,6 +103,7 @@ typedef struct {
   double eps;            /* simplex pseudocount 1e-5                                             */
   int verbose;           /* print per-iteration lines like @info in algorithms_outer.jl:18,162   */
   int profile;           /* time every pass with HIP events (hk_fit_result.t_*)                  */
+  int use_graph;         /* replay each QN iteration from a captured hipGraph (ignored if profile) */
 } hk_fit_opts;
 
 typedef struct {
@@ -125,6 +129,16 @@ int hk_fit(hk_ctx* ctx, const hk_fit_opts* opts, double ll_start, double* ll_tra
  * ll_new was known on the host. */
 int hk_fit_ex(hk_ctx* ctx, const hk_fit_opts* opts, double ll_start, double* ll_trace, double* t_trace,
               hk_fit_result* res);
+
+/* Fits of several contexts in lockstep (each on its own stream; with use_graph every round launches one
+ * captured graph per active fit).  opts, ll_start (may be NULL) and res have n entries; seconds_at_finish
+ * (may be NULL) receives the wall time at which each fit stopped. */
+int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const double* ll_start, hk_fit_result* res,
+                 double* seconds_at_finish);
+
+/* Autotuner: time candidate chunk shapes of the passes on the context's data (reps launches each) and keep
+ * the fastest; the chosen q/p chunk counts are returned. */
+int hk_autotune(hk_ctx* ctx, int reps, int* q_chunks, int* p_chunks);
 
 /* Pass geometry tuning: number of j chunks of the individual-parallel passes and i chunks of the
  * block-parallel passes (0 = automatic). */
