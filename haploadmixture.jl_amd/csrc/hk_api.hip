@@ -558,6 +558,7 @@ void hk_fit_opts_default(hk_fit_opts* o) {
   o->verbose = 0;
   o->profile = 0;
   o->use_graph = 1;
+  o->em_fix_q = 0;
 }
 
 }  // extern "C"
@@ -685,8 +686,12 @@ static int enqueue_em_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm) {
   double* xn = c->xb[1];
   const int64_t N = (int64_t)K * (c->I + 4 * c->J);
   if (tm) tm->begin(T_EM);
-  CHK(run_qpass(c, MODE_GRAD, qof(c, x), pof(c, x)));
-  HIPC(c, launch_em_q(K, c->I, c->J, c->res_q, K, c->dm, qof(c, x), qof(c, xn), o->eps, c->s));
+  if (o->em_fix_q) {  // em_p! (algorithms_inner.jl:40): P-only EM update, q kept
+    HIPC(c, launch_copy_f64((int64_t)K * c->I, qof(c, x), qof(c, xn), c->s));
+  } else {
+    CHK(run_qpass(c, MODE_GRAD, qof(c, x), pof(c, x)));
+    HIPC(c, launch_em_q(K, c->I, c->J, c->res_q, K, c->dm, qof(c, x), qof(c, xn), o->eps, c->s));
+  }
   CHK(run_ppass(c, MODE_GRAD, qof(c, x), pof(c, x)));
   HIPC(c, launch_em_p(K, c->J, c->res_p, K, pof(c, x), pof(c, xn), o->eps, c->s));
   if (tm) tm->end();
@@ -976,4 +981,9 @@ extern "C" int hk_copy_bench(hk_ctx* c, int64_t bytes, int reps, double* gbps) {
   HIPC(c, e);
   *gbps = 2.0 * (double)(n16 * 16) * reps / (ms * 1e-3) / 1e9;
   return HK_OK;
+}
+
+extern "C" int hk_features(hk_ctx* c, int n_blocks, uint64_t seed, float* F) {
+  if (!c || !F || n_blocks < 1 || n_blocks > c->J) return fail(c, HK_EINVAL, "invalid arguments (1 <= n_blocks <= J)");
+  return project_features(c->g, n_blocks, seed, F, c->s, c->err, sizeof c->err);
 }

@@ -69,4 +69,8 @@ hipError_t launch_axpy_step(int64_t n, const double* a, const double* d, double 
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy16(const void* src, void* dst, int64_t n16, hipStream_t s);
 
+// Random-projection features (hk_mlp.hip): F_host (I x 256) from the first n_blocks blocks of gq.
+int project_features(const PassGeom& g, int n_blocks, uint64_t seed, float* F_host, hipStream_t s, char* err,
+                     size_t errlen);
+
 }  // namespace hk

@@ -1,0 +1,584 @@
+// hk_mlp.hip — learned warm start for Q: genotype random projection and a 3-layer MLP trained and run
+// with hand-written kernels (gfx950).
+//
+// Features (per individual i): f_i = S^{-1/2} sum_{s observed} (g_is - mu_s) R_s, with g the allele-2
+// dosage of SNP s (first S = 2 * n_blocks SNPs of the context), mu_s its mean over observed individuals
+// and R a seeded Gaussian S x D matrix (Box-Muller over a counter hash, D = 256).
+//
+// MLP: x (D) -> relu(W1 x + b1) (H) -> relu(W2 . + b2) (H) -> W3 . + b3 (K) -> softmax.  Weights are kept
+// as FP32 masters (Adam state) and used in the matrix products as BF16 (round-to-nearest-even); every
+// product accumulates in FP32 on v_mfma_f32_16x16x4f32.  The loss is the cross-entropy against the
+// converged Q of the plain method, with the clusters of each batch relabelled by the permutation that
+// minimises it (cluster labels of an admixture fit are arbitrary).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/haplo_hip.h"
+#include "hk_device.h"
+#include "hk_internal.h"
+
+namespace hk {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (uint16_t)(u >> 16);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Random projection
+// ---------------------------------------------------------------------------------------------------
+
+// R[s * D + d] ~ N(0, 1): Box-Muller on two 32-bit halves of one counter hash.
+__global__ void k_gauss(float* __restrict__ R, int64_t n, uint64_t seed) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint64_t h = mix64(mix64(seed) ^ (uint64_t)t);
+  const float u1 = ((float)(uint32_t)(h >> 40) + 0.5f) * 0x1.0p-24f;
+  const float u2 = ((float)(uint32_t)(h & 0xffffff) + 0.5f) * 0x1.0p-24f;
+  R[t] = sqrtf(-2.0f * logf(u1)) * cosf(6.2831853f * u2);
+}
+
+// dosage of BED code c: 0 -> 0, 2 (het) -> 1, 3 -> 2; code 1 = missing.
+__device__ __forceinline__ float dosage(unsigned c) { return c == 3u ? 2.0f : (c == 2u ? 1.0f : 0.0f); }
+
+// mu[s] for s < 2 nb: mean dosage over observed individuals (thread per SNP; fixed order).
+__global__ void k_snp_mean(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int nb, float* __restrict__ mu) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= 2 * nb) return;
+  const int j = s >> 1, hi = s & 1;
+  const uint32_t* col = gq + (int64_t)(j >> 3) * Ip;
+  const int sh = 4 * (j & 7) + 2 * hi;
+  double sum = 0.0;
+  int64_t n = 0;
+  for (int64_t i = 0; i < I; ++i) {
+    const unsigned c = (col[i] >> sh) & 3u;
+    if (c != 1u) { sum += dosage(c); ++n; }
+  }
+  mu[s] = n ? (float)(sum / (double)n) : 0.0f;
+}
+
+// F[i * D + d]: workgroup = 64 individuals x 256 features; thread = 16 individuals x 4 features.
+// LDS holds 32 SNPs of R and of the centred dosages per stage.
+constexpr int PROJ_D = 256;
+__global__ __launch_bounds__(256) void k_project(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int nb,
+                                                 const float* __restrict__ mu, const float* __restrict__ R,
+                                                 float scale, float* __restrict__ F) {
+  __shared__ float Rs[32][PROJ_D];
+  __shared__ float As[64][33];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int64_t i0 = (int64_t)blockIdx.x * 64;
+  float acc[16][4];
+#pragma unroll
+  for (int a = 0; a < 16; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+  const int S = 2 * nb;
+  for (int s0 = 0; s0 < S; s0 += 32) {
+    const int ns = min(32, S - s0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * PROJ_D; e += 256) {
+      const int r = e / PROJ_D, d = e - r * PROJ_D;
+      Rs[r][d] = r < ns ? R[(int64_t)(s0 + r) * PROJ_D + d] : 0.0f;
+    }
+    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+      const int ii = e >> 5, r = e & 31;
+      const int64_t i = i0 + ii;
+      float v = 0.0f;
+      if (r < ns && i < I) {
+        const int s = s0 + r, j = s >> 1;
+        const unsigned c = (gq[(int64_t)(j >> 3) * Ip + i] >> (4 * (j & 7) + 2 * (s & 1))) & 3u;
+        v = c == 1u ? 0.0f : dosage(c) - mu[s];
+      }
+      As[ii][r] = v;
+    }
+    __syncthreads();
+    for (int r = 0; r < ns; ++r) {
+      const float4 rv = *reinterpret_cast<const float4*>(&Rs[r][4 * tx]);
+#pragma unroll
+      for (int a = 0; a < 16; ++a) {
+        const float x = As[16 * ty + a][r];
+        acc[a][0] = __builtin_fmaf(x, rv.x, acc[a][0]);
+        acc[a][1] = __builtin_fmaf(x, rv.y, acc[a][1]);
+        acc[a][2] = __builtin_fmaf(x, rv.z, acc[a][2]);
+        acc[a][3] = __builtin_fmaf(x, rv.w, acc[a][3]);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 16; ++a) {
+    const int64_t i = i0 + 16 * ty + a;
+    if (i < I) {
+      float4 o = {acc[a][0] * scale, acc[a][1] * scale, acc[a][2] * scale, acc[a][3] * scale};
+      *reinterpret_cast<float4*>(&F[i * PROJ_D + 4 * tx]) = o;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// MLP kernels
+// ---------------------------------------------------------------------------------------------------
+
+// C[M x N] (row stride ldc) = A(M x Kd) B(Kd x N) (+ C if accumulate): A FP32 with element strides
+// (sam, sak), B FP32 or BF16 with strides (sbk, sbn).  One wavefront per 16 x 16 tile, one MFMA per 4 of Kd.
+template <typename TB>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int Kd, const float* __restrict__ A, int64_t sam,
+                                              int64_t sak, const TB* __restrict__ B, int64_t sbk, int64_t sbn,
+                                              float* __restrict__ C, int64_t ldc, int accumulate) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int tiles_n = (N + 15) / 16;
+  const int tile = blockIdx.x * 4 + wv;
+  const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
+  if (tm * 16 >= M) return;  // whole wavefront exits together
+  const int row = tm * 16 + m, col = tn * 16 + m;
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < Kd; k0 += 4) {
+    const int k = k0 + g;
+    float a = 0.f, b = 0.f;
+    if (row < M && k < Kd) a = A[row * sam + k * sak];
+    if (col < N && k < Kd) {
+      if constexpr (sizeof(TB) == 2) b = bf16_to_f32(B[k * sbk + col * sbn]);
+      else b = B[k * sbk + col * sbn];
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = tm * 16 + 4 * g + v, c = tn * 16 + m;
+    if (r < M && c < N) {
+      float* o = C + (int64_t)r * ldc + c;
+      *o = accumulate ? *o + acc[v] : acc[v];
+    }
+  }
+}
+
+// Z[n x N] += bias; optionally H = relu(Z) in place.
+__global__ void k_bias_act(float* __restrict__ Z, int64_t n, int N, const float* __restrict__ b, int relu) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * N) return;
+  float v = Z[t] + b[t % N];
+  Z[t] = relu ? fmaxf(v, 0.0f) : v;
+}
+
+// Row softmax of Z (n x K) into P.
+__global__ void k_softmax(const float* __restrict__ Z, int64_t n, int K, float* __restrict__ P) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* z = Z + i * K;
+  float mx = z[0];
+  for (int k = 1; k < K; ++k) mx = fmaxf(mx, z[k]);
+  float s = 0.f;
+  for (int k = 0; k < K; ++k) s += expf(z[k] - mx);
+  for (int k = 0; k < K; ++k) P[i * K + k] = expf(z[k] - mx) / s;
+}
+
+// Mx[k * K + k2] = sum_i T[i, k2] log P[i, k]  (thread per (k, k2); fixed order)
+__global__ void k_match(const float* __restrict__ P, const float* __restrict__ T, int64_t n, int K,
+                        double* __restrict__ Mx) {
+  const int t = threadIdx.x;
+  if (t >= K * K) return;
+  const int k = t / K, k2 = t - k * K;
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += (double)T[i * K + k2] * log(fmaxf(P[i * K + k], 1e-30f));
+  Mx[t] = s;
+}
+
+// dZ[i, k] = (P[i, k] - T[i, perm[k]]) / n
+__global__ void k_dlogits(const float* __restrict__ P, const float* __restrict__ T, int64_t n, int K,
+                          const int* __restrict__ perm, float* __restrict__ dZ) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * K) return;
+  const int64_t i = t / K;
+  const int k = (int)(t - i * K);
+  dZ[t] = (P[t] - T[i * K + perm[k]]) / (float)n;
+}
+
+// dZ *= [H > 0]
+__global__ void k_relu_back(float* __restrict__ dZ, const float* __restrict__ H, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n && !(H[t] > 0.0f)) dZ[t] = 0.0f;
+}
+
+// db[c] = sum_i dZ[i, c]
+__global__ void k_colsum(const float* __restrict__ dZ, int64_t n, int N, float* __restrict__ db) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  float s = 0.f;
+  for (int64_t i = 0; i < n; ++i) s += dZ[i * N + c];
+  db[c] = s;
+}
+
+// Adam on the FP32 masters, then the BF16 copy of the weight matrices (mask[t] = 1 for matrix entries).
+__global__ void k_adam(int64_t n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float gt = g[t];
+  const float mt = b1 * m[t] + (1.f - b1) * gt;
+  const float vt = b2 * v[t] + (1.f - b2) * gt * gt;
+  m[t] = mt;
+  v[t] = vt;
+  w[t] -= lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ w, int64_t n, uint16_t* __restrict__ o) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) o[t] = f32_to_bf16(w[t]);
+}
+
+}  // namespace hk
+
+using namespace hk;
+
+// ---------------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------------
+
+struct hk_mlp {
+  int din = 0, width = 0, dout = 0;
+  hipStream_t s = nullptr;
+  // parameter block (FP32 masters), gradients, Adam moments; layout W1 b1 W2 b2 W3 b3
+  float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+  int64_t nparam = 0;
+  int64_t off[6] = {0};
+  uint16_t* wb = nullptr;  // BF16 copy of the parameter block
+  // resident data and activations
+  float *X = nullptr, *T = nullptr;
+  int64_t ndata = 0;
+  float *H1 = nullptr, *H2 = nullptr, *Z3 = nullptr, *P = nullptr, *dZ3 = nullptr, *dH2 = nullptr, *dH1 = nullptr;
+  int64_t cap = 0;
+  double* Mx = nullptr;
+  int* perm = nullptr;
+  char err[256] = {0};
+};
+
+static int mlp_fail(hk_mlp* m, int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  if (m) vsnprintf(m->err, sizeof m->err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define MH(call)                                                                                         \
+  do {                                                                                                   \
+    hipError_t e_ = (call);                                                                              \
+    if (e_ != hipSuccess) return mlp_fail(m, HK_EHIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                                          __LINE__);                                                     \
+  } while (0)
+
+static unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+template <typename TB>
+static hipError_t gemm(hipStream_t s, int M, int N, int Kd, const float* A, int64_t sam, int64_t sak, const TB* B,
+                       int64_t sbk, int64_t sbn, float* C, int64_t ldc, int acc) {
+  const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
+  hipLaunchKernelGGL((k_gemm<TB>), dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, M, N, Kd, A, sam, sak, B, sbk,
+                     sbn, C, ldc, acc);
+  return hipGetLastError();
+}
+
+static int ensure_cap(hk_mlp* m, int64_t n) {
+  if (n <= m->cap) return HK_OK;
+  float** bufs[] = {&m->H1, &m->H2, &m->Z3, &m->P, &m->dZ3, &m->dH2, &m->dH1};
+  for (float** b : bufs)
+    if (*b) {
+      MH(hipFree(*b));
+      *b = nullptr;
+    }
+  MH(hipMalloc(&m->H1, sizeof(float) * n * m->width));
+  MH(hipMalloc(&m->H2, sizeof(float) * n * m->width));
+  MH(hipMalloc(&m->Z3, sizeof(float) * n * m->dout));
+  MH(hipMalloc(&m->P, sizeof(float) * n * m->dout));
+  MH(hipMalloc(&m->dZ3, sizeof(float) * n * m->dout));
+  MH(hipMalloc(&m->dH2, sizeof(float) * n * m->width));
+  MH(hipMalloc(&m->dH1, sizeof(float) * n * m->width));
+  m->cap = n;
+  return HK_OK;
+}
+
+// Forward on device rows X (n x din) -> P (softmax), keeping H1, H2 for the backward pass.
+static int mlp_forward_dev(hk_mlp* m, const float* X, int64_t n) {
+  const int D = m->din, H = m->width, K = m->dout;
+  const uint16_t* W1 = m->wb + m->off[0];
+  const uint16_t* W2 = m->wb + m->off[2];
+  const uint16_t* W3 = m->wb + m->off[4];
+  MH(gemm<uint16_t>(m->s, (int)n, H, D, X, D, 1, W1, H, 1, m->H1, H, 0));
+  hipLaunchKernelGGL(k_bias_act, dim3(nblk(n * H)), dim3(256), 0, m->s, m->H1, n, H, m->w + m->off[1], 1);
+  MH(gemm<uint16_t>(m->s, (int)n, H, H, m->H1, H, 1, W2, H, 1, m->H2, H, 0));
+  hipLaunchKernelGGL(k_bias_act, dim3(nblk(n * H)), dim3(256), 0, m->s, m->H2, n, H, m->w + m->off[3], 1);
+  MH(gemm<uint16_t>(m->s, (int)n, K, H, m->H2, H, 1, W3, K, 1, m->Z3, K, 0));
+  hipLaunchKernelGGL(k_bias_act, dim3(nblk(n * K)), dim3(256), 0, m->s, m->Z3, n, K, m->w + m->off[5], 0);
+  hipLaunchKernelGGL(k_softmax, dim3(nblk(n)), dim3(256), 0, m->s, m->Z3, n, K, m->P);
+  MH(hipGetLastError());
+  return HK_OK;
+}
+
+static int refresh_bf16(hk_mlp* m) {
+  hipLaunchKernelGGL(k_to_bf16, dim3(nblk(m->nparam)), dim3(256), 0, m->s, m->w, m->nparam, m->wb);
+  MH(hipGetLastError());
+  return HK_OK;
+}
+
+extern "C" {
+
+int hk_mlp_destroy(hk_mlp* m) {
+  if (!m) return HK_OK;
+  if (m->s) (void)hipStreamSynchronize(m->s);
+  void* ptrs[] = {m->w, m->g, m->m, m->v, m->wb, m->X, m->T, m->H1, m->H2, m->Z3, m->P, m->dZ3, m->dH2, m->dH1,
+                  m->Mx, m->perm};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (m->s) (void)hipStreamDestroy(m->s);
+  delete m;
+  return HK_OK;
+}
+
+const char* hk_mlp_last_error(const hk_mlp* m) { return m ? m->err : ""; }
+
+int hk_mlp_create(hk_mlp** out, int din, int width, int dout, uint64_t seed) {
+  if (!out || din < 1 || width < 1 || dout < 2 || dout > 8) return HK_EINVAL;
+  hk_mlp* m = new hk_mlp();
+  m->din = din;
+  m->width = width;
+  m->dout = dout;
+  const int64_t sizes[6] = {(int64_t)din * width, width, (int64_t)width * width, width, (int64_t)width * dout, dout};
+  int64_t o = 0;
+  for (int t = 0; t < 6; ++t) {
+    m->off[t] = o;
+    o += (sizes[t] + 63) / 64 * 64;
+  }
+  m->nparam = o;
+  int rc = HK_OK;
+  auto H = [&](hipError_t e) {
+    if (e != hipSuccess && rc == HK_OK) rc = mlp_fail(m, HK_EHIP, "hk_mlp_create: %s", hipGetErrorString(e));
+  };
+  H(hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking));
+  H(hipMalloc(&m->w, sizeof(float) * o));
+  H(hipMalloc(&m->g, sizeof(float) * o));
+  H(hipMalloc(&m->m, sizeof(float) * o));
+  H(hipMalloc(&m->v, sizeof(float) * o));
+  H(hipMalloc(&m->wb, sizeof(uint16_t) * o));
+  H(hipMalloc(&m->Mx, sizeof(double) * 64));
+  H(hipMalloc(&m->perm, sizeof(int) * 8));
+  if (rc == HK_OK) {
+    // He-uniform initialisation on the host (seeded), biases zero.
+    std::vector<float> w(o, 0.0f);
+    uint64_t st = seed * 0x9e3779b97f4a7c15ULL + 1;
+    auto u01 = [&]() {
+      st += 0x9e3779b97f4a7c15ULL;
+      uint64_t z = st;
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+      z ^= z >> 31;
+      return ((z >> 11) + 0.5) * 0x1.0p-53;
+    };
+    const int fan_in[3] = {din, width, width};
+    for (int l = 0; l < 3; ++l) {
+      const double a = sqrt(6.0 / fan_in[l]);
+      for (int64_t t = 0; t < sizes[2 * l]; ++t) w[m->off[2 * l] + t] = (float)((2.0 * u01() - 1.0) * a);
+    }
+    H(hipMemcpyAsync(m->w, w.data(), sizeof(float) * o, hipMemcpyHostToDevice, m->s));
+    H(hipMemsetAsync(m->m, 0, sizeof(float) * o, m->s));
+    H(hipMemsetAsync(m->v, 0, sizeof(float) * o, m->s));
+    H(hipMemsetAsync(m->g, 0, sizeof(float) * o, m->s));
+    if (rc == HK_OK) rc = refresh_bf16(m);
+    H(hipStreamSynchronize(m->s));
+  }
+  if (rc != HK_OK) {
+    hk_mlp_destroy(m);
+    return rc;
+  }
+  *out = m;
+  return HK_OK;
+}
+
+int hk_mlp_num_params(hk_mlp* m, int64_t* n, int64_t* offsets) {
+  if (!m || !n) return HK_EINVAL;
+  *n = m->nparam;
+  if (offsets)
+    for (int t = 0; t < 6; ++t) offsets[t] = m->off[t];
+  return HK_OK;
+}
+
+int hk_mlp_get_params(hk_mlp* m, float* w, float* grad) {
+  if (!m) return HK_EINVAL;
+  if (w) MH(hipMemcpyAsync(w, m->w, sizeof(float) * m->nparam, hipMemcpyDeviceToHost, m->s));
+  if (grad) MH(hipMemcpyAsync(grad, m->g, sizeof(float) * m->nparam, hipMemcpyDeviceToHost, m->s));
+  MH(hipStreamSynchronize(m->s));
+  return HK_OK;
+}
+
+int hk_mlp_set_params(hk_mlp* m, const float* w) {
+  if (!m || !w) return HK_EINVAL;
+  MH(hipMemcpyAsync(m->w, w, sizeof(float) * m->nparam, hipMemcpyHostToDevice, m->s));
+  int rc = refresh_bf16(m);
+  if (rc != HK_OK) return rc;
+  MH(hipStreamSynchronize(m->s));
+  return HK_OK;
+}
+
+int hk_mlp_load_data(hk_mlp* m, const float* X, const float* T, int64_t n) {
+  if (!m || !X || n < 1) return HK_EINVAL;
+  if (m->X) {
+    MH(hipFree(m->X));
+    m->X = nullptr;
+  }
+  if (m->T) {
+    MH(hipFree(m->T));
+    m->T = nullptr;
+  }
+  MH(hipMalloc(&m->X, sizeof(float) * n * m->din));
+  MH(hipMemcpyAsync(m->X, X, sizeof(float) * n * m->din, hipMemcpyHostToDevice, m->s));
+  if (T) {
+    MH(hipMalloc(&m->T, sizeof(float) * n * m->dout));
+    MH(hipMemcpyAsync(m->T, T, sizeof(float) * n * m->dout, hipMemcpyHostToDevice, m->s));
+  }
+  MH(hipStreamSynchronize(m->s));
+  m->ndata = n;
+  return HK_OK;
+}
+
+int hk_mlp_forward(hk_mlp* m, const float* X, int64_t n, float* P) {
+  if (!m || !X || !P || n < 1) return HK_EINVAL;
+  float* Xd = nullptr;
+  MH(hipMalloc(&Xd, sizeof(float) * n * m->din));
+  int rc = HK_OK;
+  hipError_t e = hipMemcpyAsync(Xd, X, sizeof(float) * n * m->din, hipMemcpyHostToDevice, m->s);
+  if (e == hipSuccess) {
+    rc = ensure_cap(m, n);
+    if (rc == HK_OK) rc = mlp_forward_dev(m, Xd, n);
+    if (rc == HK_OK) e = hipMemcpyAsync(P, m->P, sizeof(float) * n * m->dout, hipMemcpyDeviceToHost, m->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->s);
+  }
+  (void)hipFree(Xd);
+  if (rc != HK_OK) return rc;
+  MH(e);
+  return HK_OK;
+}
+
+// Forward + backward on resident rows [row0, row0 + n).  match != 0: relabel the target columns by the
+// permutation minimising the cross-entropy (chosen on the host from the K x K match matrix).
+int hk_mlp_grad(hk_mlp* m, int64_t row0, int64_t n, int match, float* loss, int* perm_out) {
+  if (!m || !m->X || !m->T || row0 < 0 || n < 1 || row0 + n > m->ndata) return HK_EINVAL;
+  const int D = m->din, H = m->width, K = m->dout;
+  const float* X = m->X + row0 * D;
+  const float* T = m->T + row0 * K;
+  int rc = ensure_cap(m, n);
+  if (rc != HK_OK) return rc;
+  rc = mlp_forward_dev(m, X, n);
+  if (rc != HK_OK) return rc;
+  hipLaunchKernelGGL(k_match, dim3(1), dim3(64), 0, m->s, m->P, T, n, K, m->Mx);
+  double Mx[64];
+  MH(hipMemcpyAsync(Mx, m->Mx, sizeof(double) * K * K, hipMemcpyDeviceToHost, m->s));
+  MH(hipStreamSynchronize(m->s));
+  int perm[8], best[8];
+  for (int k = 0; k < K; ++k) perm[k] = best[k] = k;
+  double best_v = 0.0;
+  for (int k = 0; k < K; ++k) best_v -= Mx[k * K + k];
+  if (match) {
+    // enumerate permutations (Heap's algorithm order is irrelevant: keep the first minimum)
+    std::vector<int> c(K, 0);
+    int i = 1;
+    while (i < K) {
+      if (c[i] < i) {
+        const int a = (i % 2 == 0) ? 0 : c[i];
+        int tmp = perm[a]; perm[a] = perm[i]; perm[i] = tmp;
+        double v = 0.0;
+        for (int k = 0; k < K; ++k) v -= Mx[k * K + perm[k]];
+        if (v < best_v) { best_v = v; for (int k = 0; k < K; ++k) best[k] = perm[k]; }
+        ++c[i];
+        i = 1;
+      } else {
+        c[i] = 0;
+        ++i;
+      }
+    }
+  }
+  MH(hipMemcpyAsync(m->perm, best, sizeof(int) * K, hipMemcpyHostToDevice, m->s));
+  if (loss) *loss = (float)(best_v / (double)n);
+  if (perm_out)
+    for (int k = 0; k < K; ++k) perm_out[k] = best[k];
+  const uint16_t* W2 = m->wb + m->off[2];
+  const uint16_t* W3 = m->wb + m->off[4];
+  float* gW1 = m->g + m->off[0];
+  float* gb1 = m->g + m->off[1];
+  float* gW2 = m->g + m->off[2];
+  float* gb2 = m->g + m->off[3];
+  float* gW3 = m->g + m->off[4];
+  float* gb3 = m->g + m->off[5];
+  hipLaunchKernelGGL(k_dlogits, dim3(nblk(n * K)), dim3(256), 0, m->s, m->P, T, n, K, m->perm, m->dZ3);
+  // layer 3: gW3 (H x K) = H2^T dZ3, gb3, dH2 = dZ3 W3^T
+  MH(gemm<float>(m->s, H, K, (int)n, m->H2, 1, H, m->dZ3, K, 1, gW3, K, 0));
+  hipLaunchKernelGGL(k_colsum, dim3(nblk(K, 64)), dim3(64), 0, m->s, m->dZ3, n, K, gb3);
+  MH(gemm<uint16_t>(m->s, (int)n, H, K, m->dZ3, K, 1, W3, 1, K, m->dH2, H, 0));
+  hipLaunchKernelGGL(k_relu_back, dim3(nblk(n * H)), dim3(256), 0, m->s, m->dH2, m->H2, n * H);
+  // layer 2
+  MH(gemm<float>(m->s, H, H, (int)n, m->H1, 1, H, m->dH2, H, 1, gW2, H, 0));
+  hipLaunchKernelGGL(k_colsum, dim3(nblk(H)), dim3(256), 0, m->s, m->dH2, n, H, gb2);
+  MH(gemm<uint16_t>(m->s, (int)n, H, H, m->dH2, H, 1, W2, 1, H, m->dH1, H, 0));
+  hipLaunchKernelGGL(k_relu_back, dim3(nblk(n * H)), dim3(256), 0, m->s, m->dH1, m->H1, n * H);
+  // layer 1
+  MH(gemm<float>(m->s, D, H, (int)n, X, 1, D, m->dH1, H, 1, gW1, H, 0));
+  hipLaunchKernelGGL(k_colsum, dim3(nblk(H)), dim3(256), 0, m->s, m->dH1, n, H, gb1);
+  MH(hipGetLastError());
+  MH(hipStreamSynchronize(m->s));
+  return HK_OK;
+}
+
+int hk_mlp_adam(hk_mlp* m, float lr, float beta1, float beta2, float eps, int step) {
+  if (!m || step < 1) return HK_EINVAL;
+  const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(nblk(m->nparam)), dim3(256), 0, m->s, m->nparam, m->w, m->g, m->m, m->v, lr, beta1,
+                     beta2, eps, c1, c2);
+  MH(hipGetLastError());
+  int rc = refresh_bf16(m);
+  if (rc != HK_OK) return rc;
+  MH(hipStreamSynchronize(m->s));
+  return HK_OK;
+}
+
+}  // extern "C"
+
+int hk::project_features(const PassGeom& g, int n_blocks, uint64_t seed, float* F_host, hipStream_t s,
+                             char* err, size_t errlen) {
+  const int S = 2 * n_blocks;
+  float *R = nullptr, *mu = nullptr, *F = nullptr;
+  hipError_t e = hipMalloc(&R, sizeof(float) * (int64_t)S * PROJ_D);
+  if (e == hipSuccess) e = hipMalloc(&mu, sizeof(float) * S);
+  if (e == hipSuccess) e = hipMalloc(&F, sizeof(float) * g.I * PROJ_D);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_gauss, dim3(nblk((int64_t)S * PROJ_D)), dim3(256), 0, s, R, (int64_t)S * PROJ_D, seed);
+    hipLaunchKernelGGL(k_snp_mean, dim3(nblk(S)), dim3(256), 0, s, g.gq, g.Ip, g.I, n_blocks, mu);
+    hipLaunchKernelGGL(k_project, dim3((unsigned)((g.I + 63) / 64)), dim3(256), 0, s, g.gq, g.Ip, g.I, n_blocks, mu,
+                       R, 1.0f / sqrtf((float)S), F);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(F_host, F, sizeof(float) * g.I * PROJ_D, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (R) (void)hipFree(R);
+  if (mu) (void)hipFree(mu);
+  if (F) (void)hipFree(F);
+  if (e != hipSuccess) {
+    snprintf(err, errlen, "feature projection: %s", hipGetErrorString(e));
+    return HK_EHIP;
+  }
+  return HK_OK;
+}

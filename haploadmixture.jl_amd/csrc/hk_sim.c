@@ -189,7 +189,94 @@ int hk_sim_init(int64_t I, int64_t J, int K, uint64_t seed, double* q, double* p
   return 0;
 }
 
-/* Dosage sum helper used by tests: number of threads the library was built to use. */
+/*
+ * hk_sim_bed_world: datasets that share one population structure ("world"), for training and evaluating
+ * a learned warm start.  Block j of every dataset is block j of a fixed panel: ancestral pi_j ~ Dir(1,1,1,1)
+ * and a drift realisation D_kj ~ Dir(pi_j (1-F0)/F0), F0 = 0.15, are drawn from world_seed; a dataset with
+ * Fst = F uses P_kj = (1 - lam) pi_j + lam D_kj with lam = sqrt(F / F0) (mean pi_j, variance ~ F pi(1-pi),
+ * the Balding-Nichols moments), so population k means the same thing in every dataset.  Individuals (Q_i ~
+ * Dir(alpha_q)), haplotype draws and missingness come from data_seed.
+ */
+int hk_sim_bed_world(int64_t I, int64_t J, int K, double fst, double miss, double alpha_q, uint64_t world_seed,
+                     uint64_t data_seed, uint8_t* bed, double* q_true, double* p_true) {
+  const double F0 = 0.15;
+  if (I <= 0 || J <= 0 || K <= 0 || K > 64 || !(fst > 0.0 && fst <= F0) || miss < 0.0 || miss >= 1.0 ||
+      !(alpha_q > 0.0) || !bed)
+    return -1;
+  const int64_t bps = (I + 3) / 4;
+  const double lam = sqrt(fst / F0);
+  double* qcum = (double*)malloc(sizeof(double) * (size_t)K * (size_t)I);
+  if (!qcum) return -1;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < I; ++i) {
+    rng_t r = {hash3(data_seed, ST_Q, (uint64_t)i)};
+    double a[64], w[64];
+    for (int k = 0; k < K; ++k) a[k] = alpha_q;
+    rng_dirichlet(&r, a, K, w);
+    double c = 0.0;
+    for (int k = 0; k < K; ++k) {
+      if (q_true) q_true[k + (int64_t)K * i] = w[k];
+      c += w[k];
+      qcum[k + (int64_t)K * i] = c;
+    }
+    qcum[K - 1 + (int64_t)K * i] = 1.0;
+  }
+  const double scale = (1.0 - F0) / F0;
+  const uint32_t miss_thr = (uint32_t)(miss * 4294967296.0);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t j = 0; j < J; ++j) {
+    rng_t r = {hash3(world_seed, ST_P, (uint64_t)j)};
+    double one4[4] = {1.0, 1.0, 1.0, 1.0}, pi[4], al[4];
+    double pcum[64][4];
+    rng_dirichlet(&r, one4, 4, pi);
+    for (int h = 0; h < 4; ++h) al[h] = pi[h] * scale > 1e-300 ? pi[h] * scale : 1e-300;
+    for (int k = 0; k < K; ++k) {
+      double d[4];
+      rng_dirichlet(&r, al, 4, d);
+      double c = 0.0;
+      for (int h = 0; h < 4; ++h) {
+        const double w = (1.0 - lam) * pi[h] + lam * d[h];
+        if (p_true) p_true[k + (int64_t)K * (4 * j + h)] = w;
+        c += w;
+        pcum[k][h] = c;
+      }
+      pcum[k][3] = 1.0;
+    }
+    uint8_t* col1 = bed + (2 * j) * bps;
+    uint8_t* col2 = bed + (2 * j + 1) * bps;
+    memset(col1, 0, (size_t)bps);
+    memset(col2, 0, (size_t)bps);
+    const uint64_t base = hash3(data_seed, ST_G, (uint64_t)j);
+    for (int64_t i = 0; i < I; ++i) {
+      uint64_t h1 = mix64(base + 2 * (uint64_t)i), h2 = mix64(base + 2 * (uint64_t)i + 1);
+      double uz1 = (double)(uint32_t)h1 * 0x1.0p-32, uh1 = (double)(uint32_t)(h1 >> 32) * 0x1.0p-32;
+      double uz2 = (double)(uint32_t)h2 * 0x1.0p-32, uh2 = (double)(uint32_t)(h2 >> 32) * 0x1.0p-32;
+      const double* qc = qcum + (int64_t)K * i;
+      int z1 = 0, z2 = 0;
+      while (z1 < K - 1 && uz1 >= qc[z1]) ++z1;
+      while (z2 < K - 1 && uz2 >= qc[z2]) ++z2;
+      int hap1 = 0, hap2 = 0;
+      while (hap1 < 3 && uh1 >= pcum[z1][hap1]) ++hap1;
+      while (hap2 < 3 && uh2 >= pcum[z2][hap2]) ++hap2;
+      int g1 = (hap1 >> 1) + (hap2 >> 1);
+      int g2 = (hap1 & 1) + (hap2 & 1);
+      uint8_t c1 = (uint8_t)(g1 == 0 ? 0 : g1 == 1 ? 2 : 3);
+      uint8_t c2 = (uint8_t)(g2 == 0 ? 0 : g2 == 1 ? 2 : 3);
+      if (miss_thr) {
+        uint64_t hm = mix64(base ^ (0x5bd1e995ULL + 2 * (uint64_t)i) * 0x2545f4914f6cdd1dULL);
+        if ((uint32_t)hm < miss_thr) c1 = 1;
+        if ((uint32_t)(hm >> 32) < miss_thr) c2 = 1;
+      }
+      const int sh = (int)(i & 3) * 2;
+      col1[i >> 2] |= (uint8_t)(c1 << sh);
+      col2[i >> 2] |= (uint8_t)(c2 << sh);
+    }
+  }
+  free(qcum);
+  return 0;
+}
+
+/* Number of threads the library was built to use. */
 int hk_sim_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

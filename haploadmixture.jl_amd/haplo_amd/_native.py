@@ -42,6 +42,7 @@ class FitOpts(ctypes.Structure):
         ("em_iters", _int), ("n_iter", _int), ("rtol", _dbl), ("Qn", _int),
         ("fix_q", _int), ("fix_p", _int), ("q_penalty", _int), ("p_penalty", _int),
         ("rho", _dbl), ("eps", _dbl), ("verbose", _int), ("profile", _int), ("use_graph", _int),
+        ("em_fix_q", _int),
     ]
 
 
@@ -109,6 +110,18 @@ def hip():
     L.hk_sync.argtypes = [_vp]
     L.hk_time_pass.argtypes = [_vp, _int, _int, ctypes.POINTER(_dbl)]
     L.hk_copy_bench.argtypes = [_vp, _i64, _int, ctypes.POINTER(_dbl)]
+    L.hk_features.argtypes = [_vp, _int, ctypes.c_uint64, _vp]
+    L.hk_mlp_create.argtypes = [ctypes.POINTER(_vp), _int, _int, _int, ctypes.c_uint64]
+    L.hk_mlp_destroy.argtypes = [_vp]
+    L.hk_mlp_last_error.argtypes = [_vp]
+    L.hk_mlp_last_error.restype = ctypes.c_char_p
+    L.hk_mlp_num_params.argtypes = [_vp, ctypes.POINTER(_i64), _vp]
+    L.hk_mlp_get_params.argtypes = [_vp, _vp, _vp]
+    L.hk_mlp_set_params.argtypes = [_vp, _vp]
+    L.hk_mlp_load_data.argtypes = [_vp, _vp, _vp, _i64]
+    L.hk_mlp_forward.argtypes = [_vp, _vp, _i64, _vp]
+    L.hk_mlp_grad.argtypes = [_vp, _i64, _i64, _int, ctypes.POINTER(ctypes.c_float), _vp]
+    L.hk_mlp_adam.argtypes = [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _int]
     _hip = L
     return L
 
@@ -122,6 +135,7 @@ def sim():
     L = ctypes.CDLL(SIM_LIB)
     L.hk_sim_bed.argtypes = [_i64, _i64, _int, _dbl, _dbl, _dbl, ctypes.c_uint64, _vp, _vp, _vp]
     L.hk_sim_init.argtypes = [_i64, _i64, _int, ctypes.c_uint64, _vp, _vp]
+    L.hk_sim_bed_world.argtypes = [_i64, _i64, _int, _dbl, _dbl, _dbl, ctypes.c_uint64, ctypes.c_uint64, _vp, _vp, _vp]
     L.hk_sim_threads.restype = _int
     _sim = L
     return L

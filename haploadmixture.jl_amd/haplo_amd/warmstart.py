@@ -1,0 +1,234 @@
+"""Learned warm start for Q (the reference's ``init_q`` hook, src/driver.jl:56,184-185).
+
+A 3-layer MLP (256 -> 512 -> 512 -> K, ReLU, softmax; BF16 weights, FP32 accumulation) maps a seeded
+Gaussian random projection of an individual's SNP dosages (first ``PANEL_BLOCKS`` blocks of the panel) to
+its ancestry proportions.  It is trained with hand-written kernels (libhaplo_hip.so: ``hk_features``,
+``hk_mlp_*``) on (projection, converged Q) pairs that the plain method produces on simulated datasets of
+one population structure (``sim.simulate_world``), with a permutation-matched cross-entropy (fit labels
+are arbitrary).  The warm-started fit then runs EM on P with Q held at the prediction (``em_fix_q``)
+before the usual QN/SQP loop; random initialisation stays the fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native
+from ._native import HipError, ptr
+from .engine import HipAdmixData, fit_batch
+from .sim import init_params, simulate_world
+
+D_FEAT = 256
+WIDTH = 512
+PANEL_BLOCKS = 20000
+FEATURE_SEED = 77
+WORLD_SEED = 2024
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def features(engine: HipAdmixData, n_blocks: int | None = None, seed: int = FEATURE_SEED) -> np.ndarray:
+    nb = min(engine.J, PANEL_BLOCKS) if n_blocks is None else int(n_blocks)
+    F = np.zeros((engine.I, D_FEAT), dtype=np.float32)
+    engine._check(engine._L.hk_features(engine._h, nb, seed, ptr(F)))
+    return F
+
+
+class MLP:
+    def __init__(self, K: int, din: int = D_FEAT, width: int = WIDTH, seed: int = 1):
+        self._L = _native.hip()
+        self.K, self.din, self.width = int(K), int(din), int(width)
+        h = ctypes.c_void_p()
+        rc = self._L.hk_mlp_create(ctypes.byref(h), self.din, self.width, self.K, seed)
+        if rc != 0:
+            raise HipError(rc, "hk_mlp_create failed")
+        self._h = h
+        n = ctypes.c_int64()
+        self.offsets = np.zeros(6, dtype=np.int64)
+        self._check(self._L.hk_mlp_num_params(self._h, ctypes.byref(n), ptr(self.offsets)))
+        self.nparam = n.value
+        self.step = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise HipError(rc, self._L.hk_mlp_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.hk_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # parameters -------------------------------------------------------------------------------------
+    def params(self, grad=False):
+        w = np.zeros(self.nparam, dtype=np.float32)
+        g = np.zeros(self.nparam, dtype=np.float32) if grad else None
+        self._check(self._L.hk_mlp_get_params(self._h, ptr(w), ptr(g)))
+        return (w, g) if grad else w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != self.nparam:
+            raise ValueError("parameter block size mismatch")
+        self._check(self._L.hk_mlp_set_params(self._h, ptr(w)))
+
+    def unpack(self, flat):
+        """Views (W1, b1, W2, b2, W3, b3) of a flat parameter block."""
+        D, H, K = self.din, self.width, self.K
+        sizes = [(D, H), (H,), (H, H), (H,), (H, K), (K,)]
+        out = []
+        for o, sh in zip(self.offsets, sizes):
+            out.append(flat[o:o + int(np.prod(sh))].reshape(sh))
+        return out
+
+    def save(self, path):
+        np.savez(path, w=self.params(), K=self.K, din=self.din, width=self.width)
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path)
+        m = cls(int(z["K"]), int(z["din"]), int(z["width"]))
+        m.set_params(z["w"])
+        return m
+
+    # compute ------------------------------------------------------------------------------------------
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        P = np.zeros((X.shape[0], self.K), dtype=np.float32)
+        self._check(self._L.hk_mlp_forward(self._h, ptr(X), X.shape[0], ptr(P)))
+        return P
+
+    def load_data(self, X, T):
+        self._X = np.ascontiguousarray(X, dtype=np.float32)
+        self._T = None if T is None else np.ascontiguousarray(T, dtype=np.float32)
+        self._check(self._L.hk_mlp_load_data(self._h, ptr(self._X), ptr(self._T), self._X.shape[0]))
+
+    def grad(self, row0, n, match=True):
+        loss = ctypes.c_float()
+        perm = np.zeros(8, dtype=np.int32)
+        self._check(self._L.hk_mlp_grad(self._h, int(row0), int(n), int(match), ctypes.byref(loss), ptr(perm)))
+        return loss.value, perm[: self.K].copy()
+
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        self.step += 1
+        self._check(self._L.hk_mlp_adam(self._h, lr, b1, b2, eps, self.step))
+
+
+# ------------------------------------------------------------------------------------------------------
+# training data, training, warm-started fits
+# ------------------------------------------------------------------------------------------------------
+
+@dataclass
+class TrainingSet:
+    X: np.ndarray        # (N, 256) features
+    T: np.ndarray        # (N, K) converged Q (rows on the simplex)
+    groups: list         # (row0, n) per dataset
+    fit_seconds: float
+    fsts: list
+
+
+def make_training_set(n_datasets=200, I=2000, J=20000, K=5, fst_range=(0.02, 0.15), miss=0.01, seed=1000,
+                      world_seed=WORLD_SEED, precision="mixed", rtol=1e-6, group=8, log=None) -> TrainingSet:
+    rng = np.random.default_rng(seed)
+    fsts = list(rng.uniform(*fst_range, size=n_datasets))
+    Xs, Ts, groups = [], [], []
+    row = 0
+    t_fit = 0.0
+    for g0 in range(0, n_datasets, group):
+        engs = []
+        for d in range(g0, min(n_datasets, g0 + group)):
+            s = simulate_world(I, J, K, fsts[d], miss=miss, world_seed=world_seed, data_seed=seed + d, truth=False)
+            e = HipAdmixData(s.bed, I, J, K, precision=precision)
+            q0, p0 = init_params(I, J, K, seed=seed + 7919 * d)
+            e.set_params(q0, p0)
+            engs.append(e)
+        t0 = time.perf_counter()
+        fit_batch(engs, [HipAdmixData.default_opts(rtol=rtol, n_iter=500) for _ in engs])
+        t_fit += time.perf_counter() - t0
+        for e in engs:
+            q, _ = e.get_params()
+            Xs.append(features(e))
+            Ts.append(np.ascontiguousarray(q.T, dtype=np.float32))
+            groups.append((row, I))
+            row += I
+            e.close()
+        if log:
+            log(f"training data: {min(n_datasets, g0 + group)}/{n_datasets} datasets")
+    return TrainingSet(np.concatenate(Xs), np.concatenate(Ts), groups, t_fit, fsts)
+
+
+def train(ts: TrainingSet, K: int, epochs=20, batch=500, lr=1e-3, seed=3, log=None) -> tuple:
+    mlp = MLP(K, seed=seed)
+    mlp.load_data(ts.X, ts.T)
+    rng = np.random.default_rng(seed)
+    hist = []
+    steps = [(r0 + b, min(batch, n - b)) for (r0, n) in ts.groups for b in range(0, n, batch)]
+    for ep in range(epochs):
+        order = rng.permutation(len(steps))
+        tot = 0.0
+        for t in order:
+            r0, n = steps[t]
+            loss, _ = mlp.grad(r0, n, match=True)
+            mlp.adam(lr=lr)
+            tot += loss * n
+        hist.append(tot / ts.X.shape[0])
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: loss {hist[-1]:.5f}")
+    return mlp, hist
+
+
+def predict_q(engine: HipAdmixData, mlp: MLP, floor=1e-5) -> np.ndarray:
+    """Q0 (K x I) for the engine's individuals: softmax(MLP(features)), floored and renormalised."""
+    P = mlp.forward(features(engine)).astype(np.float64)
+    P = np.maximum(P, floor)
+    P /= P.sum(axis=1, keepdims=True)
+    return np.asfortranarray(P.T)
+
+
+def fit_warm(engine: HipAdmixData, mlp: MLP, p_seed=11, em_iters=5, **fit_kw):
+    """Warm-started fit: Q0 from the network, P random, EM on P with Q0 held, then QN/SQP on both."""
+    t0 = time.perf_counter()
+    q0 = predict_q(engine, mlp)
+    _, p0 = init_params(engine.I, engine.J, engine.K, seed=p_seed)
+    engine.set_params(q0, p0)
+    t_pred = time.perf_counter() - t0
+    r = engine.fit(em_iters=em_iters, em_fix_q=1, **fit_kw)
+    r["predict_seconds"] = t_pred
+    return r
+
+
+def default_weights_path(K=5, world_seed=WORLD_SEED):
+    return os.path.join(DATA_DIR, f"warmstart_world{world_seed}_K{K}.npz")
+
+
+def time_to_rel(trace, t_trace, ll_ref, rel=1e-6):
+    """(iterations, seconds) at which a fit's log-likelihood first came within rel of ll_ref."""
+    hit = np.nonzero(np.abs(np.asarray(trace) - ll_ref) <= rel * abs(ll_ref))[0]
+    if hit.size == 0:
+        return None, None
+    return int(hit[0]) + 1, float(t_trace[hit[0]])
+
+
+def train_default(out_path=None, n_datasets=200, epochs=20, log=print):
+    """Regenerates the committed weights: 200 datasets (I=2000, J=20,000, K=5, Fst ~ U[0.02, 0.15])."""
+    ts = make_training_set(n_datasets=n_datasets, log=log)
+    mlp, hist = train(ts, 5, epochs=epochs, log=log)
+    out_path = out_path or default_weights_path()
+    os.makedirs(os.path.dirname(out_path), exist_ok=True)
+    mlp.save(out_path)
+    return out_path, hist, ts
+
+
+if __name__ == "__main__":
+    import sys
+
+    path, hist, ts = train_default(out_path=sys.argv[1] if len(sys.argv) > 1 else None)
+    print(f"wrote {path}; final loss {hist[-1]:.5f}; fits {ts.fit_seconds:.1f}s")

@@ -104,6 +104,7 @@ typedef struct {
   int verbose;           /* print per-iteration lines like @info in algorithms_outer.jl:18,162   */
   int profile;           /* time every pass with HIP events (hk_fit_result.t_*)                  */
   int use_graph;         /* replay each QN iteration from a captured hipGraph (ignored if profile) */
+  int em_fix_q;          /* EM warm-up updates P only (em_p!, algorithms_inner.jl:40): keeps a given Q */
 } hk_fit_opts;
 
 typedef struct {
@@ -152,6 +153,31 @@ int hk_sync(hk_ctx* ctx);
  * 2 = update_p pass, 3 = EM pass; reps launches timed with HIP events on the context stream.
  * ms_per_launch receives the mean device time of the pass's main kernel. */
 int hk_time_pass(hk_ctx* ctx, int kind, int reps, double* ms_per_launch);
+
+/* ---- Learned warm start (no counterpart in the reference: its init_q hook, driver.jl:56,184, is fed) ----
+ * Features: f_i = S^-1/2 sum_{s observed} (g_is - mu_s) R_s over the first S = 2 n_blocks SNPs, R a seeded
+ * Gaussian S x 256 matrix, mu_s the observed mean dosage; F is I x 256 row-major (host). */
+int hk_features(hk_ctx* ctx, int n_blocks, uint64_t seed, float* F);
+
+/* MLP din -> width -> width -> dout (ReLU, softmax), BF16 weights, FP32 accumulation, Adam. */
+typedef struct hk_mlp hk_mlp;
+int hk_mlp_create(hk_mlp** out, int din, int width, int dout, uint64_t seed);
+int hk_mlp_destroy(hk_mlp* m);
+const char* hk_mlp_last_error(const hk_mlp* m);
+/* Parameter block: FP32 masters W1 (din x width) b1 W2 (width x width) b2 W3 (width x dout) b3, row-major,
+ * each segment starting at offsets[t] (6 entries); n = total length. */
+int hk_mlp_num_params(hk_mlp* m, int64_t* n, int64_t* offsets);
+int hk_mlp_get_params(hk_mlp* m, float* w, float* grad);
+int hk_mlp_set_params(hk_mlp* m, const float* w);
+/* Keep a training set on the device: X (n x din), T (n x dout) targets (may be NULL for inference only). */
+int hk_mlp_load_data(hk_mlp* m, const float* X, const float* T, int64_t n);
+/* P (n x dout) = softmax(MLP(X)) for host X. */
+int hk_mlp_forward(hk_mlp* m, const float* X, int64_t n, float* P);
+/* Forward + backward on resident rows [row0, row0+n): gradients stay on the device (hk_mlp_get_params);
+ * match != 0 relabels the target columns by the cross-entropy-minimising permutation. */
+int hk_mlp_grad(hk_mlp* m, int64_t row0, int64_t n, int match, float* loss, int* perm);
+/* One Adam step (bias-corrected, step >= 1) on the stored gradients; refreshes the BF16 weights. */
+int hk_mlp_adam(hk_mlp* m, float lr, float beta1, float beta2, float eps, int step);
 
 /* Calibration: copy `bytes` device-to-device `reps` times with a plain streaming kernel (k_copy16, 16 B per
  * lane) on the context stream; gbps = (read + write bytes) / time.  Used to scale HBM byte counters. */
