@@ -7,6 +7,9 @@ the MI355X.  Differences, each forced by the build:
   * ``sparsity`` (SKFR AIM pre-selection, driver.jl:84-131) is not provided;
   * ``T2=np.float32`` selects the MIXED engine (FP32 per-pair products, FP64 accumulation), the
     analogue of the reference's ``T2=Float32`` GPU arrays (driver.jl:149, cuda/structs.jl:19-30).
+Addition: ``warm_start=True`` fills the ``init_q`` hook (driver.jl:56,184-185) with the learned
+initialiser (``warmstart``): Q0 from the shipped network for this K, then EM on P with Q0 held instead
+of ``init_em!``.  Without a network for K, or when ``init_q`` is given, the usual start runs.
 """
 from __future__ import annotations
 
@@ -14,7 +17,7 @@ import math
 
 import numpy as np
 
-from . import plink
+from . import plink, warmstart
 from .engine import HipAdmixData
 from .sim import init_params
 
@@ -40,7 +43,7 @@ class AdmixData2:
 
 def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iters=5, T=np.float64,
                     T2=np.float64, Q=3, use_gpu=True, verbose=False, fix_q=False, fix_p=False,
-                    init_q=None, init_p=None, engine_opts=None):
+                    init_q=None, init_p=None, engine_opts=None, warm_start=False):
     """src/driver.jl:133-201."""
     if not use_gpu:
         raise NotImplementedError("this library is the GPU engine; use_gpu=False has no implementation here")
@@ -62,8 +65,19 @@ def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iter
         if engine_opts:
             base.update(engine_opts)
         ll_start = math.nan
+        mlp = warmstart.load_default(K) if (warm_start and init_q is None) else None
+        if warm_start and init_q is None and mlp is None:
+            print(f"No warm-start network for K={K}; starting from random Q")
+        if mlp is not None:
+            # learned Q0, then 2 * em_iters EM updates of P with Q0 held (each costs one P pass and one
+            # log-likelihood pass, a third of a full EM update), in place of init_em!
+            eng.set_params(warmstart.predict_q(eng, mlp), d.p)
+            mlp.close()
+            r = eng.fit(eng.default_opts(**{**base, "em_iters": max(em_iters, 1) * 2, "em_fix_q": 1, "n_iter": 0,
+                                            "fix_q": 0, "fix_p": 0}))
+            ll_start = r["ll"]
         # init_em! unless both init_q and init_p are given (driver.jl:181-183); it runs without fix_q/fix_p.
-        if init_q is None or init_p is None:
+        elif init_q is None or init_p is None:
             r = eng.fit(eng.default_opts(**{**base, "em_iters": em_iters, "n_iter": 0, "fix_q": 0, "fix_p": 0}))
             ll_start = r["ll"]
         if init_q is not None or init_p is not None:
@@ -90,7 +104,7 @@ def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iter
 def run_admixture(filename, I, J, K, *, rng=1, sparsity=None, prefix=None, skfr_tries=1, skfr_max_inner_iter=50,
                   skfr_mode="global", admix_n_iter=1000, admix_rtol=1e-7, admix_n_em_iters=5, T=np.float64,
                   T2=np.float64, Q=3, use_gpu=True, verbose=False, progress_bar=False, fix_q=False, fix_p=False,
-                  init_q=None, init_p=None):
+                  init_q=None, init_p=None, warm_start=False):
     """src/driver.jl:38-79.  Returns (d, clusters, aims); clusters/aims are None without SKFR."""
     if not str(filename).endswith(".bed"):
         raise ValueError("filename should end with .bed")
@@ -99,5 +113,5 @@ def run_admixture(filename, I, J, K, *, rng=1, sparsity=None, prefix=None, skfr_
         raise NotImplementedError("SKFR AIM pre-selection (sparsity=...) is not part of this engine")
     d = _admixture_base(filename, I, J, K, n_iter=admix_n_iter, rtol=admix_rtol, rng=rng,
                         em_iters=admix_n_em_iters, T=T, T2=T2, Q=Q, use_gpu=use_gpu, verbose=verbose,
-                        fix_q=fix_q, fix_p=fix_p, init_q=init_q, init_p=init_p)
+                        fix_q=fix_q, fix_p=fix_p, init_q=init_q, init_p=init_p, warm_start=warm_start)
     return d, None, None
