@@ -559,6 +559,7 @@ void hk_fit_opts_default(hk_fit_opts* o) {
   o->profile = 0;
   o->use_graph = 1;
   o->em_fix_q = 0;
+  o->guarantee_increase = 0;
 }
 
 }  // extern "C"
@@ -573,7 +574,45 @@ void hk_fit_opts_default(hk_fit_opts* o) {
 // Fixed buffers (x = xb[0], x_next = xb[1], x_next2 = xb[2], x_tmp = xb[3]) and device-resident
 // iteration counter / log-likelihoods make the sequence replayable from a captured hipGraph.
 // ---------------------------------------------------------------------------------------------------
-static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm) {
+// Log-likelihood of (q, p) on the host (one pass + a synchronisation).
+static int eval_ll(hk_ctx* c, const double* q, const double* p, Timer* tm, double* ll) {
+  CHK(run_qpass(c, MODE_LL, q, p, tm, T_LL, c->d_ll + 5));
+  HIPC(c, hipMemcpyAsync(c->h_ll + 5, c->d_ll + 5, sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPC(c, hipStreamSynchronize(c->s));
+  *ll = c->h_ll[5];
+  return HK_OK;
+}
+
+// guarantee_increase line search of update_q! / update_p! (algorithms_inner.jl:170-199, 314-343): out
+// already holds proj(base + delta); try proj(base + f delta) for f = 1, 1/2, ..., 2^-9 and accept the
+// first whose log-likelihood is >= ll_tmp (Q step) or > ll_tmp (P step), raising ll_tmp to it.  When no
+// factor qualifies, out keeps the last one tried (f = 2^-9), as the reference's loop leaves it.
+static int halve_step(hk_ctx* c, const hk_fit_opts* o, Timer* tm, bool is_q, const double* base, const double* delta,
+                      double* out, const double* q_ll, const double* p_ll, double* ll_tmp) {
+  const int64_t n = is_q ? (int64_t)c->K * c->I : (int64_t)c->K * 4 * c->J;
+  double f = 1.0;
+  for (;;) {
+    if (f < 1.0) {
+      HIPC(c, launch_axpy_step(n, base, delta, f, out, c->s));
+      if (is_q) HIPC(c, launch_project_q(c->K, c->I, out, o->eps, c->s));
+      else HIPC(c, launch_project_p(c->K, c->J, out, o->eps, c->s));
+    }
+    double ll;
+    CHK(eval_ll(c, q_ll, p_ll, tm, &ll));
+    if (is_q ? ll >= *ll_tmp : ll > *ll_tmp) {
+      *ll_tmp = ll;
+      return HK_OK;
+    }
+    if (o->verbose)
+      fprintf(stderr, "Step size halved in update_%c. comparison: %.17g, current: %.17g, current step size: %g\n",
+              is_q ? 'q' : 'p', *ll_tmp, ll, f);
+    f /= 2;
+    if (f < 1e-3) return HK_OK;
+  }
+}
+
+// ll_prev: the iteration's starting log-likelihood (needed by guarantee_increase only).
+static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm, double ll_prev = NAN) {
   const int K = c->K;
   const int64_t I = c->I, J = c->J;
   const int64_t N = (int64_t)K * (I + 4 * J), nq = (int64_t)K * I, np = (int64_t)K * 4 * J;
@@ -583,18 +622,26 @@ static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm) {
   double* xt = c->xb[3];
   const StepOpts qso{o->q_penalty, o->rho, o->eps};
   const StepOpts pso{o->p_penalty, o->rho, o->eps};
+  // guarantee_increase: the raw QP steps live in x_tmp (d.q_tmp / d.p_tmp in the reference) until ZAL
+  // overwrites it; every step is then line-searched on the host, so the iteration cannot be captured.
+  const bool gi = o->guarantee_increase != 0;
+  double* dq = gi ? qof(c, xt) : nullptr;
+  double* dp = gi ? pof(c, xt) : nullptr;
+  double ll_tmp = ll_prev;
   auto qstep = [&](const double* q, const double* p, double* q_out) -> int {
     CHK(run_qpass(c, MODE_CURV, q, p, tm, T_QPASS));
     if (tm) tm->begin(T_QSTEP);
-    HIPC(c, launch_q_step(K, I, c->res_q, q, q_out, qso, c->fails, c->s));
+    HIPC(c, launch_q_step(K, I, c->res_q, q, q_out, qso, c->fails, c->s, dq));
     if (tm) tm->end();
+    if (gi) CHK(halve_step(c, o, tm, true, q, dq, q_out, q_out, p, &ll_tmp));
     return HK_OK;
   };
   auto pstep = [&](const double* q, const double* p, double* p_out) -> int {
     CHK(run_ppass(c, MODE_CURV, q, p, tm, T_PPASS));
     if (tm) tm->begin(T_PSTEP);
-    HIPC(c, launch_p_step(K, J, c->res_p, p, p_out, pso, c->fails, c->s));
+    HIPC(c, launch_p_step(K, J, c->res_p, p, p_out, pso, c->fails, c->s, dp));
     if (tm) tm->end();
+    if (gi) CHK(halve_step(c, o, tm, false, p, dp, p_out, q, p_out, &ll_tmp));
     return HK_OK;
   };
   // F(x): q_next from (q, p), p_next from (q_next, p)  (update_q!, update_p!; Gauss-Seidel order)
@@ -637,8 +684,8 @@ static void drop_graph(hk_ctx* c) {
 }
 
 // Capture one iteration into a graph (once per option set) and launch it; eager enqueue otherwise.
-static int launch_iteration(hk_ctx* c, const hk_fit_opts* o, bool use_graph) {
-  if (!use_graph) return enqueue_iteration(c, o, nullptr);
+static int launch_iteration(hk_ctx* c, const hk_fit_opts* o, bool use_graph, double ll_prev) {
+  if (!use_graph || o->guarantee_increase) return enqueue_iteration(c, o, nullptr, ll_prev);
   const unsigned long long key = graph_key(c, o);
   if (!c->graph_exec || c->graph_key != key) {
     drop_graph(c);
@@ -817,10 +864,11 @@ extern "C" int hk_fit_ex(hk_ctx* c, const hk_fit_opts* o, double ll_start, doubl
   f.res = res;
   CHK(fit_prepare(f, ll_start));
   CHK(fit_em(f, ll_start));
-  const bool use_graph = o->use_graph && !o->profile;  // per-kernel events need the eager path
+  // per-kernel events and the step-halving line search need the eager path
+  const bool use_graph = o->use_graph && !o->profile && !o->guarantee_increase;
   while (f.active) {
-    if (use_graph) CHK(launch_iteration(c, o, true));
-    else CHK(enqueue_iteration(c, o, f.tm.on ? &f.tm : nullptr));
+    if (use_graph) CHK(launch_iteration(c, o, true, f.ll_new));
+    else CHK(enqueue_iteration(c, o, f.tm.on ? &f.tm : nullptr, f.ll_new));
     CHK(read_ll(c, 3));
     CHK(fit_after_iteration(f));
   }
@@ -871,7 +919,7 @@ extern "C" int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const
   for (int b = 0; b < n; ++b) active += runs[b].active;
   while (active > 0) {
     for (int b = 0; b < n; ++b)
-      if (runs[b].active) CHK(launch_iteration(ctxs[b], &opts[b], opts[b].use_graph != 0));
+      if (runs[b].active) CHK(launch_iteration(ctxs[b], &opts[b], opts[b].use_graph != 0, runs[b].ll_new));
     for (int b = 0; b < n; ++b)
       if (runs[b].active) {
         CHK(read_ll(ctxs[b], 3));

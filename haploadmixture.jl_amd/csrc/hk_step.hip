@@ -291,7 +291,7 @@ __device__ int wave_qp(const double (&Hrow)[N], double g, double lo, double hi, 
 template <int K, bool PEN>
 __global__ __launch_bounds__(64) void k_q_step(int64_t I, const double* __restrict__ res, const double* __restrict__ q,
                                                double* __restrict__ q_next, double rho, double eps,
-                                               int* __restrict__ fails) {
+                                               int* __restrict__ fails, double* __restrict__ dq) {
   constexpr bool penalty = PEN;
   constexpr int NT = K + K * (K + 1) / 2;
   __shared__ double lds[K * K > 64 ? K * K : 64];
@@ -318,6 +318,7 @@ __global__ __launch_bounds__(64) void k_q_step(int64_t I, const double* __restri
   else it = wave_qp<K, 1>(Hrow, g, lo, hi, 0, x, lds);
   if (it < 0) x = 0.0;
   if (it < 0 && lane == 0) atomicAdd(fails, 1);
+  if (dq && lane < K) dq[lane + (int64_t)K * i] = x;  // raw step, for step halving
   // q_next = q + dq, then project_q! (lane 0 gathers the K values).
   double v[K];
 #pragma unroll
@@ -335,7 +336,7 @@ __global__ __launch_bounds__(64) void k_q_step(int64_t I, const double* __restri
 template <int K, bool PEN>
 __global__ __launch_bounds__(64) void k_p_step(int64_t J, const double* __restrict__ res, const double* __restrict__ p,
                                                double* __restrict__ p_next, double rho, double eps,
-                                               int* __restrict__ fails) {
+                                               int* __restrict__ fails, double* __restrict__ dp) {
   constexpr bool penalty = PEN;
   constexpr int NT = K + K * (K + 1) / 2;
   constexpr int N = 4 * K;
@@ -370,6 +371,7 @@ __global__ __launch_bounds__(64) void k_p_step(int64_t J, const double* __restri
   else it = wave_qp<N, K>(Hrow, g, lo, hi, k, x, lds);
   if (it < 0) x = 0.0;
   if (it < 0 && lane == 0) atomicAdd(fails, 1);
+  if (dp && act) dp[k + (int64_t)K * (4 * j + l)] = x;  // raw step, for step halving
   // p_next = p + dp, then project_p! on each (k, j) 4-vector: lane k gathers haplotypes 0..3.
   const double nv = pv + x;
   double v[4];
@@ -624,28 +626,28 @@ static hipError_t with_k(int K, F&& f) {
 }
 
 hipError_t launch_q_step(int K, int64_t I, const double* res_q, const double* q, double* q_next, const StepOpts& o,
-                         int* fails, hipStream_t s) {
+                         int* fails, hipStream_t s, double* dq) {
   return with_k(K, [&](auto kc) {
     constexpr int KT = decltype(kc)::value;
     if (o.penalty)
       hipLaunchKernelGGL((k_q_step<KT, true>), dim3((unsigned)I), dim3(64), 0, s, I, res_q, q, q_next, o.rho, o.eps,
-                         fails);
+                         fails, dq);
     else
       hipLaunchKernelGGL((k_q_step<KT, false>), dim3((unsigned)I), dim3(64), 0, s, I, res_q, q, q_next, o.rho, o.eps,
-                         fails);
+                         fails, dq);
   });
 }
 
 hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
-                         int* fails, hipStream_t s) {
+                         int* fails, hipStream_t s, double* dp) {
   return with_k(K, [&](auto kc) {
     constexpr int KT = decltype(kc)::value;
     if (o.penalty)
       hipLaunchKernelGGL((k_p_step<KT, true>), dim3((unsigned)J), dim3(64), 0, s, J, res_p, p, p_next, o.rho, o.eps,
-                         fails);
+                         fails, dp);
     else
       hipLaunchKernelGGL((k_p_step<KT, false>), dim3((unsigned)J), dim3(64), 0, s, J, res_p, p, p_next, o.rho, o.eps,
-                         fails);
+                         fails, dp);
   });
 }
 

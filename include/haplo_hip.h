@@ -105,6 +105,8 @@ typedef struct {
   int profile;           /* time every pass with HIP events (hk_fit_result.t_*)                  */
   int use_graph;         /* replay each QN iteration from a captured hipGraph (ignored if profile) */
   int em_fix_q;          /* EM warm-up updates P only (em_p!, algorithms_inner.jl:40): keeps a given Q */
+  int guarantee_increase; /* step halving of every Q/P step until the log-likelihood does not drop
+                             (update_q!/update_p!, algorithms_inner.jl:170-199,314-343); eager path */
 } hk_fit_opts;
 
 typedef struct {

@@ -545,8 +545,22 @@ int hko_qp_solve(int n, const double* H, const double* g, const double* lo, cons
 /* update_q! QP stage: XtX, Xtz as returned by the pass (Xtz NOT negated).  q_next = proj(q + dq).
  * penalty != 0 uses H = XtX_i + rho 11' with no equality (the intended reading of :158-162; the
  * reference adds rho to the whole XtX array once per row, which depends on thread scheduling). */
+static int q_step_core(int K, int64_t I, const double* XtX, const double* Xtz, const double* q, double* q_next,
+                       int penalty, double rho, double eps, double* dq);
+
 int hko_q_step(int K, int64_t I, const double* XtX, const double* Xtz, const double* q, double* q_next,
                int penalty, double rho, double eps) {
+  return q_step_core(K, I, XtX, Xtz, q, q_next, penalty, rho, eps, NULL);
+}
+
+int hko_q_step_ex(int K, int64_t I, const double* XtX, const double* Xtz, const double* q, double* q_next,
+                  int penalty, double rho, double eps, double* dq) {
+  return q_step_core(K, I, XtX, Xtz, q, q_next, penalty, rho, eps, dq);
+}
+
+/* dq (may be NULL) receives the raw step qdiff (d.q_tmp, algorithms_inner.jl:101). */
+static int q_step_core(int K, int64_t I, const double* XtX, const double* Xtz, const double* q, double* q_next,
+                       int penalty, double rho, double eps, double* dq) {
   int fails = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : fails)
   for (int64_t i = 0; i < I; ++i) {
@@ -569,6 +583,8 @@ int hko_q_step(int K, int64_t I, const double* XtX, const double* Xtz, const dou
       for (int a = 0; a < K; ++a) x[a] = 0.0;
     }
     for (int a = 0; a < K; ++a) q_next[a + (int64_t)K * i] = q[a + (int64_t)K * i] + x[a];
+    if (dq)
+      for (int a = 0; a < K; ++a) dq[a + (int64_t)K * i] = x[a];
   }
   hko_project_q(q_next, K, I, eps);
   return fails;
@@ -577,8 +593,22 @@ int hko_q_step(int K, int64_t I, const double* XtX, const double* Xtz, const dou
 /* update_p! QP stage: 4K x 4K block-diagonal curvature (:272-283), default penalty rho on every
  * (l, l2) block diagonal (:295-302), box [0, 1] on p + dp; without penalty, K equalities
  * sum_l dp[l, k] = 0.  Variables are ordered (l-1)K + k as in pv[j] (structs.jl:218). */
+static int p_step_core(int K, int64_t J, const double* XtX, const double* Xtz, const double* p, double* p_next,
+                       int penalty, double rho, double eps, double* dp);
+
 int hko_p_step(int K, int64_t J, const double* XtX, const double* Xtz, const double* p, double* p_next,
                int penalty, double rho, double eps) {
+  return p_step_core(K, J, XtX, Xtz, p, p_next, penalty, rho, eps, NULL);
+}
+
+int hko_p_step_ex(int K, int64_t J, const double* XtX, const double* Xtz, const double* p, double* p_next,
+                  int penalty, double rho, double eps, double* dp) {
+  return p_step_core(K, J, XtX, Xtz, p, p_next, penalty, rho, eps, dp);
+}
+
+/* dp (may be NULL) receives the raw step pdiff (d.p_tmp, algorithms_inner.jl:228). */
+static int p_step_core(int K, int64_t J, const double* XtX, const double* Xtz, const double* p, double* p_next,
+                       int penalty, double rho, double eps, double* dp) {
   int fails = 0;
   const int n = 4 * K;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : fails)
@@ -612,6 +642,8 @@ int hko_p_step(int K, int64_t J, const double* XtX, const double* Xtz, const dou
       for (int t = 0; t < n; ++t) x[t] = 0.0;
     }
     for (int t = 0; t < n; ++t) p_next[(int64_t)K * 4 * j + t] = pj[t] + x[t];
+    if (dp)
+      for (int t = 0; t < n; ++t) dp[(int64_t)K * 4 * j + t] = x[t];
   }
   hko_project_p(p_next, K, J, eps);
   return fails;
@@ -721,7 +753,36 @@ typedef struct {
   int q_penalty, p_penalty;
   double rho;        /* 1e7 */
   double eps;        /* pseudocount 1e-5 */
+  int guarantee_increase; /* admixture_qn!(...; guarantee_increase), algorithms_outer.jl:41 */
 } hko_fit_opts;
+
+/* Step-halving loop of update_q! / update_p! with guarantee_increase (algorithms_inner.jl:170-199 and
+ * 314-343), after the QP stage left out = proj(base + delta) for factor 1:
+ *   factor = 1; loop { out = proj(base + factor delta); ll = loglik(q_ll, p_ll);
+ *                      if ll >= ll_tmp (Q) / ll > ll_tmp (P) { ll_tmp = ll; break }
+ *                      factor /= 2; if factor < 1e-3 { factor = 0 }; if factor == 0 break }
+ * so when no factor qualifies, out keeps the last one tried (2^-9) and ll_tmp is unchanged. */
+static void halve_step(const uint8_t* bed, int64_t bps, int64_t I, int64_t J, int K, int is_q, const double* base,
+                       const double* delta, double* out, const double* q_ll, const double* p_ll, double* ll_tmp,
+                       double eps) {
+  const int64_t n = is_q ? (int64_t)K * I : (int64_t)K * 4 * J;
+  double factor = 1.0;
+  for (;;) {
+    if (factor < 1.0) {
+      for (int64_t t = 0; t < n; ++t) out[t] = base[t] + factor * delta[t];
+      if (is_q) hko_project_q(out, K, I, eps);
+      else hko_project_p(out, K, J, eps);
+    }
+    const double ll = hko_loglik(bed, bps, I, J, K, q_ll, p_ll);
+    if (is_q ? ll >= *ll_tmp : ll > *ll_tmp) {
+      *ll_tmp = ll;
+      return;
+    }
+    factor /= 2;
+    if (factor < 1e-3) factor = 0.0;
+    if (factor == 0.0) return;
+  }
+}
 
 /* x is K x (I + 4J): q = x[:, 0:I], p = x[:, I:]; q and p are updated in place.
  * ll_trace (may be NULL) receives ll_new after each QN iteration (length n_iter).
@@ -763,26 +824,34 @@ int hko_fit(const uint8_t* bed, int64_t bps, int64_t I, int64_t J, int K, const 
   }
   if (isnan(ll_new)) ll_new = hko_loglik(bed, bps, I, J, K, q, p);
   int iters = 0;
+  const int gi = o->guarantee_increase;
+  double* dq = gi ? q_tmp : NULL; /* qdiff / pdiff live in x_tmp until ZAL overwrites it */
+  double* dp = gi ? p_tmp : NULL;
   for (int i = 1; i <= o->n_iter; ++i) {
     iters = i;
     const double ll_prev = ll_new;
+    double ll_tmp = ll_new;
     /* first sweep: F(x) */
     if (!o->fix_q) {
       hko_update_q(bed, bps, I, J, K, q, p, XtXq, Xtzq);
-      hko_q_step(K, I, XtXq, Xtzq, q, q_next, o->q_penalty, o->rho, o->eps);
+      q_step_core(K, I, XtXq, Xtzq, q, q_next, o->q_penalty, o->rho, o->eps, dq);
+      if (gi) halve_step(bed, bps, I, J, K, 1, q, dq, q_next, q_next, p, &ll_tmp, o->eps);
     } else memcpy(q_next, q, sizeof(double) * K * I);
     if (!o->fix_p) {
       hko_update_p(bed, bps, I, J, K, q_next, p, XtXp, Xtzp);
-      hko_p_step(K, J, XtXp, Xtzp, p, p_next, o->p_penalty, o->rho, o->eps);
+      p_step_core(K, J, XtXp, Xtzp, p, p_next, o->p_penalty, o->rho, o->eps, dp);
+      if (gi) halve_step(bed, bps, I, J, K, 0, p, dp, p_next, q_next, p_next, &ll_tmp, o->eps);
     } else memcpy(p_next, p, sizeof(double) * K * 4 * J);
     /* second sweep: F(F(x)) */
     if (!o->fix_q) {
       hko_update_q(bed, bps, I, J, K, q_next, p_next, XtXq, Xtzq);
-      hko_q_step(K, I, XtXq, Xtzq, q_next, q_next2, o->q_penalty, o->rho, o->eps);
+      q_step_core(K, I, XtXq, Xtzq, q_next, q_next2, o->q_penalty, o->rho, o->eps, dq);
+      if (gi) halve_step(bed, bps, I, J, K, 1, q_next, dq, q_next2, q_next2, p_next, &ll_tmp, o->eps);
     } else memcpy(q_next2, q, sizeof(double) * K * I);
     if (!o->fix_p) {
       hko_update_p(bed, bps, I, J, K, q_next2, p_next, XtXp, Xtzp);
-      hko_p_step(K, J, XtXp, Xtzp, p_next, p_next2, o->p_penalty, o->rho, o->eps);
+      p_step_core(K, J, XtXp, Xtzp, p_next, p_next2, o->p_penalty, o->rho, o->eps, dp);
+      if (gi) halve_step(bed, bps, I, J, K, 0, p_next, dp, p_next2, q_next2, p_next2, &ll_tmp, o->eps);
     } else memcpy(p_next2, p, sizeof(double) * K * 4 * J);
     const double ll_basic = hko_loglik(bed, bps, I, J, K, q_next2, p_next2);
     hko_zal(N, o->Qn, i, U, V, x, x_next, x_next2, x_tmp);

@@ -45,6 +45,7 @@ class FitOpts(ctypes.Structure):
         ("p_penalty", _int),
         ("rho", _dbl),
         ("eps", _dbl),
+        ("guarantee_increase", _int),
     ]
 
 
@@ -76,6 +77,10 @@ def lib():
         L.hko_q_step.restype = _int
         L.hko_p_step.argtypes = [_int, _i64, _f64p_f, _f64p_f, _f64p_f, _f64p_f, _int, _dbl, _dbl]
         L.hko_p_step.restype = _int
+        L.hko_q_step_ex.argtypes = [_int, _i64, _f64p_f, _f64p_f, _f64p_f, _f64p_f, _int, _dbl, _dbl, _f64p_f]
+        L.hko_q_step_ex.restype = _int
+        L.hko_p_step_ex.argtypes = [_int, _i64, _f64p_f, _f64p_f, _f64p_f, _f64p_f, _int, _dbl, _dbl, _f64p_f]
+        L.hko_p_step_ex.restype = _int
         L.hko_em_normalize.argtypes = [_int, _i64, _i64, _i64p, _f64p_f, _f64p_f, _int, _int, _dbl]
         L.hko_zal.argtypes = [_i64, _int, _int, _f64p_f, _f64p_f, _f64p, _f64p, _f64p, _f64p]
         L.hko_zal.restype = _int
@@ -193,6 +198,25 @@ def p_step(XtX, Xtz, p, penalty=True, rho=1e7, eps=1e-5):
     return pn, fails
 
 
+def q_step_delta(XtX, Xtz, q, penalty=False, rho=1e7, eps=1e-5):
+    """q_step plus the raw step qdiff before projection: (q_next, qdiff, fails)."""
+    K, I = q.shape
+    qn = np.zeros((K, I), order="F")
+    dq = np.zeros((K, I), order="F")
+    fails = lib().hko_q_step_ex(K, I, _F(XtX), _F(Xtz), _F(q), qn, int(penalty), rho, eps, dq)
+    return qn, dq, fails
+
+
+def p_step_delta(XtX, Xtz, p, penalty=True, rho=1e7, eps=1e-5):
+    """p_step plus the raw step pdiff before projection: (p_next, pdiff, fails)."""
+    K = p.shape[0]
+    J = p.shape[1] // 4
+    pn = np.zeros(p.shape, order="F")
+    dp = np.zeros(p.shape, order="F")
+    fails = lib().hko_p_step_ex(K, J, _F(XtX), _F(Xtz), _F(p), pn, int(penalty), rho, eps, dp)
+    return pn, dp, fails
+
+
 def em_normalize(qn, pn, dm, J, eps=1e-5):
     """em! normalisation + projections (algorithms_inner.jl:66-76)."""
     qn = _F(qn).copy(order="F")
@@ -212,11 +236,14 @@ def zal(U, V, x0, x1, x2, it, Qn=3):
 
 
 def fit(bed, I, J, q0, p0, em_iters=5, n_iter=1000, rtol=1e-7, Qn=3, fix_q=False, fix_p=False,
-        q_penalty=False, p_penalty=True, rho=1e7, eps=1e-5):
-    """init_em! + admixture_qn! (algorithms_outer.jl:1-172).  Returns (q, p, ll, iters, trace)."""
+        q_penalty=False, p_penalty=True, rho=1e7, eps=1e-5, guarantee_increase=False):
+    """init_em! + admixture_qn! (algorithms_outer.jl:1-172), with the step halving of update_q! /
+    update_p! when guarantee_increase (algorithms_inner.jl:170-199,314-343).
+    Returns (q, p, ll, iters, trace)."""
     K = q0.shape[0]
     x = np.asfortranarray(np.concatenate([q0, p0], axis=1))
-    o = FitOpts(em_iters, n_iter, rtol, Qn, int(fix_q), int(fix_p), int(q_penalty), int(p_penalty), rho, eps)
+    o = FitOpts(em_iters, n_iter, rtol, Qn, int(fix_q), int(fix_p), int(q_penalty), int(p_penalty), rho, eps,
+                int(guarantee_increase))
     ll = _dbl(0.0)
     trace = np.zeros(max(n_iter, 1))
     iters = lib().hko_fit(bed, _bps(I), I, J, K, ctypes.byref(o), x, ctypes.byref(ll), trace)

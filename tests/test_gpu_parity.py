@@ -118,6 +118,43 @@ def test_fit_first_iterations_track_oracle(oracle):
     assert rel(r["trace"], otr) <= 1e-10
 
 
+@pytest.mark.parametrize("opts", [dict(fix_q=True), dict(fix_p=True), dict(q_penalty=True),
+                                  dict(p_penalty=False), dict(guarantee_increase=True)],
+                         ids=["fix_q", "fix_p", "q_penalty", "p_equality", "guarantee_increase"])
+def test_fit_modes_track_oracle(opts, oracle):
+    """admixture_qn! modes (fix_q / fix_p: algorithms_outer.jl:71-116; penalty variants: :58-64 and
+    algorithms_inner.jl:158-168,295-343; guarantee_increase: algorithms_inner.jl:170-199,314-343),
+    iteration by iteration against the CPU loop with the same options (FP64 engine)."""
+    print(f"fit mode {opts}", flush=True)
+    I, J, K = 150, 120, 4
+    s, q, p = case(I, J, K, fst=0.1, miss=0.02)
+    _, _, _, _, otr = oracle.fit(s.bed, I, J, q, p, n_iter=6, rtol=0.0, **opts)
+    o = {k: int(v) for k, v in opts.items()}
+    with _eng(s, q, p) as e:
+        r = e.fit(n_iter=6, rtol=0.0, **o)
+    assert r["qp_failures"] == 0
+    assert len(r["trace"]) == 6
+    assert rel(r["trace"], otr) <= 1e-10
+
+
+def test_guarantee_increase_past_convergence(oracle):
+    """Step halving where it acts: 30 iterations of a tiny problem run past convergence, so that steps
+    are at round-off scale and halvings / exhausted searches occur (the oracle's composition test counts
+    them).  Halving decisions compare log-likelihoods to the last bit, so device and CPU may take
+    different factors there; the log-likelihood trace agrees to 1e-10 and parameters to 1e-6."""
+    I, J, K = 30, 20, 2
+    s, q, p = case(I, J, K, miss=0.05, fst=0.1)
+    oq, op, oll, oit, otr = oracle.fit(s.bed, I, J, q, p, em_iters=0, n_iter=30, rtol=0.0, guarantee_increase=True)
+    with _eng(s, q, p) as e:
+        r = e.fit(em_iters=0, n_iter=30, rtol=0.0, guarantee_increase=1, verbose=1)
+        gq, gp = e.get_params()
+    assert len(r["trace"]) == 30
+    assert rel(r["trace"], otr) <= 1e-10
+    assert np.max(np.abs(gq - oq)) <= 1e-6 and np.max(np.abs(gp - op)) <= 1e-6
+    tr = np.concatenate([[r["ll_init"]], r["trace"]])
+    assert np.all(np.diff(tr) >= -1e-12 * abs(tr[-1]))
+
+
 def test_deterministic_repeat():
     """Atomics-free reductions: two runs are bitwise identical."""
     s, q, p = case(1000, 800, 5)

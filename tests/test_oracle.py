@@ -221,3 +221,80 @@ def test_equality_qp_semidefinite_curvature(oracle, brute):
         xb = brute.qp_bruteforce(H, g, lo, hi, grp)
         assert it >= 0
         assert np.max(np.abs(x - xb)) <= 1e-8, (t, x, xb)
+
+
+def _gi_fit_py(oracle, bed, I, J, q, p, n_iter, Qn=3, eps=1e-5):
+    """admixture_qn! with guarantee_increase (algorithms_outer.jl:65-171; algorithms_inner.jl:170-199,
+    314-343), composed in Python from the oracle's single-step pieces; returns (q, p, trace)."""
+    K = q.shape[0]
+    N = K * (I + 4 * J)
+    U = np.zeros((N, Qn), order="F")
+    V = np.zeros((N, Qn), order="F")
+
+    def halve(base, delta, out, llf, ll_tmp, is_q):
+        f = 1.0
+        while True:
+            if f < 1.0:
+                halvings[is_q] += 1
+                out = oracle.project_q(base + f * delta) if is_q else oracle.project_p(base + f * delta)
+            ll = llf(out)
+            if (ll >= ll_tmp) if is_q else (ll > ll_tmp):
+                return out, ll
+            f /= 2
+            if f < 1e-3:
+                return out, ll_tmp
+
+    halvings = {True: 0, False: 0}
+    ll = lambda a, b: oracle.loglik(bed, I, J, a, b)  # noqa: E731
+    ll_new = ll(q, p)
+    trace = []
+    for it in range(1, n_iter + 1):
+        ll_tmp = ll_new
+        qn, dq, _ = oracle.q_step_delta(*oracle.update_q(bed, I, J, q, p), q)
+        qn, ll_tmp = halve(q, dq, qn, lambda z: ll(z, p), ll_tmp, True)
+        pn, dp, _ = oracle.p_step_delta(*oracle.update_p(bed, I, J, qn, p), p)
+        pn, ll_tmp = halve(p, dp, pn, lambda z: ll(qn, z), ll_tmp, False)
+        qn2, dq, _ = oracle.q_step_delta(*oracle.update_q(bed, I, J, qn, pn), qn)
+        qn2, ll_tmp = halve(qn, dq, qn2, lambda z: ll(z, pn), ll_tmp, True)
+        pn2, dp, _ = oracle.p_step_delta(*oracle.update_p(bed, I, J, qn2, pn), pn)
+        pn2, ll_tmp = halve(pn, dp, pn2, lambda z: ll(qn2, z), ll_tmp, False)
+        ll_basic = ll(qn2, pn2)
+        flat = lambda a, b: np.concatenate([a, b], axis=1).ravel(order="F")  # noqa: E731
+        xt, _ = oracle.zal(U, V, flat(q, p), flat(qn, pn), flat(qn2, pn2), it, Qn)
+        xt = xt.reshape(K, I + 4 * J, order="F")
+        qt, pt = oracle.project_q(xt[:, :I]), oracle.project_p(xt[:, I:])
+        ll_qn = ll(qt, pt)
+        if ll_basic < ll_qn:
+            q, p, ll_new = qt, pt, ll_qn
+        else:
+            q, p, ll_new = qn2, pn2, ll_basic
+        trace.append(ll_new)
+    return q, p, np.array(trace), halvings
+
+
+@pytest.mark.parametrize("I,J,K,n", [(40, 30, 3, 4), (30, 20, 2, 30)])
+def test_guarantee_increase_loop_matches_composition(oracle, I, J, K, n):
+    """hko_fit(guarantee_increase) against the same loop composed from the single-step pieces: pins which
+    iterate, which step and which comparison (>= for Q, > for P) every halving uses.  The 30-iteration
+    case runs past convergence, where steps are tiny and halvings (and exhausted searches) happen."""
+    s, q, p = case(I, J, K, miss=0.05, fst=0.1)
+    qc, pc, llc, iters, trc = oracle.fit(s.bed, I, J, q, p, em_iters=0, n_iter=n, rtol=0.0, guarantee_increase=True)
+    qr, pr, trr, halvings = _gi_fit_py(oracle, s.bed, I, J, q, p, n)
+    assert iters == n
+    assert rel(trc, trr) <= 1e-12
+    assert rel(qc, qr) <= 1e-10 and rel(pc, pr) <= 1e-10
+    if n > 10:
+        assert halvings[True] > 0 and halvings[False] > 0
+
+
+def test_guarantee_increase_never_decreases(oracle):
+    """With step halving every accepted iterate is at least as likely as the last (from a random start,
+    no EM warm-up, where unguarded SQP steps can overshoot)."""
+    for seed in (3, 4, 5):
+        I, J, K = 50, 40, 4
+        s, q, p = case(I, J, K, miss=0.02, fst=0.2, seed=seed)
+        _, _, ll, iters, tr = oracle.fit(s.bed, I, J, q, p, em_iters=0, n_iter=30, rtol=1e-10,
+                                         guarantee_increase=True)
+        ll0 = oracle.loglik(s.bed, I, J, q, p)
+        full = np.concatenate([[ll0], tr])
+        assert np.all(np.diff(full) >= -1e-12 * abs(ll)), seed
