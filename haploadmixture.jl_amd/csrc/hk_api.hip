@@ -86,11 +86,17 @@ static int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 static int ntq(int K) { return K + K * (K + 1) / 2; }
 
 // Chunking of the passes so that one launch has >= ~2048 workgroups (8 per CU on 256 CUs).
+// Default chunking: enough workgroups for ~32 per CU on 256 CUs (the passes keep 2-4 resident per CU, so
+// the last round is a small fraction), but at least 16 genotype words per chunk so that the per-chunk
+// partial sums stay small next to the genotype stream.
 static void plan_chunks(hk_ctx* c) {
   PassGeom& g = c->g;
+  constexpr int64_t kTargetWG = 8192, kMinWords = 16;
   const int64_t wgq = g.Ip / 256, wgp = 4 * g.Jp / 256;
-  int64_t qc = c->qchunks_req > 0 ? c->qchunks_req : (2048 + wgq - 1) / wgq;
-  int64_t pc = c->pchunks_req > 0 ? c->pchunks_req : (2048 + wgp - 1) / wgp;
+  int64_t qc = c->qchunks_req > 0 ? c->qchunks_req : (kTargetWG + wgq - 1) / wgq;
+  int64_t pc = c->pchunks_req > 0 ? c->pchunks_req : (kTargetWG + wgp - 1) / wgp;
+  if (c->qchunks_req <= 0 && qc > g.J8 / kMinWords) qc = g.J8 / kMinWords;
+  if (c->pchunks_req <= 0 && pc > g.I8 / kMinWords) pc = g.I8 / kMinWords;
   if (qc < 1) qc = 1;
   if (qc > g.J8) qc = g.J8;
   if (pc < 1) pc = 1;

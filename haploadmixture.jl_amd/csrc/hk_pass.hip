@@ -407,12 +407,17 @@ __global__ void k_make_outer(const TP* __restrict__ x, int64_t ncol, int64_t nco
 
 // P side: a wave owns 16 blocks (A rows); each MFMA step takes 4 individuals (A columns / B rows).
 // The workgroup (64 blocks) stages q and the q q' products of 32 individuals (4 genotype words) in LDS.
-template <int K, typename TP>
-__global__ __launch_bounds__(256) void k_pside_mm(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8,
-                                                  int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
-                                                  const TP* __restrict__ qq, double* __restrict__ out) {
+// PF: staging double-buffered through registers (the global loads of stage s+1 -- q, q q' and the
+// genotype words -- are issued before stage s is computed and written to LDS after it); without PF the
+// loads of a stage are issued together at its start, which keeps fewer registers live (K >= 7).
+template <int K, typename TP, bool PF>
+__device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8, int wpc,
+                                              const TP* __restrict__ q, const TP* __restrict__ p,
+                                              const TP* __restrict__ qq, double* __restrict__ out) {
   constexpr int NKK = K * (K + 1) / 2, NT = K + NKK, NTILE = (NKK + 15) / 16, QS = 16 * NTILE;
   constexpr int WPS = 4;  // genotype words per LDS stage
+  constexpr int NQ = 8 * WPS * K, NQQ = 8 * WPS * QS;
+  constexpr int LQ = (NQ + 255) / 256, LQQ = (NQQ + 255) / 256;
   constexpr bool F32 = sizeof(TP) == 4;
   using M = Mfma<TP>;
   using V4 = typename M::v4;
@@ -439,62 +444,96 @@ __global__ __launch_bounds__(256) void k_pside_mm(const uint32_t* __restrict__ g
 #pragma unroll
       for (int v = 0; v < 4; ++v) D[h][t][v] = 0.0;
     }
+  // registers of the next stage: staged individuals i0 .. i0 + 8 WPS - 1 are clamped to I - 1
+  TP rq[LQ], rqq[LQQ];
+  uint32_t wnext[WPS] = {};
+  auto fetch = [&](int iws_) {
+    const int64_t i0 = (int64_t)iws_ * 8;
+    const int nw_ = min(WPS, iw1 - iws_);
+#pragma unroll
+    for (int u = 0; u < LQ; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      const int r = e / K, k = e - r * K;
+      int64_t ii = i0 + r;
+      ii = ii < I ? ii : I - 1;
+      rq[u] = e < NQ ? q[k + (int64_t)K * ii] : (TP)0;
+    }
+#pragma unroll
+    for (int u = 0; u < LQQ; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      const int r = e / QS, n = e - r * QS;
+      int64_t ii = i0 + r;
+      ii = ii < I ? ii : I - 1;
+      rqq[u] = e < NQQ ? qq[ii * QS + n] : (TP)0;
+    }
+    if constexpr (PF) {
+#pragma unroll
+      for (int w = 0; w < WPS; ++w) wnext[w] = w < nw_ ? gp[(int64_t)(iws_ + w) * Jp + j] : 0u;
+    }
+  };
+  if (PF && iw0 < iw1) fetch(iw0);
   int nword = 0;
   for (int iws = iw0; iws < iw1; iws += WPS) {
     const int nw = min(WPS, iw1 - iws);
     __syncthreads();
-    {
-      const int64_t i0 = (int64_t)iws * 8;  // individuals i0 .. i0 + 8 WPS - 1 (clamped to I - 1)
-      for (int e = threadIdx.x; e < 8 * WPS * K; e += 256) {
-        const int r = e / K, k = e - r * K;
-        int64_t ii = i0 + r;
-        ii = ii < I ? ii : I - 1;
-        qs[r][k] = q[k + (int64_t)K * ii];
-      }
-      for (int e = threadIdx.x; e < 8 * WPS * QS; e += 256) {
-        const int r = e / QS, n = e - r * QS;
-        int64_t ii = i0 + r;
-        ii = ii < I ? ii : I - 1;
-        qqs[r][n] = qq[ii * QS + n];
-      }
+    if (!PF) fetch(iws);
+#pragma unroll
+    for (int u = 0; u < LQ; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      if (e < NQ) (&qs[0][0])[e] = rq[u];
     }
+#pragma unroll
+    for (int u = 0; u < LQQ; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      if (e < NQQ) (&qqs[0][0])[e] = rqq[u];
+    }
+    uint32_t wcur[WPS];
+#pragma unroll
+    for (int w = 0; w < WPS; ++w) wcur[w] = wnext[w];
     __syncthreads();
-    for (int w = 0; w < nw; ++w) {
-      const uint32_t word = gp[(int64_t)(iws + w) * Jp + j];
+    if (PF && iws + WPS < iw1) fetch(iws + WPS);
+    auto do_word = [&](int w, uint32_t word) {
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const int r = 8 * w + 4 * s + g;  // staged individual of this lane's A column
-        TP X[4];
+        for (int s = 0; s < 2; ++s) {
+          const int r = 8 * w + 4 * s + g;  // staged individual of this lane's A column
+          TP X[4];
 #pragma unroll
-        for (int h = 0; h < 4; ++h) {
-          TP x = (TP)0;
+          for (int h = 0; h < 4; ++h) {
+            TP x = (TP)0;
 #pragma unroll
-          for (int k = 0; k < K; ++k) x = hk_fma(qs[r][k], P[h][k], x);
-          X[h] = x;
-        }
-        TP v[4];
-        pair_weights<TP>(X, (word >> (4 * (4 * s + g))) & 15u, v);
-        TP b[NTILE];
+            for (int k = 0; k < K; ++k) x = hk_fma(qs[r][k], P[h][k], x);
+            X[h] = x;
+          }
+          TP v[4];
+          pair_weights<TP>(X, (word >> (4 * (4 * s + g))) & 15u, v);
+          TP b[NTILE];
 #pragma unroll
-        for (int t = 0; t < NTILE; ++t) b[t] = qqs[r][16 * t + m];
-#pragma unroll
-        for (int h = 0; h < 4; ++h)
-#pragma unroll
-          for (int t = 0; t < NTILE; ++t) C[h][t] = M::mma(v[h], b[t], C[h][t]);
-      }
-      if constexpr (F32) {
-        if (++nword == 8) {
-          nword = 0;
+          for (int t = 0; t < NTILE; ++t) b[t] = qqs[r][16 * t + m];
 #pragma unroll
           for (int h = 0; h < 4; ++h)
 #pragma unroll
-            for (int t = 0; t < NTILE; ++t) {
-#pragma unroll
-              for (int v = 0; v < 4; ++v) D[h][t][v] += (double)C[h][t][v];
-              C[h][t] = V4{0, 0, 0, 0};
-            }
+            for (int t = 0; t < NTILE; ++t) C[h][t] = M::mma(v[h], b[t], C[h][t]);
         }
-      }
+        if constexpr (F32) {
+          if (++nword == 8) {
+            nword = 0;
+#pragma unroll
+            for (int h = 0; h < 4; ++h)
+#pragma unroll
+              for (int t = 0; t < NTILE; ++t) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) D[h][t][v] += (double)C[h][t][v];
+                C[h][t] = V4{0, 0, 0, 0};
+              }
+          }
+        }
+    };
+    if constexpr (PF) {
+#pragma unroll
+      for (int w = 0; w < WPS; ++w)
+        if (w < nw) do_word(w, wcur[w]);
+    } else {
+      for (int w = 0; w < nw; ++w) do_word(w, gp[(int64_t)(iws + w) * Jp + j]);
     }
   }
 #pragma unroll
@@ -512,14 +551,33 @@ __global__ __launch_bounds__(256) void k_pside_mm(const uint32_t* __restrict__ g
       }
 }
 
-// Q side: a wave owns 2 x 16 individuals (two A row tiles); each MFMA step takes 4 blocks (A columns /
-// B rows).  The workgroup (128 individuals) stages p and the p p' products of 16 blocks (2 words) in LDS.
+// MIXED, K <= 6: prefetching variant held to 3 waves per SIMD (168 VGPRs); K >= 7 and FP64 need more
+// registers and use the variant without prefetch.
 template <int K, typename TP>
-__global__ __launch_bounds__(256) void k_qside_mm(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pside_mm_pf(
+    const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8, int wpc, const TP* __restrict__ q,
+    const TP* __restrict__ p, const TP* __restrict__ qq, double* __restrict__ out) {
+  pside_mm_body<K, TP, true>(gp, Jp, I, I8, wpc, q, p, qq, out);
+}
+// waves_per_eu(2) keeps K = 8 at two waves per SIMD (and K = 7 at three) without spills.
+template <int K, typename TP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pside_mm(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8,
                                                   int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
-                                                  const TP* __restrict__ pp, double* __restrict__ part) {
+                                                  const TP* __restrict__ qq, double* __restrict__ out) {
+  pside_mm_body<K, TP, false>(gp, Jp, I, I8, wpc, q, p, qq, out);
+}
+
+// Q side: a wave owns 2 x 16 individuals (two A row tiles); each MFMA step takes 4 blocks (A columns /
+// B rows).  The workgroup (128 individuals) stages p and the p p' products of 16 blocks (2 words) in LDS;
+// PF as on the P side.
+template <int K, typename TP, bool PF>
+__device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8, int wpc,
+                                              const TP* __restrict__ q, const TP* __restrict__ p,
+                                              const TP* __restrict__ pp, double* __restrict__ part) {
   constexpr int NKK = K * (K + 1) / 2, NT = K + NKK, NTILE = (NKK + 15) / 16, QS = 16 * NTILE;
   constexpr int WPS = 2, R = 2;
+  constexpr int NP = 8 * WPS * 4 * K, NPP = 8 * WPS * 4 * QS;
+  constexpr int LP = (NP + 255) / 256, LPP = (NPP + 255) / 256;
   constexpr bool F32 = sizeof(TP) == 4;
   using M = Mfma<TP>;
   using V4 = typename M::v4;
@@ -548,60 +606,105 @@ __global__ __launch_bounds__(256) void k_qside_mm(const uint32_t* __restrict__ g
 #pragma unroll
       for (int v = 0; v < 4; ++v) D[r][t][v] = 0.0;
     }
+  // registers of the next stage: exactly nw words (the chunk's last stage may hold one word, and p / pp
+  // end at block Jp), the rest zero
+  TP rp[LP], rpp[LPP];
+  uint32_t wnext[WPS][R] = {};
+  auto fetch = [&](int jws_) {
+    const int nw_ = min(WPS, jw1 - jws_);
+    const TP* ps = p + (int64_t)K * 4 * 8 * jws_;
+    const TP* pps = pp + (int64_t)4 * 8 * jws_ * QS;
+#pragma unroll
+    for (int u = 0; u < LP; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      rp[u] = e < 8 * nw_ * 4 * K ? ps[e] : (TP)0;
+    }
+#pragma unroll
+    for (int u = 0; u < LPP; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      rpp[u] = e < 8 * nw_ * 4 * QS ? pps[e] : (TP)0;
+    }
+    if constexpr (PF) {
+#pragma unroll
+      for (int w = 0; w < WPS; ++w)
+#pragma unroll
+        for (int r = 0; r < R; ++r) wnext[w][r] = w < nw_ ? gq[(int64_t)(jws_ + w) * Ip + ibase + 16 * r + m] : 0u;
+    }
+  };
+  if (PF && jw0 < jw1) fetch(jw0);
   int nword = 0;
   for (int jws = jw0; jws < jw1; jws += WPS) {
     const int nw = min(WPS, jw1 - jws);
     __syncthreads();
-    {
-      const TP* ps = p + (int64_t)K * 4 * 8 * jws;
-      const TP* pps = pp + (int64_t)4 * 8 * jws * QS;
-      // exactly nw words: the chunk's last stage may hold one word, and p / pp end at block Jp
-      for (int e = threadIdx.x; e < 8 * nw * 4 * K; e += 256) (&Pl[0][0])[e] = ps[e];
-      for (int e = threadIdx.x; e < 8 * nw * 4 * QS; e += 256) (&PPl[0][0][0])[e] = pps[e];
+    if (!PF) fetch(jws);
+#pragma unroll
+    for (int u = 0; u < LP; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      if (e < NP) (&Pl[0][0])[e] = rp[u];
     }
+#pragma unroll
+    for (int u = 0; u < LPP; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      if (e < NPP) (&PPl[0][0][0])[e] = rpp[u];
+    }
+    uint32_t wcur[WPS][R];
+#pragma unroll
+    for (int w = 0; w < WPS; ++w)
+#pragma unroll
+      for (int r = 0; r < R; ++r) wcur[w][r] = wnext[w][r];
     __syncthreads();
-    for (int w = 0; w < nw; ++w) {
-      uint32_t word[R];
+    if (PF && jws + WPS < jw1) fetch(jws + WPS);
+    auto do_word = [&](int w, const uint32_t (&word)[R]) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) word[r] = gq[(int64_t)(jws + w) * Ip + ibase + 16 * r + m];
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const int blk = 8 * w + 4 * s + g;  // staged block of this lane's A column
-        TP b[4][NTILE];
-#pragma unroll
-        for (int h = 0; h < 4; ++h)
-#pragma unroll
-          for (int t = 0; t < NTILE; ++t) b[h][t] = PPl[blk][h][16 * t + m];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          TP X[4];
-#pragma unroll
-          for (int h = 0; h < 4; ++h) {
-            TP x = (TP)0;
-#pragma unroll
-            for (int k = 0; k < K; ++k) x = hk_fma(qi[r][k], Pl[blk][h * K + k], x);
-            X[h] = x;
-          }
-          TP v[4];
-          pair_weights<TP>(X, (word[r] >> (4 * (4 * s + g))) & 15u, v);
+        for (int s = 0; s < 2; ++s) {
+          const int blk = 8 * w + 4 * s + g;  // staged block of this lane's A column
+          TP b[4][NTILE];
 #pragma unroll
           for (int h = 0; h < 4; ++h)
 #pragma unroll
-            for (int t = 0; t < NTILE; ++t) C[r][t] = M::mma(v[h], b[h][t], C[r][t]);
-        }
-      }
-      if constexpr (F32) {
-        if (++nword == 8) {
-          nword = 0;
+            for (int t = 0; t < NTILE; ++t) b[h][t] = PPl[blk][h][16 * t + m];
 #pragma unroll
-          for (int r = 0; r < R; ++r)
+          for (int r = 0; r < R; ++r) {
+            TP X[4];
 #pragma unroll
-            for (int t = 0; t < NTILE; ++t) {
+            for (int h = 0; h < 4; ++h) {
+              TP x = (TP)0;
 #pragma unroll
-              for (int v = 0; v < 4; ++v) D[r][t][v] += (double)C[r][t][v];
-              C[r][t] = V4{0, 0, 0, 0};
+              for (int k = 0; k < K; ++k) x = hk_fma(qi[r][k], Pl[blk][h * K + k], x);
+              X[h] = x;
             }
+            TP v[4];
+            pair_weights<TP>(X, (word[r] >> (4 * (4 * s + g))) & 15u, v);
+#pragma unroll
+            for (int h = 0; h < 4; ++h)
+#pragma unroll
+              for (int t = 0; t < NTILE; ++t) C[r][t] = M::mma(v[h], b[h][t], C[r][t]);
+          }
         }
+        if constexpr (F32) {
+          if (++nword == 8) {
+            nword = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+              for (int t = 0; t < NTILE; ++t) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) D[r][t][v] += (double)C[r][t][v];
+                C[r][t] = V4{0, 0, 0, 0};
+              }
+          }
+        }
+    };
+    if constexpr (PF) {
+#pragma unroll
+      for (int w = 0; w < WPS; ++w)
+        if (w < nw) do_word(w, wcur[w]);
+    } else {
+      for (int w = 0; w < nw; ++w) {
+        uint32_t word[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) word[r] = gq[(int64_t)(jws + w) * Ip + ibase + 16 * r + m];
+        do_word(w, word);
       }
     }
   }
@@ -616,6 +719,19 @@ __global__ __launch_bounds__(256) void k_qside_mm(const uint32_t* __restrict__ g
         const int64_t row = ibase + 16 * r + M::row(g, v);  // individual of this result element
         if (n < NKK) part[((int64_t)chunk * NT + K + n) * Ip + row] = val;
       }
+}
+
+template <int K, typename TP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_qside_mm_pf(
+    const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8, int wpc, const TP* __restrict__ q,
+    const TP* __restrict__ p, const TP* __restrict__ pp, double* __restrict__ part) {
+  qside_mm_body<K, TP, true>(gq, Ip, I, J8, wpc, q, p, pp, part);
+}
+template <int K, typename TP>
+__global__ __launch_bounds__(256) void k_qside_mm(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
+                                                  int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
+                                                  const TP* __restrict__ pp, double* __restrict__ part) {
+  qside_mm_body<K, TP, false>(gq, Ip, I, J8, wpc, q, p, pp, part);
 }
 
 // -----------------------------------------------------------------------------------------------
@@ -746,8 +862,13 @@ static hipError_t qside_k(int mode, const PassGeom& g, const void* q, const void
     constexpr int QS = 16 * ((K * (K + 1) / 2 + 15) / 16);
     hipLaunchKernelGGL((k_make_outer<K, TP>), dim3((unsigned)((4 * g.Jp * QS + 255) / 256)), dim3(256), 0, s, pp,
                        4 * g.Jp, 4 * g.Jp, QS, (TP*)g.pp);
-    hipLaunchKernelGGL((k_qside_mm<K, TP>), dim3((unsigned)(g.Ip / 128), (unsigned)g.qchunks), block, 0, s, g.gq, g.Ip,
-                       g.I, (int)g.J8, g.qwpc, qq, pp, (const TP*)g.pp, part);
+    const dim3 gmm((unsigned)(g.Ip / 128), (unsigned)g.qchunks);
+    if constexpr (K <= 6 && sizeof(TP) == 4)
+      hipLaunchKernelGGL((k_qside_mm_pf<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
+                         (const TP*)g.pp, part);
+    else
+      hipLaunchKernelGGL((k_qside_mm<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
+                         (const TP*)g.pp, part);
   }
   else
     hipLaunchKernelGGL((k_qside<K, TP, MODE_CURV>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part);
@@ -766,8 +887,13 @@ static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void
     constexpr int QS = 16 * ((K * (K + 1) / 2 + 15) / 16);
     hipLaunchKernelGGL((k_make_outer<K, TP>), dim3((unsigned)((g.Ip * QS + 255) / 256)), dim3(256), 0, s, qq, g.I,
                        g.Ip, QS, (TP*)g.qq);
-    hipLaunchKernelGGL((k_pside_mm<K, TP>), dim3((unsigned)(g.Jp / 64), (unsigned)g.pchunks), block, 0, s, g.gp, g.Jp,
-                       g.I, (int)g.I8, g.pwpc, qq, pp, (const TP*)g.qq, out);
+    const dim3 gmm((unsigned)(g.Jp / 64), (unsigned)g.pchunks);
+    if constexpr (K <= 6 && sizeof(TP) == 4)
+      hipLaunchKernelGGL((k_pside_mm_pf<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
+                         (const TP*)g.qq, out);
+    else
+      hipLaunchKernelGGL((k_pside_mm<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
+                         (const TP*)g.qq, out);
   } else
     hipLaunchKernelGGL((k_pside<K, TP, MODE_CURV>), grid, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out);
   return hipGetLastError();
