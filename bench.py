@@ -270,8 +270,10 @@ def main():
         t = json.load(open(tpath)).get(a.workload)
         if t and prec == "mixed" and "pside" in name:
             traffic = t["hbm_bytes_per_launch"]
+    headline = a.workload == "mixed_I20k_J200k_K6" and prec == "mixed"
     out = {
-        "metric": "time-to-converge (s) at 1e-6 rel loglik, I=20k J=200k K=6; HBM GB/s",
+        "metric": ("time-to-converge (s) at 1e-6 rel loglik, I=20k J=200k K=6; HBM GB/s" if headline else
+                   f"time-to-converge (s) at rtol {a.rtol:g}, I={I} J={J} K={K}, {prec}"),
         "value": round(ms_per_step / 1000.0, 4),
         "unit": "s",
         "n_gpus": 1,

@@ -739,29 +739,37 @@ __global__ __launch_bounds__(256) void k_qside_mm(const uint32_t* __restrict__ g
 // -----------------------------------------------------------------------------------------------
 
 // part[c][t][Ip] -> res[i][NT] for i < I
-__global__ void k_reduce_q(const double* __restrict__ part, int nchunk, int NT, int64_t Ip, int64_t I,
-                           double* __restrict__ res) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= I) return;
-  for (int t = 0; t < NT; ++t) {
-    double s = 0.0;
-    for (int c = 0; c < nchunk; ++c) s += part[((int64_t)c * NT + t) * Ip + i];
-    res[i * NT + t] = s;
+// Sum over chunks of slot t of individual i, chunks in order (several loads in flight).
+__device__ __forceinline__ double chunk_sum(const double* __restrict__ part, int nchunk, int64_t stride, int64_t off) {
+  double s = 0.0;
+  int c = 0;
+  for (; c + 4 <= nchunk; c += 4) {
+    const double a0 = part[(int64_t)c * stride + off], a1 = part[(int64_t)(c + 1) * stride + off];
+    const double a2 = part[(int64_t)(c + 2) * stride + off], a3 = part[(int64_t)(c + 3) * stride + off];
+    s += a0;
+    s += a1;
+    s += a2;
+    s += a3;
   }
+  for (; c < nchunk; ++c) s += part[(int64_t)c * stride + off];
+  return s;
 }
 
-// Curvature pass: part[c][t][Ip] (XtX slots t >= K) -> res[i][NT] with Xtz = XtX q (FP64 q).
-__global__ void k_reduce_q_curv(const double* __restrict__ part, int nchunk, int K, int64_t Ip, int64_t I,
-                                const double* __restrict__ q, double* __restrict__ res) {
+// part[c][t][Ip] -> res[i][NT] for i < I, slots t0 + blockIdx.y (thread per (i, t)).
+__global__ void k_reduce_q(const double* __restrict__ part, int nchunk, int NT, int t0, int64_t Ip, int64_t I,
+                           double* __restrict__ res) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = t0 + blockIdx.y;
+  if (i >= I) return;
+  res[i * NT + t] = chunk_sum(part, nchunk, (int64_t)NT * Ip, (int64_t)t * Ip + i);
+}
+
+// Curvature pass: Xtz = XtX q (FP64 q) into slots 0..K-1 of res[i][NT], after k_reduce_q filled the XtX slots.
+__global__ void k_xtz_q(int K, int64_t I, const double* __restrict__ q, double* __restrict__ res) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= I) return;
   const int NT = K + K * (K + 1) / 2;
   double* r = res + i * NT;
-  for (int t = K; t < NT; ++t) {
-    double s = 0.0;
-    for (int c = 0; c < nchunk; ++c) s += part[((int64_t)c * NT + t) * Ip + i];
-    r[t] = s;
-  }
   for (int k = 0; k < K; ++k) {
     double s = 0.0;
     for (int k2 = 0; k2 < K; ++k2) {
@@ -801,19 +809,28 @@ __global__ void k_reduce_p(const double* __restrict__ part, int nchunk, int NT, 
   res[idx] = s;
 }
 
-// Log-likelihood: rows[i] = sum_c part[c][0][i]; total over rows in a fixed tree order.
-__global__ __launch_bounds__(1024) void k_reduce_ll(const double* __restrict__ part, int nchunk, int64_t Ip,
-                                                    int64_t I, double* __restrict__ rows,
-                                                    double* __restrict__ total) {
-  __shared__ double sh[1024];
-  double s = 0.0;
-  for (int64_t i = threadIdx.x; i < I; i += 1024) {
-    double r = 0.0;
-    for (int c = 0; c < nchunk; ++c) r += part[(int64_t)c * Ip + i];
+// Log-likelihood, stage 1: rows[i] = sum_c part[c][i]; bpart[b] = sum of the rows of workgroup b (tree).
+__global__ __launch_bounds__(256) void k_reduce_ll_rows(const double* __restrict__ part, int nchunk, int64_t Ip,
+                                                        int64_t I, double* __restrict__ rows,
+                                                        double* __restrict__ bpart) {
+  __shared__ double sh[256];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double r = 0.0;
+  if (i < I) {
+    r = chunk_sum(part, nchunk, Ip, i);
     if (rows) rows[i] = r;
-    s += r;
   }
-  s = block_sum<1024>(s, sh);
+  r = block_sum<256>(r, sh);
+  if (threadIdx.x == 0) bpart[blockIdx.x] = r;
+}
+
+// Stage 2: total = sum of the nb workgroup sums (fixed order).
+__global__ __launch_bounds__(256) void k_reduce_ll_total(const double* __restrict__ bpart, int nb,
+                                                         double* __restrict__ total) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += bpart[b];
+  s = block_sum<256>(s, sh);
   if (threadIdx.x == 0) *total = s;
 }
 
@@ -947,14 +964,17 @@ hipError_t launch_decode(const PassGeom& g, uint8_t* codes, hipStream_t s) {
 }
 
 hipError_t launch_reduce_q(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_q, dim3((unsigned)((g.I + 255) / 256)), dim3(256), 0, s, part, nchunk, NT, g.Ip, g.I, res);
+  hipLaunchKernelGGL(k_reduce_q, dim3((unsigned)((g.I + 255) / 256), (unsigned)NT), dim3(256), 0, s, part, nchunk, NT, 0,
+                     g.Ip, g.I, res);
   return hipGetLastError();
 }
 
 hipError_t launch_reduce_q_curv(const double* part, int nchunk, int K, const PassGeom& g, const double* q, double* res,
                                hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_q_curv, dim3((unsigned)((g.I + 255) / 256)), dim3(256), 0, s, part, nchunk, K, g.Ip, g.I,
-                     q, res);
+  const int NT = K + K * (K + 1) / 2;
+  const unsigned nb = (unsigned)((g.I + 255) / 256);
+  hipLaunchKernelGGL(k_reduce_q, dim3(nb, (unsigned)(NT - K)), dim3(256), 0, s, part, nchunk, NT, K, g.Ip, g.I, res);
+  hipLaunchKernelGGL(k_xtz_q, dim3(nb), dim3(256), 0, s, K, g.I, q, res);
   return hipGetLastError();
 }
 
@@ -969,9 +989,14 @@ hipError_t launch_reduce_p(const double* part, int nchunk, int NT, const PassGeo
   return hipGetLastError();
 }
 
+// The workgroup sums go to part + nchunk * Ip: the log-likelihood pass fills only the first nchunk * Ip
+// entries of the partial buffer, which holds nchunk * NT * Ip (NT >= 2).
 hipError_t launch_reduce_ll(const double* part, int nchunk, const PassGeom& g, double* rows, double* total,
                             hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_ll, dim3(1), dim3(1024), 0, s, part, nchunk, g.Ip, g.I, rows, total);
+  const int nb = (int)((g.I + 255) / 256);
+  double* bpart = const_cast<double*>(part) + (int64_t)nchunk * g.Ip;
+  hipLaunchKernelGGL(k_reduce_ll_rows, dim3((unsigned)nb), dim3(256), 0, s, part, nchunk, g.Ip, g.I, rows, bpart);
+  hipLaunchKernelGGL(k_reduce_ll_total, dim3(1), dim3(256), 0, s, bpart, nb, total);
   return hipGetLastError();
 }
 

@@ -43,8 +43,8 @@ class AdmixData2:
 
 def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iters=5, T=np.float64,
                     T2=np.float64, Q=3, use_gpu=True, verbose=False, fix_q=False, fix_p=False,
-                    init_q=None, init_p=None, engine_opts=None, warm_start=False):
-    """src/driver.jl:133-201."""
+                    init_q=None, init_p=None, engine_opts=None, warm_start=False, guarantee_increase=False):
+    """src/driver.jl:133-201.  guarantee_increase is admixture_qn!'s keyword (algorithms_outer.jl:41)."""
     if not use_gpu:
         raise NotImplementedError("this library is the GPU engine; use_gpu=False has no implementation here")
     if T is not np.float64:
@@ -61,7 +61,8 @@ def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iter
     try:
         d.doublemissing = eng.double_missing()
         eng.set_params(d.q, d.p)
-        base = dict(Qn=Q, fix_q=int(fix_q), fix_p=int(fix_p), verbose=int(verbose))
+        base = dict(Qn=Q, fix_q=int(fix_q), fix_p=int(fix_p), verbose=int(verbose),
+                    guarantee_increase=int(guarantee_increase))
         if engine_opts:
             base.update(engine_opts)
         ll_start = math.nan
@@ -104,7 +105,7 @@ def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iter
 def run_admixture(filename, I, J, K, *, rng=1, sparsity=None, prefix=None, skfr_tries=1, skfr_max_inner_iter=50,
                   skfr_mode="global", admix_n_iter=1000, admix_rtol=1e-7, admix_n_em_iters=5, T=np.float64,
                   T2=np.float64, Q=3, use_gpu=True, verbose=False, progress_bar=False, fix_q=False, fix_p=False,
-                  init_q=None, init_p=None, warm_start=False):
+                  init_q=None, init_p=None, warm_start=False, guarantee_increase=False):
     """src/driver.jl:38-79.  Returns (d, clusters, aims); clusters/aims are None without SKFR."""
     if not str(filename).endswith(".bed"):
         raise ValueError("filename should end with .bed")
@@ -113,5 +114,6 @@ def run_admixture(filename, I, J, K, *, rng=1, sparsity=None, prefix=None, skfr_
         raise NotImplementedError("SKFR AIM pre-selection (sparsity=...) is not part of this engine")
     d = _admixture_base(filename, I, J, K, n_iter=admix_n_iter, rtol=admix_rtol, rng=rng,
                         em_iters=admix_n_em_iters, T=T, T2=T2, Q=Q, use_gpu=use_gpu, verbose=verbose,
-                        fix_q=fix_q, fix_p=fix_p, init_q=init_q, init_p=init_p, warm_start=warm_start)
+                        fix_q=fix_q, fix_p=fix_p, init_q=init_q, init_p=init_p, warm_start=warm_start,
+                        guarantee_increase=guarantee_increase)
     return d, None, None
