@@ -119,6 +119,26 @@ struct NAcc {
   static constexpr int v = MODE == MODE_LL ? 1 : (MODE == MODE_GRAD ? K : K + K * (K + 1) / 2);
 };
 
+// Pair likelihood of the log-likelihood pass, Lh = L / 2^[a SNP is het] (1 for a double-missing pair),
+// from X = [[a, b], [c, d]] (X_h, h = 2 allele(SNP 2j) + allele(SNP 2j+1)).  Per SNP the compatibility
+// matrix is u u' (hom 0: u = e0, hom 1: u = e1, missing: u = 1) or e0 e1' + e1 e0' (het), so
+//   no het:      L = (u1' X u2)^2
+//   SNP 1 het:   L = 2 (X u2)_0 (X u2)_1          SNP 2 het:  L = 2 (u1' X)_0 (u1' X)_1
+//   both het:    L = 2 (a d + b c)
+// (equal to <X, S1 X S2> of the closed form above; pinned against it and the brute-force enumeration).
+template <typename TP>
+__device__ __forceinline__ TP pair_lh(const TP (&X)[4], unsigned nib) {
+  const unsigned c1 = nib & 3u, c2 = nib >> 2;
+  const TP m10 = c1 < 2u ? (TP)1 : (TP)0, m11 = (c1 & 1u) ? (TP)1 : (TP)0;  // u1 (zero if het)
+  const TP m20 = c2 < 2u ? (TP)1 : (TP)0, m21 = (c2 & 1u) ? (TP)1 : (TP)0;  // u2
+  const TP t0 = hk_fma(m21, X[1], m20 * X[0]), t1 = hk_fma(m21, X[3], m20 * X[2]);  // X u2
+  const TP w0 = hk_fma(m11, X[2], m10 * X[0]), w1 = hk_fma(m11, X[3], m10 * X[1]);  // u1' X
+  const TP sq = hk_fma(m11, t1, m10 * t0);
+  const bool h1 = c1 == 2u, h2 = c2 == 2u;
+  TP Lh = h1 ? (h2 ? hk_fma(X[0], X[3], X[1] * X[2]) : t0 * t1) : (h2 ? w0 * w1 : sq * sq);
+  return nib == HK_NIB_DM ? (TP)1 : Lh;
+}
+
 template <int K, typename TP, int MODE>
 __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
                                                int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
@@ -182,10 +202,7 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
       const TP L = X[0] * Y[0] + X[1] * Y[1] + X[2] * Y[2] + X[3] * Y[3];
       const bool dm = nib == HK_NIB_DM;
       if constexpr (MODE == MODE_LL) {
-        const bool het = (c1 == 2u) | (c2 == 2u);
-        TP Lh = het ? (TP)0.5 * L : L;
-        Lh = dm ? (TP)1 : Lh;
-        llm *= (double)Lh;  // FP64 running product; renormalised by frexp once per genotype word
+        llm *= (double)pair_lh<TP>(X, nib);  // FP64 running product; renormalised by frexp once per word
       } else {
         const TP rl = dm ? (TP)0 : (TP)2 * hk_rcp(L);
         TP w[4];

@@ -64,6 +64,34 @@ def test_pair_loglik_all_genotype_classes(brute):
             assert abs(brute.pair_loglik(g1, g2, qp) - v) <= 1e-13 * max(1.0, abs(v))
 
 
+def _pair_lh_factorized(X, c1, c2):
+    """The log-likelihood kernels' per-pair form (hk_pass.hip pair_lh), restated for the check below:
+    squares / products of the per-SNP compatibility forms instead of <X, S1 X S2> / 2^het."""
+    a, b, c, d = X
+    m10, m11 = float(c1 < 2), float(c1 & 1)
+    m20, m21 = float(c2 < 2), float(c2 & 1)
+    t0, t1 = m20 * a + m21 * b, m20 * c + m21 * d
+    w0, w1 = m10 * a + m11 * c, m10 * b + m11 * d
+    s = m10 * t0 + m11 * t1
+    h1, h2 = c1 == 2, c2 == 2
+    if c1 == 1 and c2 == 1:
+        return 1.0
+    return (a * d + b * c) if (h1 and h2) else (t0 * t1 if h1 else (w0 * w1 if h2 else s * s))
+
+
+def test_factorized_pair_likelihood_matches_bruteforce(brute):
+    """Every nibble (both BED codes) and random mixtures: the factorized pair likelihood equals the
+    ordered-haplotype-pair enumeration (with the reference's 1/2 per het SNP), so it cannot drop a term."""
+    geno = {0: 0, 1: -1, 2: 1, 3: 2}  # BED code -> allele-2 count (-1 missing), loops.jl:1-2
+    rng = np.random.default_rng(11)
+    for _ in range(25):
+        X = rng.dirichlet(np.ones(4))
+        for c1, c2 in itertools.product(range(4), repeat=2):
+            want = brute.pair_loglik(geno[c1], geno[c2], X)
+            got = np.log(_pair_lh_factorized(X, c1, c2))
+            assert abs(got - want) <= 1e-13 * max(1.0, abs(want)), (c1, c2)
+
+
 def test_counts_sum_to_two(brute):
     rng = np.random.default_rng(1)
     for g1, g2 in itertools.product([-1, 0, 1, 2], repeat=2):
