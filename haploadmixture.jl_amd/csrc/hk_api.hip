@@ -271,14 +271,13 @@ static int init_fit_buffers(hk_ctx* c) {
   const int64_t I = c->I;
   PassGeom& g = c->g;
   {
-    const int qs = 16 * ((K * (K + 1) / 2 + 15) / 16);
+    const int qs = outer_stride(K);
     const size_t el = c->prec == HK_PREC_FP64 ? sizeof(double) : sizeof(float);
     HIPC(c, hipMalloc(&c->qq, el * g.Ip * qs));
     HIPC(c, hipMalloc(&c->pp, el * 4 * g.Jp * qs));
   }
   g.qq = c->qq;
   g.pp = c->pp;
-  g.use_mfma = 1;
   c->xlen = (int64_t)K * (I + 4 * g.Jp);
   for (int b = 0; b < 4; ++b) {
     HIPC(c, hipMalloc(&c->xb[b], sizeof(double) * c->xlen));

@@ -15,8 +15,8 @@
 //   loglikelihood_full_loop2 (loops.jl:236-258)  -> k_qside<MODE_LL>
 //   em_loop! q part (loops.jl:260-295)           -> k_qside<MODE_GRAD>  (q_next = q .* grad)
 //   em_loop!/em_p_loop! p part (:260-326)        -> k_pside<MODE_GRAD>  (p_next = p .* grad)
-//   update_q_loop! (loops.jl:328-365)            -> k_qside<MODE_CURV>
-//   update_p_loop! (loops.jl:367-403)            -> k_pside<MODE_CURV>
+//   update_q_loop! (loops.jl:328-365)            -> k_qside_mm (matrix cores + VALU tail)
+//   update_p_loop! (loops.jl:367-403)            -> k_pside_mm (matrix cores + VALU tail)
 //
 // Genotype streams in HBM (built once from the BED payload by k_repack_*), 4 bits per pair
 // (bits 0-1 = BED code of SNP 2j, bits 2-3 = BED code of SNP 2j+1), 0.5 byte per pair:
@@ -218,34 +218,6 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
             if constexpr (F32) pa[k] += g; else acc[k] += g;
           }
         }
-        if constexpr (MODE == MODE_CURV) {
-          TP v[4];
-#pragma unroll
-          for (int h = 0; h < 4; ++h) v[h] = w[h] * hk_rcp(X[h]);
-#pragma unroll
-          for (int k = 0; k < K; ++k) {
-            TP u0 = v[0] * P[0][k], u1 = v[1] * P[1][k], u2 = v[2] * P[2][k], u3 = v[3] * P[3][k];
-#pragma unroll
-            for (int k2 = 0; k2 <= k; ++k2) {
-              const int t = K + k * (k + 1) / 2 + k2;
-              if constexpr (F32) {
-                TP s = pa[t];
-                s = hk_fma(u0, P[0][k2], s);
-                s = hk_fma(u1, P[1][k2], s);
-                s = hk_fma(u2, P[2][k2], s);
-                s = hk_fma(u3, P[3][k2], s);
-                pa[t] = s;
-              } else {
-                double s = acc[t];
-                s = hk_fma(u0, P[0][k2], s);
-                s = hk_fma(u1, P[1][k2], s);
-                s = hk_fma(u2, P[2][k2], s);
-                s = hk_fma(u3, P[3][k2], s);
-                acc[t] = s;
-              }
-            }
-          }
-        }
       }
     }
     if constexpr (MODE == MODE_LL) {
@@ -324,19 +296,6 @@ __global__ __launch_bounds__(256) void k_pside(const uint32_t* __restrict__ gp, 
           else acc[k] = hk_fma(w, Q[k], acc[k]);
         }
       }
-      if constexpr (MODE == MODE_CURV) {
-        const TP v = w * hk_rcp(X);
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const TP u = v * Q[k];
-#pragma unroll
-          for (int k2 = 0; k2 <= k; ++k2) {
-            const int t = K + k * (k + 1) / 2 + k2;
-            if constexpr (F32) pa[t] = hk_fma(u, Q[k2], pa[t]);
-            else acc[t] = hk_fma(u, Q[k2], acc[t]);
-          }
-        }
-      }
     }
     if constexpr (F32) {
 #pragma unroll
@@ -352,15 +311,18 @@ __global__ __launch_bounds__(256) void k_pside(const uint32_t* __restrict__ gp, 
 // Curvature passes on the matrix cores
 //
 // XtX is a GEMM with the per-pair weights as one operand:
-//     XtX_p[jh, n] = sum_i  v[i, jh] * QQ[i, n]      QQ[i, n] = q[k,i] q[k',i], n = k(k+1)/2 + k'
-//     XtX_q[i, n]  = sum_jh v[i, jh] * PP[jh, n]     PP[jh, n] = p[k,jh] p[k',jh]
-// One lane computes one (i, j) pair with all four haplotype weights v_h (no redundant per-pair work);
-// v_mfma_{f32,f64}_16x16x4 takes A = v_h (16 rows x 4), B = QQ or PP (4 x 16 columns), four MFMAs per
-// step (one per haplotype) and ceil(K(K+1)/2 / 16) column tiles.
+//     XtX_p[jh, n] = sum_i  v[i, jh] * QQ[i, n]      QQ[i, n] = 2 q[k,i] q[k',i], n = k(k+1)/2 + k'
+//     XtX_q[i, n]  = sum_jh v[i, jh] * PP[jh, n]     PP[jh, n] = 2 p[k,jh] p[k',jh]
+// with v_h = Y_h / (L X_h) = c_h / (2 qp_h^2) (pair_v; the factor 2 is in the tables).  One lane
+// computes one (i, j) pair with all four haplotype weights (no redundant per-pair work);
+// v_mfma_{f32,f64}_16x16x4 takes A = v_h (16 rows x 4), B = QQ or PP (4 x 16 columns): one MFMA per
+// haplotype and column tile.  gfx950 does not overlap FP32 MFMA and VALU issue, and one
+// v_mfma_f32_16x16x4f32 holds the SIMD as long as ~12 FMAs, so a short tail of packed columns (<= 6,
+// e.g. 16..20 for K = 6) is accumulated with VALU FMAs instead of a mostly empty extra tile (CurvShape).
 //   A operand: lane l holds A[l % 16][l / 16];  B operand: lane l holds B[l / 16][l % 16];
 //   result: lane l, register v holds D[row][l % 16], row = 4 (l/16) + v (f32) or l/16 + 4 v (f64).
-// MIXED: FP32 products and MFMA accumulation over 64 individuals (P side) / 64 blocks (Q side), then
-// added to FP64 accumulators.  FP64: the MFMA accumulates in FP64 directly.
+// MIXED: FP32 products and FP32 accumulation over 64 individuals (P side) / 64 blocks (Q side), then
+// added to FP64 accumulators.  FP64: FP64 throughout.
 // -----------------------------------------------------------------------------------------------
 
 template <typename TP>
@@ -382,30 +344,62 @@ struct Mfma<double> {
   static __device__ __forceinline__ int row(int g, int v) { return g + 4 * v; }
 };
 
-// Per-pair gradient / curvature weights of all four haplotypes (see the header of this file).
+// Column split of the packed curvature (NKK = K(K+1)/2 entries): NTILE MFMA tiles of 16, then NR
+// VALU tail columns starting at TAIL0; the outer-product tables have rows of QS entries.
+template <int K>
+struct CurvShape {
+  static constexpr int NKK = K * (K + 1) / 2;
+  static constexpr int FULL = NKK / 16, REM = NKK - 16 * FULL;
+  static constexpr bool VTAIL = REM > 0 && REM <= 6;
+  static constexpr int NTILE = FULL + (REM > 0 && !VTAIL ? 1 : 0);
+  static constexpr int NR = VTAIL ? REM : 0;
+  static constexpr int TAIL0 = 16 * NTILE;
+  static constexpr int QS = TAIL0 + (NR + 3) / 4 * 4;
+};
+
+// Per-nibble compatibility coefficients, 128 entries in LDS (one workgroup fills them with threads
+// 0..127): lut[4 nib + f] = (s1_00, s1_01, s1_11, e)[f] of SNP 2j and lut[64 + 4 nib + f] =
+// (s2_00, s2_01, s2_11, 0)[f] of SNP 2j+1 (snp_s).  The double-missing nibble has S1 = 0 and e = 1:
+// Y = 0 and L' = 1, so its weights are exactly 0 without a test.
 template <typename TP>
-__device__ __forceinline__ void pair_weights(const TP (&X)[4], unsigned nib, TP (&v)[4]) {
-  const unsigned c1 = nib & 3u, c2 = nib >> 2;
-  TP s1_00, s1_01, s1_11, s2_00, s2_01, s2_11;
-  snp_s<TP>(c1, s1_00, s1_01, s1_11);
-  snp_s<TP>(c2, s2_00, s2_01, s2_11);
-  const TP z00 = X[0] * s2_00 + X[1] * s2_01, z01 = X[0] * s2_01 + X[1] * s2_11;
-  const TP z10 = X[2] * s2_00 + X[3] * s2_01, z11 = X[2] * s2_01 + X[3] * s2_11;
-  TP Y[4];
-  Y[0] = s1_00 * z00 + s1_01 * z10;
-  Y[1] = s1_00 * z01 + s1_01 * z11;
-  Y[2] = s1_01 * z00 + s1_11 * z10;
-  Y[3] = s1_01 * z01 + s1_11 * z11;
-  const TP L = X[0] * Y[0] + X[1] * Y[1] + X[2] * Y[2] + X[3] * Y[3];
-  const TP rl = nib == HK_NIB_DM ? (TP)0 : (TP)2 * hk_rcp(L);
-#pragma unroll
-  for (int h = 0; h < 4; ++h) v[h] = Y[h] * rl * hk_rcp(X[h]);
+__device__ __forceinline__ void fill_pair_lut(TP* lut) {
+  const int t = threadIdx.x;
+  if (t < 128) {
+    const int tab = t >> 6, nib = (t >> 2) & 15, f = t & 3;
+    TP s00, s01, s11;
+    snp_s<TP>(tab ? (unsigned)nib >> 2 : (unsigned)nib & 3u, s00, s01, s11);
+    TP v = f == 0 ? s00 : (f == 1 ? s01 : (f == 2 ? s11 : (TP)0));
+    if (tab == 0 && nib == (int)HK_NIB_DM) v = f == 3 ? (TP)1 : (TP)0;
+    lut[t] = v;
+  }
 }
 
-// Packed outer products of parameter columns: out[c * stride + n] = x[k,c] x[k',c] for
+// Curvature weights of one pair: v_h = Y_h / (L X_h) with Y = S1 X S2, L = <X, Y> (+ e); zero for
+// incompatible haplotypes and for double-missing pairs.
+template <typename TP>
+__device__ __forceinline__ void pair_v(const TP (&X)[4], const TP* __restrict__ lut, unsigned nib, TP (&v)[4]) {
+  typedef TP T4 __attribute__((ext_vector_type(4)));
+  const T4 a = *(const T4*)(lut + 4 * nib);
+  const T4 b = *(const T4*)(lut + 64 + 4 * nib);
+  const TP z00 = hk_fma(X[1], b[1], X[0] * b[0]), z01 = hk_fma(X[1], b[2], X[0] * b[1]);
+  const TP z10 = hk_fma(X[3], b[1], X[2] * b[0]), z11 = hk_fma(X[3], b[2], X[2] * b[1]);
+  TP Y[4];
+  Y[0] = hk_fma(a[1], z10, a[0] * z00);
+  Y[1] = hk_fma(a[1], z11, a[0] * z01);
+  Y[2] = hk_fma(a[2], z10, a[1] * z00);
+  Y[3] = hk_fma(a[2], z11, a[1] * z01);
+  TP L = hk_fma(X[0], Y[0], a[3]);
+  L = hk_fma(X[1], Y[1], L);
+  L = hk_fma(X[2], Y[2], L);
+  L = hk_fma(X[3], Y[3], L);
+#pragma unroll
+  for (int h = 0; h < 4; ++h) v[h] = Y[h] * hk_rcp(L * X[h]);
+}
+
+// Packed outer products of parameter columns, scaled: out[c * stride + n] = scale x[k,c] x[k',c] for
 // n = k(k+1)/2 + k' (k' <= k), 0 for n >= K(K+1)/2; columns c >= ncol repeat column ncol-1.
 template <int K, typename TP>
-__global__ void k_make_outer(const TP* __restrict__ x, int64_t ncol, int64_t ncol_pad, int stride,
+__global__ void k_make_outer(const TP* __restrict__ x, int64_t ncol, int64_t ncol_pad, int stride, TP scale,
                              TP* __restrict__ out) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= ncol_pad * stride) return;
@@ -417,9 +411,36 @@ __global__ void k_make_outer(const TP* __restrict__ x, int64_t ncol, int64_t nco
     int k = 0;
     while ((k + 1) * (k + 2) / 2 <= n) ++k;
     const int k2 = n - k * (k + 1) / 2;
-    v = x[k + (int64_t)K * cc] * x[k2 + (int64_t)K * cc];
+    v = scale * (x[k + (int64_t)K * cc] * x[k2 + (int64_t)K * cc]);
   }
   out[idx] = v;
+}
+
+// Sum of a value over the four lanes l, l ^ 16, l ^ 32, l ^ 48 (same l % 16); bitwise equal in all four.
+__device__ __forceinline__ double sum_groups(double x) {
+  x += __shfl_xor(x, 16);
+  x += __shfl_xor(x, 32);
+  return x;
+}
+
+// Reduce-scatter of FP32 tail partials over the four lanes of a block on the P side (lane l = 16 g + m
+// holds A[h][n] summed over its individual group g): afterwards lane g adds the four-lane total of
+// haplotype h = g to DA[n].  v_permlane32_swap exchanges the lane halves of two registers (g <-> g ^ 2),
+// v_permlane16_swap the odd and even rows (g <-> g ^ 1); after each swap a lane holds its own and its
+// partner's value of the haplotype it keeps, so one add per kept value and no selects.
+template <int NR>
+__device__ __forceinline__ void tail_flush_p(float (&A)[4][NR], double (&DA)[NR]) {
+#pragma unroll
+  for (int n = 0; n < NR; ++n) {
+    const auto r02 = __builtin_amdgcn_permlane32_swap(__float_as_uint(A[0][n]), __float_as_uint(A[2][n]), false, false);
+    const auto r13 = __builtin_amdgcn_permlane32_swap(__float_as_uint(A[1][n]), __float_as_uint(A[3][n]), false, false);
+    const float b0 = __uint_as_float(r02[0]) + __uint_as_float(r02[1]);  // h = 2 (g >> 1)
+    const float b1 = __uint_as_float(r13[0]) + __uint_as_float(r13[1]);  // h = 2 (g >> 1) + 1
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(b0), __float_as_uint(b1), false, false);
+    DA[n] += (double)(__uint_as_float(r[0]) + __uint_as_float(r[1]));  // h = g
+#pragma unroll
+    for (int h = 0; h < 4; ++h) A[h][n] = 0.f;
+  }
 }
 
 // P side: a wave owns 16 blocks (A rows); each MFMA step takes 4 individuals (A columns / B rows).
@@ -431,7 +452,9 @@ template <int K, typename TP, bool PF>
 __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8, int wpc,
                                               const TP* __restrict__ q, const TP* __restrict__ p,
                                               const TP* __restrict__ qq, double* __restrict__ out) {
-  constexpr int NKK = K * (K + 1) / 2, NT = K + NKK, NTILE = (NKK + 15) / 16, QS = 16 * NTILE;
+  using S = CurvShape<K>;
+  constexpr int NKK = S::NKK, NT = K + NKK, NTILE = S::NTILE, NR = S::NR, QS = S::QS, T0 = S::TAIL0;
+  constexpr int NTA = NTILE > 0 ? NTILE : 1, NRA = NR > 0 ? NR : 1;
   constexpr int WPS = 4;  // genotype words per LDS stage
   constexpr int NQ = 8 * WPS * K, NQQ = 8 * WPS * QS;
   constexpr int LQ = (NQ + 255) / 256, LQQ = (NQQ + 255) / 256;
@@ -439,28 +462,37 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
   using M = Mfma<TP>;
   using V4 = typename M::v4;
   __shared__ TP qs[8 * WPS][K];
-  __shared__ TP qqs[8 * WPS][QS];
+  __shared__ __attribute__((aligned(16))) TP qqs[8 * WPS][QS];
+  __shared__ __attribute__((aligned(16))) TP lut[128];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4;
   const int64_t jbase = ((int64_t)blockIdx.x * 4 + wv) * 16;
   const int64_t j = jbase + m;
   const int chunk = blockIdx.y;
   const int iw0 = chunk * wpc, iw1 = min(I8, iw0 + wpc);
+  fill_pair_lut<TP>(lut);
   TP P[4][K];
 #pragma unroll
   for (int h = 0; h < 4; ++h)
 #pragma unroll
     for (int k = 0; k < K; ++k) P[h][k] = p[k + (int64_t)K * (4 * j + h)];
-  V4 C[4][NTILE];
-  double D[4][NTILE][4];
+  V4 C[4][NTA];
+  double D[4][NTA][4];
+  TP A[4][NRA];    // tail partials of this lane's individuals (FP64 engine: the full sums)
+  double DA[NRA];  // MIXED: four-lane tail totals of haplotype h = g
 #pragma unroll
-  for (int h = 0; h < 4; ++h)
+  for (int h = 0; h < 4; ++h) {
 #pragma unroll
-    for (int t = 0; t < NTILE; ++t) {
+    for (int t = 0; t < NTA; ++t) {
       C[h][t] = V4{0, 0, 0, 0};
 #pragma unroll
       for (int v = 0; v < 4; ++v) D[h][t][v] = 0.0;
     }
+#pragma unroll
+    for (int n = 0; n < NRA; ++n) A[h][n] = (TP)0;
+  }
+#pragma unroll
+  for (int n = 0; n < NRA; ++n) DA[n] = 0.0;
   // registers of the next stage: staged individuals i0 .. i0 + 8 WPS - 1 are clamped to I - 1
   TP rq[LQ], rqq[LQQ];
   uint32_t wnext[WPS] = {};
@@ -511,19 +543,20 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
     if (PF && iws + WPS < iw1) fetch(iws + WPS);
     auto do_word = [&](int w, uint32_t word) {
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          const int r = 8 * w + 4 * s + g;  // staged individual of this lane's A column
-          TP X[4];
+      for (int s = 0; s < 2; ++s) {
+        const int r = 8 * w + 4 * s + g;  // staged individual of this lane's A column
+        TP X[4];
 #pragma unroll
-          for (int h = 0; h < 4; ++h) {
-            TP x = (TP)0;
+        for (int h = 0; h < 4; ++h) {
+          TP x = (TP)0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) x = hk_fma(qs[r][k], P[h][k], x);
-            X[h] = x;
-          }
-          TP v[4];
-          pair_weights<TP>(X, (word >> (4 * (4 * s + g))) & 15u, v);
-          TP b[NTILE];
+          for (int k = 0; k < K; ++k) x = hk_fma(qs[r][k], P[h][k], x);
+          X[h] = x;
+        }
+        TP v[4];
+        pair_v<TP>(X, lut, (word >> (4 * (4 * s + g))) & 15u, v);
+        if constexpr (NTILE > 0) {
+          TP b[NTA];
 #pragma unroll
           for (int t = 0; t < NTILE; ++t) b[t] = qqs[r][16 * t + m];
 #pragma unroll
@@ -531,19 +564,30 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
 #pragma unroll
             for (int t = 0; t < NTILE; ++t) C[h][t] = M::mma(v[h], b[t], C[h][t]);
         }
-        if constexpr (F32) {
-          if (++nword == 8) {
-            nword = 0;
+        if constexpr (NR > 0) {
+          TP tq[NRA];
 #pragma unroll
-            for (int h = 0; h < 4; ++h)
+          for (int n = 0; n < NR; ++n) tq[n] = qqs[r][T0 + n];
 #pragma unroll
-              for (int t = 0; t < NTILE; ++t) {
+          for (int h = 0; h < 4; ++h)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) D[h][t][v] += (double)C[h][t][v];
-                C[h][t] = V4{0, 0, 0, 0};
-              }
-          }
+            for (int n = 0; n < NR; ++n) A[h][n] = hk_fma(v[h], tq[n], A[h][n]);
         }
+      }
+      if constexpr (F32) {
+        if (++nword == 8) {
+          nword = 0;
+#pragma unroll
+          for (int h = 0; h < 4; ++h)
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) D[h][t][v] += (double)C[h][t][v];
+              C[h][t] = V4{0, 0, 0, 0};
+            }
+          if constexpr (NR > 0) tail_flush_p<NR>(A, DA);
+        }
+      }
     };
     if constexpr (PF) {
 #pragma unroll
@@ -552,6 +596,12 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
     } else {
       for (int w = 0; w < nw; ++w) do_word(w, gp[(int64_t)(iws + w) * Jp + j]);
     }
+  }
+  // gradient slots (formed by the finish kernel): lane (m, g) clears those of (block jbase + m, h = g)
+  {
+    double* o = out + ((int64_t)chunk * 4 * Jp + 4 * j + g) * NT;
+#pragma unroll
+    for (int k = 0; k < K; ++k) o[k] = 0.0;
   }
 #pragma unroll
   for (int h = 0; h < 4; ++h)
@@ -562,10 +612,25 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
         const double val = F32 ? D[h][t][v] + (double)C[h][t][v] : (double)C[h][t][v];
         const int n = 16 * t + m;
         const int64_t row = jbase + M::row(g, v);  // block of this result element
-        double* o = out + ((int64_t)chunk * 4 * Jp + 4 * row + h) * NT;
-        if (n < NKK) o[K + n] = val;
-        if (t == 0 && m < K) o[m] = 0.0;  // gradient slots are formed by the finish kernel
+        if (n < NKK) out[((int64_t)chunk * 4 * Jp + 4 * row + h) * NT + K + n] = val;
       }
+  // VALU tail: the four lanes of a block hold sums over different individuals; lane g writes h = g
+  if constexpr (NR > 0) {
+    double* o = out + ((int64_t)chunk * 4 * Jp + 4 * j + g) * NT + K + T0;
+    if constexpr (F32) {
+      tail_flush_p<NR>(A, DA);
+#pragma unroll
+      for (int n = 0; n < NR; ++n) o[n] = DA[n];
+    } else {
+#pragma unroll
+      for (int h = 0; h < 4; ++h)
+#pragma unroll
+        for (int n = 0; n < NR; ++n) {
+          const double tot = sum_groups((double)A[h][n]);
+          if (g == h) o[n] = tot;
+        }
+    }
+  }
 }
 
 // MIXED, K <= 6: prefetching variant held to 3 waves per SIMD (168 VGPRs); K >= 7 and FP64 need more
@@ -591,7 +656,9 @@ template <int K, typename TP, bool PF>
 __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8, int wpc,
                                               const TP* __restrict__ q, const TP* __restrict__ p,
                                               const TP* __restrict__ pp, double* __restrict__ part) {
-  constexpr int NKK = K * (K + 1) / 2, NT = K + NKK, NTILE = (NKK + 15) / 16, QS = 16 * NTILE;
+  using S = CurvShape<K>;
+  constexpr int NKK = S::NKK, NT = K + NKK, NTILE = S::NTILE, NR = S::NR, QS = S::QS, T0 = S::TAIL0;
+  constexpr int NTA = NTILE > 0 ? NTILE : 1, NRA = NR > 0 ? NR : 1;
   constexpr int WPS = 2, R = 2;
   constexpr int NP = 8 * WPS * 4 * K, NPP = 8 * WPS * 4 * QS;
   constexpr int LP = (NP + 255) / 256, LPP = (NPP + 255) / 256;
@@ -599,12 +666,14 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
   using M = Mfma<TP>;
   using V4 = typename M::v4;
   __shared__ TP Pl[8 * WPS][4 * K];
-  __shared__ TP PPl[8 * WPS][4][QS];
+  __shared__ __attribute__((aligned(16))) TP PPl[8 * WPS][4][QS];
+  __shared__ __attribute__((aligned(16))) TP lut[128];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4;
   const int64_t ibase = ((int64_t)blockIdx.x * 4 + wv) * 16 * R;
   const int chunk = blockIdx.y;
   const int jw0 = chunk * wpc, jw1 = min(J8, jw0 + wpc);
+  fill_pair_lut<TP>(lut);
   TP qi[R][K];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -613,16 +682,24 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
 #pragma unroll
     for (int k = 0; k < K; ++k) qi[r][k] = q[k + (int64_t)K * ic];
   }
-  V4 C[R][NTILE];
-  double D[R][NTILE][4];
+  V4 C[R][NTA];
+  double D[R][NTA][4];
+  TP A[R][NRA];
+  double DA[R][NRA];
 #pragma unroll
-  for (int r = 0; r < R; ++r)
+  for (int r = 0; r < R; ++r) {
 #pragma unroll
-    for (int t = 0; t < NTILE; ++t) {
+    for (int t = 0; t < NTA; ++t) {
       C[r][t] = V4{0, 0, 0, 0};
 #pragma unroll
       for (int v = 0; v < 4; ++v) D[r][t][v] = 0.0;
     }
+#pragma unroll
+    for (int n = 0; n < NRA; ++n) {
+      A[r][n] = (TP)0;
+      DA[r][n] = 0.0;
+    }
+  }
   // registers of the next stage: exactly nw words (the chunk's last stage may hold one word, and p / pp
   // end at block Jp), the rest zero
   TP rp[LP], rpp[LPP];
@@ -673,44 +750,62 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
     if (PF && jws + WPS < jw1) fetch(jws + WPS);
     auto do_word = [&](int w, const uint32_t (&word)[R]) {
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          const int blk = 8 * w + 4 * s + g;  // staged block of this lane's A column
-          TP b[4][NTILE];
+      for (int s = 0; s < 2; ++s) {
+        const int blk = 8 * w + 4 * s + g;  // staged block of this lane's A column
+        TP b[4][NTA];
+#pragma unroll
+        for (int h = 0; h < 4; ++h)
+#pragma unroll
+          for (int t = 0; t < NTILE; ++t) b[h][t] = PPl[blk][h][16 * t + m];
+        TP tp[4][NRA];
+#pragma unroll
+        for (int h = 0; h < 4; ++h)
+#pragma unroll
+          for (int n = 0; n < NR; ++n) tp[h][n] = PPl[blk][h][T0 + n];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          TP X[4];
+#pragma unroll
+          for (int h = 0; h < 4; ++h) {
+            TP x = (TP)0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) x = hk_fma(qi[r][k], Pl[blk][h * K + k], x);
+            X[h] = x;
+          }
+          TP v[4];
+          pair_v<TP>(X, lut, (word[r] >> (4 * (4 * s + g))) & 15u, v);
 #pragma unroll
           for (int h = 0; h < 4; ++h)
 #pragma unroll
-            for (int t = 0; t < NTILE; ++t) b[h][t] = PPl[blk][h][16 * t + m];
+            for (int t = 0; t < NTILE; ++t) C[r][t] = M::mma(v[h], b[h][t], C[r][t]);
+#pragma unroll
+          for (int n = 0; n < NR; ++n) {
+            TP a = A[r][n];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) a = hk_fma(v[h], tp[h][n], a);
+            A[r][n] = a;
+          }
+        }
+      }
+      if constexpr (F32) {
+        if (++nword == 8) {
+          nword = 0;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            TP X[4];
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
-              TP x = (TP)0;
+            for (int t = 0; t < NTILE; ++t) {
 #pragma unroll
-              for (int k = 0; k < K; ++k) x = hk_fma(qi[r][k], Pl[blk][h * K + k], x);
-              X[h] = x;
+              for (int v = 0; v < 4; ++v) D[r][t][v] += (double)C[r][t][v];
+              C[r][t] = V4{0, 0, 0, 0};
             }
-            TP v[4];
-            pair_weights<TP>(X, (word[r] >> (4 * (4 * s + g))) & 15u, v);
 #pragma unroll
-            for (int h = 0; h < 4; ++h)
-#pragma unroll
-              for (int t = 0; t < NTILE; ++t) C[r][t] = M::mma(v[h], b[h][t], C[r][t]);
+            for (int n = 0; n < NR; ++n) {
+              DA[r][n] += (double)A[r][n];
+              A[r][n] = (TP)0;
+            }
           }
         }
-        if constexpr (F32) {
-          if (++nword == 8) {
-            nword = 0;
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-#pragma unroll
-              for (int t = 0; t < NTILE; ++t) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) D[r][t][v] += (double)C[r][t][v];
-                C[r][t] = V4{0, 0, 0, 0};
-              }
-          }
-        }
+      }
     };
     if constexpr (PF) {
 #pragma unroll
@@ -736,10 +831,18 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
         const int64_t row = ibase + 16 * r + M::row(g, v);  // individual of this result element
         if (n < NKK) part[((int64_t)chunk * NT + K + n) * Ip + row] = val;
       }
+  // VALU tail: the four lanes of an individual hold sums over different blocks
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int n = 0; n < NR; ++n) {
+      const double tot = sum_groups(F32 ? DA[r][n] + (double)A[r][n] : (double)A[r][n]);
+      if (g == (n & 3)) part[((int64_t)chunk * NT + K + T0 + n) * Ip + ibase + 16 * r + m] = tot;
+    }
 }
 
 template <int K, typename TP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_qside_mm_pf(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_qside_mm_pf(
     const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8, int wpc, const TP* __restrict__ q,
     const TP* __restrict__ p, const TP* __restrict__ pp, double* __restrict__ part) {
   qside_mm_body<K, TP, true>(gq, Ip, I, J8, wpc, q, p, pp, part);
@@ -892,10 +995,10 @@ static hipError_t qside_k(int mode, const PassGeom& g, const void* q, const void
     hipLaunchKernelGGL((k_qside<K, TP, MODE_LL>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part);
   else if (mode == MODE_GRAD)
     hipLaunchKernelGGL((k_qside<K, TP, MODE_GRAD>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part);
-  else if (g.use_mfma) {
-    constexpr int QS = 16 * ((K * (K + 1) / 2 + 15) / 16);
+  else {
+    constexpr int QS = CurvShape<K>::QS;
     hipLaunchKernelGGL((k_make_outer<K, TP>), dim3((unsigned)((4 * g.Jp * QS + 255) / 256)), dim3(256), 0, s, pp,
-                       4 * g.Jp, 4 * g.Jp, QS, (TP*)g.pp);
+                       4 * g.Jp, 4 * g.Jp, QS, (TP)2, (TP*)g.pp);
     const dim3 gmm((unsigned)(g.Ip / 128), (unsigned)g.qchunks);
     if constexpr (K <= 6 && sizeof(TP) == 4)
       hipLaunchKernelGGL((k_qside_mm_pf<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
@@ -904,8 +1007,6 @@ static hipError_t qside_k(int mode, const PassGeom& g, const void* q, const void
       hipLaunchKernelGGL((k_qside_mm<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
                          (const TP*)g.pp, part);
   }
-  else
-    hipLaunchKernelGGL((k_qside<K, TP, MODE_CURV>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part);
   return hipGetLastError();
 }
 
@@ -917,10 +1018,10 @@ static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void
   const TP* pp = (const TP*)p;
   if (mode == MODE_GRAD)
     hipLaunchKernelGGL((k_pside<K, TP, MODE_GRAD>), grid, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out);
-  else if (g.use_mfma) {
-    constexpr int QS = 16 * ((K * (K + 1) / 2 + 15) / 16);
+  else {
+    constexpr int QS = CurvShape<K>::QS;
     hipLaunchKernelGGL((k_make_outer<K, TP>), dim3((unsigned)((g.Ip * QS + 255) / 256)), dim3(256), 0, s, qq, g.I,
-                       g.Ip, QS, (TP*)g.qq);
+                       g.Ip, QS, (TP)2, (TP*)g.qq);
     const dim3 gmm((unsigned)(g.Jp / 64), (unsigned)g.pchunks);
     if constexpr (K <= 6 && sizeof(TP) == 4)
       hipLaunchKernelGGL((k_pside_mm_pf<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
@@ -928,8 +1029,7 @@ static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void
     else
       hipLaunchKernelGGL((k_pside_mm<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
                          (const TP*)g.qq, out);
-  } else
-    hipLaunchKernelGGL((k_pside<K, TP, MODE_CURV>), grid, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out);
+  }
   return hipGetLastError();
 }
 
@@ -956,6 +1056,20 @@ hipError_t launch_pside(int K, int prec, int mode, const PassGeom& g, const void
                         hipStream_t s) {
   if (prec == PREC_FP64) { HK_K_SWITCH(pside_k, double, mode, g, q, p, out, s) }
   HK_K_SWITCH(pside_k, float, mode, g, q, p, out, s)
+}
+
+int outer_stride(int K) {
+  switch (K) {
+    case 1: return CurvShape<1>::QS;
+    case 2: return CurvShape<2>::QS;
+    case 3: return CurvShape<3>::QS;
+    case 4: return CurvShape<4>::QS;
+    case 5: return CurvShape<5>::QS;
+    case 6: return CurvShape<6>::QS;
+    case 7: return CurvShape<7>::QS;
+    case 8: return CurvShape<8>::QS;
+    default: return 0;
+  }
 }
 
 int nacc(int K, int mode) { return mode == MODE_LL ? 1 : (mode == MODE_GRAD ? K : K + K * (K + 1) / 2); }
