@@ -111,7 +111,64 @@ __global__ void k_decode(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I,
 }
 
 // -----------------------------------------------------------------------------------------------
-// Q side
+// Per-pair algebra (shared by every pass)
+//
+// Per-nibble compatibility coefficients, 128 entries in LDS (each workgroup fills them with threads
+// 0..127): lut[4 nib + f] = (s1_00, s1_01, s1_11, e)[f] of SNP 2j and lut[64 + 4 nib + f] =
+// (s2_00, s2_01, s2_11, h)[f] of SNP 2j+1 (snp_s), with e = 1 for the double-missing nibble (whose S1 is
+// then 0) and h = 1/2 when either SNP is heterozygous.  With Y = S1 X S2 and L' = <X, Y> + e:
+//   likelihood pass:  Lh = L' h          (= L / 2^[a SNP is het]; 1 for a double-missing pair)
+//   gradient passes:  w_h / 2 = Y_h / L'  (0 for a double-missing pair)
+//   curvature passes: v_h = Y_h / (L' X_h) = c_h / (2 qp_h^2)
+// so no per-pair test or branch is needed for any genotype class.
+// -----------------------------------------------------------------------------------------------
+
+template <typename TP>
+__device__ __forceinline__ void fill_pair_lut(TP* lut) {
+  const int t = threadIdx.x;
+  if (t < 128) {
+    const int tab = t >> 6, nib = (t >> 2) & 15, f = t & 3;
+    TP s00, s01, s11;
+    snp_s<TP>(tab ? (unsigned)nib >> 2 : (unsigned)nib & 3u, s00, s01, s11);
+    TP v = f == 0 ? s00 : (f == 1 ? s01 : (f == 2 ? s11 : (TP)0));
+    if (tab == 1 && f == 3) v = ((nib & 3) == 2 || (nib >> 2) == 2) ? (TP)0.5 : (TP)1;
+    if (tab == 0 && nib == (int)HK_NIB_DM) v = f == 3 ? (TP)1 : (TP)0;
+    lut[t] = v;
+  }
+}
+
+// Y = S1 X S2 and L' = <X, Y> + e from the two table entries of a nibble.
+template <typename TP>
+struct PairForm {
+  typedef TP T4 __attribute__((ext_vector_type(4)));
+  TP Y[4], L, h;
+  __device__ __forceinline__ PairForm(const TP (&X)[4], const TP* __restrict__ lut, unsigned nib) {
+    const T4 a = *(const T4*)(lut + 4 * nib);
+    const T4 b = *(const T4*)(lut + 64 + 4 * nib);
+    const TP z00 = hk_fma(X[1], b[1], X[0] * b[0]), z01 = hk_fma(X[1], b[2], X[0] * b[1]);
+    const TP z10 = hk_fma(X[3], b[1], X[2] * b[0]), z11 = hk_fma(X[3], b[2], X[2] * b[1]);
+    Y[0] = hk_fma(a[1], z10, a[0] * z00);
+    Y[1] = hk_fma(a[1], z11, a[0] * z01);
+    Y[2] = hk_fma(a[2], z10, a[1] * z00);
+    Y[3] = hk_fma(a[2], z11, a[1] * z01);
+    L = hk_fma(X[0], Y[0], a[3]);
+    L = hk_fma(X[1], Y[1], L);
+    L = hk_fma(X[2], Y[2], L);
+    L = hk_fma(X[3], Y[3], L);
+    h = b[3];
+  }
+};
+
+// Curvature weights v_h = Y_h / (L X_h) (the factor 2 of c_h / qp_h^2 is in the outer-product tables).
+template <typename TP>
+__device__ __forceinline__ void pair_v(const TP (&X)[4], const TP* __restrict__ lut, unsigned nib, TP (&v)[4]) {
+  const PairForm<TP> f(X, lut, nib);
+#pragma unroll
+  for (int h = 0; h < 4; ++h) v[h] = f.Y[h] * hk_rcp(f.L * X[h]);
+}
+
+// -----------------------------------------------------------------------------------------------
+// Q side (individual-parallel): log-likelihood and EM / gradient sums
 // -----------------------------------------------------------------------------------------------
 
 template <int K, int MODE>
@@ -119,32 +176,19 @@ struct NAcc {
   static constexpr int v = MODE == MODE_LL ? 1 : (MODE == MODE_GRAD ? K : K + K * (K + 1) / 2);
 };
 
-// Pair likelihood of the log-likelihood pass, Lh = L / 2^[a SNP is het] (1 for a double-missing pair),
-// from X = [[a, b], [c, d]] (X_h, h = 2 allele(SNP 2j) + allele(SNP 2j+1)).  Per SNP the compatibility
-// matrix is u u' (hom 0: u = e0, hom 1: u = e1, missing: u = 1) or e0 e1' + e1 e0' (het), so
-//   no het:      L = (u1' X u2)^2
-//   SNP 1 het:   L = 2 (X u2)_0 (X u2)_1          SNP 2 het:  L = 2 (u1' X)_0 (u1' X)_1
-//   both het:    L = 2 (a d + b c)
-// (equal to <X, S1 X S2> of the closed form above; pinned against it and the brute-force enumeration).
-template <typename TP>
-__device__ __forceinline__ TP pair_lh(const TP (&X)[4], unsigned nib) {
-  const unsigned c1 = nib & 3u, c2 = nib >> 2;
-  const TP m10 = c1 < 2u ? (TP)1 : (TP)0, m11 = (c1 & 1u) ? (TP)1 : (TP)0;  // u1 (zero if het)
-  const TP m20 = c2 < 2u ? (TP)1 : (TP)0, m21 = (c2 & 1u) ? (TP)1 : (TP)0;  // u2
-  const TP t0 = hk_fma(m21, X[1], m20 * X[0]), t1 = hk_fma(m21, X[3], m20 * X[2]);  // X u2
-  const TP w0 = hk_fma(m11, X[2], m10 * X[0]), w1 = hk_fma(m11, X[3], m10 * X[1]);  // u1' X
-  const TP sq = hk_fma(m11, t1, m10 * t0);
-  const bool h1 = c1 == 2u, h2 = c2 == 2u;
-  TP Lh = h1 ? (h2 ? hk_fma(X[0], X[3], X[1] * X[2]) : t0 * t1) : (h2 ? w0 * w1 : sq * sq);
-  return nib == HK_NIB_DM ? (TP)1 : Lh;
-}
-
+// Lane = individual; the block's 4K p values are wave-uniform (scalar loads).  MODE_LL: the per-pair
+// likelihoods (>= ~1e-20) are multiplied into an FP64 mantissa that is renormalised once per genotype
+// word (8 pairs, >= ~1e-160: no underflow) with an integer exponent; one log per lane at the end, so
+// rounding grows like sqrt(J) ulps instead of J logs' worth.  MODE_GRAD: Xtz_q[k, i] = sum_{j,h} w_h p_kh.
 template <int K, typename TP, int MODE>
 __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
                                                int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
                                                double* __restrict__ part) {
   constexpr int NT = NAcc<K, MODE>::v;
   constexpr bool F32 = sizeof(TP) == 4;
+  __shared__ __attribute__((aligned(16))) TP lut[128];
+  fill_pair_lut<TP>(lut);
+  __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int chunk = blockIdx.y;
   const int jw0 = chunk * wpc;
@@ -157,19 +201,14 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
   double acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = 0.0;
-  // Log-likelihood: the per-pair likelihoods (>= ~1e-20) are multiplied into an FP64 mantissa that is
-  // renormalised once per genotype word (8 pairs, >= ~1e-160: no underflow) with an integer exponent;
-  // one log per lane at the end.  Rounding grows like sqrt(J) ulps instead of J logs' worth.
   double llm = 1.0;
   int lle = 0;
+  TP pa[NT];  // FP32 partials (MIXED), flushed to acc every 8 words
+#pragma unroll
+  for (int t = 0; t < NT; ++t) pa[t] = (TP)0;
 
   for (int jw = jw0; jw < jw1; ++jw) {
     const uint32_t word = gq[(int64_t)jw * Ip + i];
-    TP pa[NT];  // FP32 word partials (MIXED)
-    if constexpr (F32) {
-#pragma unroll
-      for (int t = 0; t < NT; ++t) pa[t] = (TP)0;
-    }
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
       const TP* pj = p + (int64_t)K * 4 * (jw * 8 + n);
@@ -178,8 +217,6 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
       for (int h = 0; h < 4; ++h)
 #pragma unroll
         for (int k = 0; k < K; ++k) P[h][k] = pj[h * K + k];
-      const unsigned nib = (word >> (4 * n)) & 15u;
-      const unsigned c1 = nib & 3u, c2 = nib >> 2;
       TP X[4];
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
@@ -188,35 +225,20 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
         for (int k = 0; k < K; ++k) s = hk_fma(qi[k], P[h][k], s);
         X[h] = s;
       }
-      TP s1_00, s1_01, s1_11, s2_00, s2_01, s2_11;
-      snp_s<TP>(c1, s1_00, s1_01, s1_11);
-      snp_s<TP>(c2, s2_00, s2_01, s2_11);
-      // Z = X S2 (rows a), Y = S1 Z
-      const TP z00 = X[0] * s2_00 + X[1] * s2_01, z01 = X[0] * s2_01 + X[1] * s2_11;
-      const TP z10 = X[2] * s2_00 + X[3] * s2_01, z11 = X[2] * s2_01 + X[3] * s2_11;
-      TP Y[4];
-      Y[0] = s1_00 * z00 + s1_01 * z10;
-      Y[1] = s1_00 * z01 + s1_01 * z11;
-      Y[2] = s1_01 * z00 + s1_11 * z10;
-      Y[3] = s1_01 * z01 + s1_11 * z11;
-      const TP L = X[0] * Y[0] + X[1] * Y[1] + X[2] * Y[2] + X[3] * Y[3];
-      const bool dm = nib == HK_NIB_DM;
+      const PairForm<TP> f(X, lut, (word >> (4 * n)) & 15u);
       if constexpr (MODE == MODE_LL) {
-        llm *= (double)pair_lh<TP>(X, nib);  // FP64 running product; renormalised by frexp once per word
+        llm *= (double)(f.L * f.h);  // FP64 running product; renormalised by frexp once per word
       } else {
-        const TP rl = dm ? (TP)0 : (TP)2 * hk_rcp(L);
+        const TP rl = hk_rcp(f.L);
         TP w[4];
 #pragma unroll
-        for (int h = 0; h < 4; ++h) w[h] = Y[h] * rl;
-        if constexpr (MODE == MODE_GRAD) {
+        for (int h = 0; h < 4; ++h) w[h] = f.Y[h] * rl;
 #pragma unroll
-          for (int k = 0; k < K; ++k) {
-            TP g = w[0] * P[0][k];
-            g = hk_fma(w[1], P[1][k], g);
-            g = hk_fma(w[2], P[2][k], g);
-            g = hk_fma(w[3], P[3][k], g);
-            if constexpr (F32) pa[k] += g; else acc[k] += g;
-          }
+        for (int k = 0; k < K; ++k) {
+          TP g = pa[k];
+#pragma unroll
+          for (int h = 0; h < 4; ++h) g = hk_fma(w[h], P[h][k], g);
+          pa[k] = g;
         }
       }
     }
@@ -225,86 +247,103 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
       llm = frexp(llm, &e);
       lle += e;
     } else if constexpr (F32) {
+      if (((jw - jw0) & 7) == 7) {
 #pragma unroll
-      for (int t = 0; t < NT; ++t) acc[t] += (double)pa[t];
+        for (int t = 0; t < NT; ++t) {
+          acc[t] += (double)pa[t];
+          pa[t] = (TP)0;
+        }
+      }
     }
   }
-  if constexpr (MODE == MODE_LL) acc[0] = log(llm) + (double)lle * 0.69314718055994530942;
+  if constexpr (MODE == MODE_LL) {
+    acc[0] = log(llm) + (double)lle * 0.69314718055994530942;
+  } else {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = 2.0 * (acc[t] + (double)pa[t]);  // w_h = 2 Y_h / L'
+  }
 #pragma unroll
   for (int t = 0; t < NT; ++t) part[((int64_t)chunk * NT + t) * Ip + i] = acc[t];
 }
 
 // -----------------------------------------------------------------------------------------------
-// P side
+// P side (block-parallel): EM / gradient sums Xtz_p[k, 4j+h] = sum_i w_h q_ki
 // -----------------------------------------------------------------------------------------------
 
-template <int K, typename TP, int MODE>
-__global__ __launch_bounds__(256) void k_pside(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8,
-                                               int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
-                                               double* __restrict__ out) {
-  constexpr int NT = NAcc<K, MODE>::v;
+// Lane = block with its 4K p values in registers; individual i's q column is wave-uniform (scalar
+// loads); partial sums over an i chunk go to out[chunk][4Jp][K].
+template <int K, typename TP>
+__global__ __launch_bounds__(256) void k_pside_grad(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8,
+                                                    int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
+                                                    double* __restrict__ out) {
   constexpr bool F32 = sizeof(TP) == 4;
-  const int64_t jh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // 4j + h
-  const int64_t j = jh >> 2;
-  const unsigned h = (unsigned)(jh & 3);
-  const unsigned a = h >> 1, b = h & 1u;
+  __shared__ __attribute__((aligned(16))) TP lut[128];
+  fill_pair_lut<TP>(lut);
+  __syncthreads();
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= Jp) return;  // the grid covers Jp (a multiple of 64) in workgroups of 256
   const int chunk = blockIdx.y;
   const int iw0 = chunk * wpc;
   const int iw1 = min(I8, iw0 + wpc);
-  TP Pw[K];
+  TP P[4][K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) Pw[k] = p[k + (int64_t)K * jh];
-
-  double acc[NT];
+  for (int h = 0; h < 4; ++h)
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = 0.0;
-
+    for (int k = 0; k < K; ++k) P[h][k] = p[k + (int64_t)K * (4 * j + h)];
+  double acc[4][K];
+  TP pa[4][K];
+#pragma unroll
+  for (int h = 0; h < 4; ++h)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      acc[h][k] = 0.0;
+      pa[h][k] = (TP)0;
+    }
   for (int iw = iw0; iw < iw1; ++iw) {
     const uint32_t word = gp[(int64_t)iw * Jp + j];
-    TP pa[NT];
-    if constexpr (F32) {
-#pragma unroll
-      for (int t = 0; t < NT; ++t) pa[t] = (TP)0;
-    }
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
-      int64_t ii = (int64_t)iw * 8 + n;
+      int64_t ii = (int64_t)iw * 8 + n;  // padding individuals carry the double-missing nibble
       ii = ii < I ? ii : I - 1;
       const TP* qv = q + (int64_t)K * ii;
       TP Q[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) Q[k] = qv[k];
-      const unsigned nib = (word >> (4 * n)) & 15u;
-      const unsigned c1 = nib & 3u, c2 = nib >> 2;
-      TP X = (TP)0;
+      TP X[4];
 #pragma unroll
-      for (int k = 0; k < K; ++k) X = hk_fma(Q[k], Pw[k], X);
-      const TP X0 = quad_bcast<0>(X), X1 = quad_bcast<1>(X), X2 = quad_bcast<2>(X), X3 = quad_bcast<3>(X);
-      TP s1_00, s1_01, s1_11, s2_00, s2_01, s2_11;
-      snp_s<TP>(c1, s1_00, s1_01, s1_11);
-      snp_s<TP>(c2, s2_00, s2_01, s2_11);
-      const TP r0 = a ? s1_01 : s1_00, r1 = a ? s1_11 : s1_01;
-      const TP t0 = b ? s2_01 : s2_00, t1 = b ? s2_11 : s2_01;
-      const TP Y = r0 * (t0 * X0 + t1 * X1) + r1 * (t0 * X2 + t1 * X3);
-      const TP L = quad_sum<TP>(X * Y);
-      const bool dm = nib == HK_NIB_DM;
-      const TP w = dm ? (TP)0 : (TP)2 * Y * hk_rcp(L);
-      if constexpr (MODE == MODE_GRAD) {
+      for (int h = 0; h < 4; ++h) {
+        TP s = (TP)0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          if constexpr (F32) pa[k] = hk_fma(w, Q[k], pa[k]);
-          else acc[k] = hk_fma(w, Q[k], acc[k]);
-        }
+        for (int k = 0; k < K; ++k) s = hk_fma(Q[k], P[h][k], s);
+        X[h] = s;
+      }
+      const PairForm<TP> f(X, lut, (word >> (4 * n)) & 15u);
+      const TP rl = hk_rcp(f.L);
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const TP w = f.Y[h] * rl;
+#pragma unroll
+        for (int k = 0; k < K; ++k) pa[h][k] = hk_fma(w, Q[k], pa[h][k]);
       }
     }
     if constexpr (F32) {
+      if (((iw - iw0) & 7) == 7) {
 #pragma unroll
-      for (int t = 0; t < NT; ++t) acc[t] += (double)pa[t];
+        for (int h = 0; h < 4; ++h)
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            acc[h][k] += (double)pa[h][k];
+            pa[h][k] = (TP)0;
+          }
+      }
     }
   }
-  double* o = out + ((int64_t)chunk * 4 * Jp + jh) * NT;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) o[t] = acc[t];
+  for (int h = 0; h < 4; ++h) {
+    double* o = out + ((int64_t)chunk * 4 * Jp + 4 * j + h) * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) o[k] = 2.0 * (acc[h][k] + (double)pa[h][k]);  // w_h = 2 Y_h / L'
+  }
 }
 
 // -----------------------------------------------------------------------------------------------
@@ -356,45 +395,6 @@ struct CurvShape {
   static constexpr int TAIL0 = 16 * NTILE;
   static constexpr int QS = TAIL0 + (NR + 3) / 4 * 4;
 };
-
-// Per-nibble compatibility coefficients, 128 entries in LDS (one workgroup fills them with threads
-// 0..127): lut[4 nib + f] = (s1_00, s1_01, s1_11, e)[f] of SNP 2j and lut[64 + 4 nib + f] =
-// (s2_00, s2_01, s2_11, 0)[f] of SNP 2j+1 (snp_s).  The double-missing nibble has S1 = 0 and e = 1:
-// Y = 0 and L' = 1, so its weights are exactly 0 without a test.
-template <typename TP>
-__device__ __forceinline__ void fill_pair_lut(TP* lut) {
-  const int t = threadIdx.x;
-  if (t < 128) {
-    const int tab = t >> 6, nib = (t >> 2) & 15, f = t & 3;
-    TP s00, s01, s11;
-    snp_s<TP>(tab ? (unsigned)nib >> 2 : (unsigned)nib & 3u, s00, s01, s11);
-    TP v = f == 0 ? s00 : (f == 1 ? s01 : (f == 2 ? s11 : (TP)0));
-    if (tab == 0 && nib == (int)HK_NIB_DM) v = f == 3 ? (TP)1 : (TP)0;
-    lut[t] = v;
-  }
-}
-
-// Curvature weights of one pair: v_h = Y_h / (L X_h) with Y = S1 X S2, L = <X, Y> (+ e); zero for
-// incompatible haplotypes and for double-missing pairs.
-template <typename TP>
-__device__ __forceinline__ void pair_v(const TP (&X)[4], const TP* __restrict__ lut, unsigned nib, TP (&v)[4]) {
-  typedef TP T4 __attribute__((ext_vector_type(4)));
-  const T4 a = *(const T4*)(lut + 4 * nib);
-  const T4 b = *(const T4*)(lut + 64 + 4 * nib);
-  const TP z00 = hk_fma(X[1], b[1], X[0] * b[0]), z01 = hk_fma(X[1], b[2], X[0] * b[1]);
-  const TP z10 = hk_fma(X[3], b[1], X[2] * b[0]), z11 = hk_fma(X[3], b[2], X[2] * b[1]);
-  TP Y[4];
-  Y[0] = hk_fma(a[1], z10, a[0] * z00);
-  Y[1] = hk_fma(a[1], z11, a[0] * z01);
-  Y[2] = hk_fma(a[2], z10, a[1] * z00);
-  Y[3] = hk_fma(a[2], z11, a[1] * z01);
-  TP L = hk_fma(X[0], Y[0], a[3]);
-  L = hk_fma(X[1], Y[1], L);
-  L = hk_fma(X[2], Y[2], L);
-  L = hk_fma(X[3], Y[3], L);
-#pragma unroll
-  for (int h = 0; h < 4; ++h) v[h] = Y[h] * hk_rcp(L * X[h]);
-}
 
 // Packed outer products of parameter columns, scaled: out[c * stride + n] = scale x[k,c] x[k',c] for
 // n = k(k+1)/2 + k' (k' <= k), 0 for n >= K(K+1)/2; columns c >= ncol repeat column ncol-1.
@@ -1017,7 +1017,8 @@ static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void
   const TP* qq = (const TP*)q;
   const TP* pp = (const TP*)p;
   if (mode == MODE_GRAD)
-    hipLaunchKernelGGL((k_pside<K, TP, MODE_GRAD>), grid, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out);
+    hipLaunchKernelGGL((k_pside_grad<K, TP>), dim3((unsigned)(g.Jp / 256 + (g.Jp % 256 ? 1 : 0)), (unsigned)g.pchunks),
+                       block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out);
   else {
     constexpr int QS = CurvShape<K>::QS;
     hipLaunchKernelGGL((k_make_outer<K, TP>), dim3((unsigned)((g.Ip * QS + 255) / 256)), dim3(256), 0, s, qq, g.I,

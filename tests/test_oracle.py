@@ -64,32 +64,50 @@ def test_pair_loglik_all_genotype_classes(brute):
             assert abs(brute.pair_loglik(g1, g2, qp) - v) <= 1e-13 * max(1.0, abs(v))
 
 
-def _pair_lh_factorized(X, c1, c2):
-    """The log-likelihood kernels' per-pair form (hk_pass.hip pair_lh), restated for the check below:
-    squares / products of the per-SNP compatibility forms instead of <X, S1 X S2> / 2^het."""
-    a, b, c, d = X
-    m10, m11 = float(c1 < 2), float(c1 & 1)
-    m20, m21 = float(c2 < 2), float(c2 & 1)
-    t0, t1 = m20 * a + m21 * b, m20 * c + m21 * d
-    w0, w1 = m10 * a + m11 * c, m10 * b + m11 * d
-    s = m10 * t0 + m11 * t1
-    h1, h2 = c1 == 2, c2 == 2
-    if c1 == 1 and c2 == 1:
-        return 1.0
-    return (a * d + b * c) if (h1 and h2) else (t0 * t1 if h1 else (w0 * w1 if h2 else s * s))
+def _pair_table():
+    """The kernels' per-nibble table (hk_pass.hip fill_pair_lut), restated for the checks below:
+    nibble -> (s1_00, s1_01, s1_11, e), (s2_00, s2_01, s2_11, h)."""
+    def snp_s(c):
+        hi, lo = c >> 1, c & 1
+        return [float(hi ^ 1), float(hi ^ lo), float(lo)]
+    t1, t2 = np.zeros((16, 4)), np.zeros((16, 4))
+    for nib in range(16):
+        c1, c2 = nib & 3, nib >> 2
+        t1[nib, :3] = snp_s(c1)
+        t2[nib, :3] = snp_s(c2)
+        t2[nib, 3] = 0.5 if (c1 == 2 or c2 == 2) else 1.0
+        if nib == 5:  # both SNPs missing
+            t1[nib] = [0.0, 0.0, 0.0, 1.0]
+    return t1, t2
 
 
-def test_factorized_pair_likelihood_matches_bruteforce(brute):
-    """Every nibble (both BED codes) and random mixtures: the factorized pair likelihood equals the
-    ordered-haplotype-pair enumeration (with the reference's 1/2 per het SNP), so it cannot drop a term."""
+def _pair_form(X, nib, t1, t2):
+    """PairForm of hk_pass.hip: Y = S1 X S2, L' = <X, Y> + e, and the het factor h."""
+    a, b = t1[nib], t2[nib]
+    z00, z01 = X[0] * b[0] + X[1] * b[1], X[0] * b[1] + X[1] * b[2]
+    z10, z11 = X[2] * b[0] + X[3] * b[1], X[2] * b[1] + X[3] * b[2]
+    Y = np.array([a[0] * z00 + a[1] * z10, a[0] * z01 + a[1] * z11, a[1] * z00 + a[2] * z10, a[1] * z01 + a[2] * z11])
+    return Y, float(X @ Y) + a[3], b[3]
+
+
+def test_pair_table_forms_match_bruteforce(brute):
+    """Every nibble (both BED codes) and random mixtures: the table-driven per-pair forms of the kernels
+    equal the ordered-haplotype-pair enumeration -- likelihood L' h (with the reference's 1/2 per het
+    SNP, 1 for a double-missing pair), gradient weights 2 Y_h / L' = c_h / qp_h and curvature weights
+    Y_h / (L' qp_h) = c_h / (2 qp_h^2) (0 for a double-missing pair) -- so no genotype class can drop a term."""
     geno = {0: 0, 1: -1, 2: 1, 3: 2}  # BED code -> allele-2 count (-1 missing), loops.jl:1-2
+    t1, t2 = _pair_table()
     rng = np.random.default_rng(11)
     for _ in range(25):
         X = rng.dirichlet(np.ones(4))
         for c1, c2 in itertools.product(range(4), repeat=2):
+            nib = c1 | (c2 << 2)
+            Y, L, h = _pair_form(X, nib, t1, t2)
             want = brute.pair_loglik(geno[c1], geno[c2], X)
-            got = np.log(_pair_lh_factorized(X, c1, c2))
-            assert abs(got - want) <= 1e-13 * max(1.0, abs(want)), (c1, c2)
+            assert abs(np.log(L * h) - want) <= 1e-13 * max(1.0, abs(want)), (c1, c2)
+            c = brute.pair_counts(geno[c1], geno[c2], X)
+            assert np.allclose(2 * Y / L, c / X, rtol=1e-13, atol=1e-15), (c1, c2)
+            assert np.allclose(Y / (L * X), c / (2 * X * X), rtol=1e-13, atol=1e-15), (c1, c2)
 
 
 def test_counts_sum_to_two(brute):
