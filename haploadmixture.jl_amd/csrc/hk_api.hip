@@ -271,10 +271,11 @@ static int init_fit_buffers(hk_ctx* c) {
   const int64_t I = c->I;
   PassGeom& g = c->g;
   {
-    const int qs = outer_stride(K);
+    int64_t nq = 0, np = 0;
+    stage_sizes(K, g.Ip, g.Jp, &nq, &np);
     const size_t el = c->prec == HK_PREC_FP64 ? sizeof(double) : sizeof(float);
-    HIPC(c, hipMalloc(&c->qq, el * g.Ip * qs));
-    HIPC(c, hipMalloc(&c->pp, el * 4 * g.Jp * qs));
+    HIPC(c, hipMalloc(&c->qq, el * nq));
+    HIPC(c, hipMalloc(&c->pp, el * np));
   }
   g.qq = c->qq;
   g.pp = c->pp;

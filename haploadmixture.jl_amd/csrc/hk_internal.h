@@ -17,13 +17,13 @@ struct PassGeom {
   const uint32_t* gp;  // [I8][Jp]
   int qchunks, qwpc;   // Q side: number of j chunks, words per chunk
   int pchunks, pwpc;   // P side: number of i chunks, words per chunk
-  void* qq;            // [Ip][outer_stride(K)] 2 q_k q_k' products (TP) for the P-side curvature pass
-  void* pp;            // [4Jp][outer_stride(K)] 2 p_k p_k' products (TP) for the Q-side curvature pass
+  void* qq;            // staging table of the P-side curvature pass (TP; rows per individual)
+  void* pp;            // staging table of the Q-side curvature pass (TP; rows per block)
 };
 
 int nacc(int K, int mode);
-// Row stride of the outer-product tables of the curvature passes (packed K(K+1)/2 entries, padded).
-int outer_stride(int K);
+// Element counts of the curvature passes' staging tables (P side: nq, Q side: np).
+void stage_sizes(int K, int64_t Ip, int64_t Jp, int64_t* nq, int64_t* np);
 
 hipError_t launch_repack(const uint8_t* raw, int64_t bps, const PassGeom& g, hipStream_t s);
 hipError_t launch_count_dm(const PassGeom& g, int64_t* dm, hipStream_t s);

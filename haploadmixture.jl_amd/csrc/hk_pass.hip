@@ -385,6 +385,14 @@ struct Mfma<double> {
 
 // Column split of the packed curvature (NKK = K(K+1)/2 entries): NTILE MFMA tiles of 16, then NR
 // VALU tail columns starting at TAIL0; the outer-product tables have rows of QS entries.
+__host__ __device__ constexpr int pow2_at_least(int v) { return v <= 1 ? 1 : 2 * pow2_at_least((v + 1) / 2); }
+
+// Column split of the packed curvature (NKK = K(K+1)/2 entries): NTILE MFMA tiles of 16, then NR VALU
+// tail columns starting at TAIL0; QS = padded length of one packed row.  Staging tables (k_stage_rows):
+//   P side, one row per individual: [2 q_k q_k' (QS) | q_k (K) | 0], RGP entries (a power of two);
+//   Q side, one row per block:      [2 p_kh p_k'h (QS), h = 0..3 | p_kh (4K, h-major) | 0], RGQ entries.
+// In LDS the rows are RLP / RLQ apart: = 16 (mod 64) words when MFMA B operands are read from them, so
+// the four lane groups of a wave (16 consecutive words each, different rows) fall on distinct banks.
 template <int K>
 struct CurvShape {
   static constexpr int NKK = K * (K + 1) / 2;
@@ -394,24 +402,36 @@ struct CurvShape {
   static constexpr int NR = VTAIL ? REM : 0;
   static constexpr int TAIL0 = 16 * NTILE;
   static constexpr int QS = TAIL0 + (NR + 3) / 4 * 4;
+  static constexpr int RGP = pow2_at_least(QS + K), RGQ = pow2_at_least(4 * QS + 4 * K);
+  static constexpr int RLP = NTILE > 0 ? RGP + 16 : RGP, RLQ = NTILE > 0 ? RGQ + 16 : RGQ;
+  static constexpr int PAD_ROWS = 32;  // rows past the last column (the last staging step reads up to 32)
 };
 
-// Packed outer products of parameter columns, scaled: out[c * stride + n] = scale x[k,c] x[k',c] for
-// n = k(k+1)/2 + k' (k' <= k), 0 for n >= K(K+1)/2; columns c >= ncol repeat column ncol-1.
+// Staging table rows (see CurvShape): row c < nrow of out (RG entries) holds, for s < sub, the QS packed
+// products 2 x[k,m] x[k',m] of column m = sub c + s, then the sub K values x[:, sub c + s], then zeros.
+// Rows c >= ncol / sub repeat the last column group.
 template <int K, typename TP>
-__global__ void k_make_outer(const TP* __restrict__ x, int64_t ncol, int64_t ncol_pad, int stride, TP scale,
+__global__ void k_stage_rows(const TP* __restrict__ x, int64_t ncol, int64_t nrow, int sub, int QS, int RG,
                              TP* __restrict__ out) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= ncol_pad * stride) return;
-  const int64_t c = idx / stride;
-  const int n = (int)(idx - c * stride);
-  const int64_t cc = c < ncol ? c : ncol - 1;
+  if (idx >= nrow * RG) return;
+  const int64_t c = idx / RG;
+  const int e = (int)(idx - c * RG);
+  const int64_t last = ncol / sub - 1;
+  const int64_t cc = c < last ? c : last;
   TP v = (TP)0;
-  if (n < K * (K + 1) / 2) {
-    int k = 0;
-    while ((k + 1) * (k + 2) / 2 <= n) ++k;
-    const int k2 = n - k * (k + 1) / 2;
-    v = scale * (x[k + (int64_t)K * cc] * x[k2 + (int64_t)K * cc]);
+  if (e < sub * QS) {
+    const int sgrp = e / QS, n = e - sgrp * QS;
+    if (n < K * (K + 1) / 2) {
+      int k = 0;
+      while ((k + 1) * (k + 2) / 2 <= n) ++k;
+      const int k2 = n - k * (k + 1) / 2;
+      const TP* xc = x + (int64_t)K * (sub * cc + sgrp);
+      v = (TP)2 * (xc[k] * xc[k2]);
+    }
+  } else if (e < sub * (QS + K)) {
+    const int t = e - sub * QS, sgrp = t / K, k = t - sgrp * K;
+    v = x[k + (int64_t)K * (sub * cc + sgrp)];
   }
   out[idx] = v;
 }
@@ -454,15 +474,14 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
                                               const TP* __restrict__ qq, double* __restrict__ out) {
   using S = CurvShape<K>;
   constexpr int NKK = S::NKK, NT = K + NKK, NTILE = S::NTILE, NR = S::NR, QS = S::QS, T0 = S::TAIL0;
+  constexpr int RG = S::RGP, RL = S::RLP;
   constexpr int NTA = NTILE > 0 ? NTILE : 1, NRA = NR > 0 ? NR : 1;
-  constexpr int WPS = 4;  // genotype words per LDS stage
-  constexpr int NQ = 8 * WPS * K, NQQ = 8 * WPS * QS;
-  constexpr int LQ = (NQ + 255) / 256, LQQ = (NQQ + 255) / 256;
+  constexpr int WPS = 4;  // genotype words per LDS stage (32 individuals)
+  constexpr int LQQ = 8 * WPS * RG / 256;
   constexpr bool F32 = sizeof(TP) == 4;
   using M = Mfma<TP>;
   using V4 = typename M::v4;
-  __shared__ TP qs[8 * WPS][K];
-  __shared__ __attribute__((aligned(16))) TP qqs[8 * WPS][QS];
+  __shared__ __attribute__((aligned(16))) TP qrow[8 * WPS * RL];  // staged table rows of 32 individuals
   __shared__ __attribute__((aligned(16))) TP lut[128];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4;
@@ -493,28 +512,15 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
   }
 #pragma unroll
   for (int n = 0; n < NRA; ++n) DA[n] = 0.0;
-  // registers of the next stage: staged individuals i0 .. i0 + 8 WPS - 1 are clamped to I - 1
-  TP rq[LQ], rqq[LQQ];
+  // registers of the next stage: the 8 WPS table rows from individual 8 iws_ on (the table has
+  // PAD_ROWS rows past Ip, so a chunk's last stage stays inside it)
+  TP rqq[LQQ];
   uint32_t wnext[WPS] = {};
   auto fetch = [&](int iws_) {
-    const int64_t i0 = (int64_t)iws_ * 8;
+    const TP* base = qq + (int64_t)iws_ * 8 * RG;
     const int nw_ = min(WPS, iw1 - iws_);
 #pragma unroll
-    for (int u = 0; u < LQ; ++u) {
-      const int e = threadIdx.x + 256 * u;
-      const int r = e / K, k = e - r * K;
-      int64_t ii = i0 + r;
-      ii = ii < I ? ii : I - 1;
-      rq[u] = e < NQ ? q[k + (int64_t)K * ii] : (TP)0;
-    }
-#pragma unroll
-    for (int u = 0; u < LQQ; ++u) {
-      const int e = threadIdx.x + 256 * u;
-      const int r = e / QS, n = e - r * QS;
-      int64_t ii = i0 + r;
-      ii = ii < I ? ii : I - 1;
-      rqq[u] = e < NQQ ? qq[ii * QS + n] : (TP)0;
-    }
+    for (int u = 0; u < LQQ; ++u) rqq[u] = base[threadIdx.x + 256 * u];
     if constexpr (PF) {
 #pragma unroll
       for (int w = 0; w < WPS; ++w) wnext[w] = w < nw_ ? gp[(int64_t)(iws_ + w) * Jp + j] : 0u;
@@ -527,14 +533,9 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
     __syncthreads();
     if (!PF) fetch(iws);
 #pragma unroll
-    for (int u = 0; u < LQ; ++u) {
-      const int e = threadIdx.x + 256 * u;
-      if (e < NQ) (&qs[0][0])[e] = rq[u];
-    }
-#pragma unroll
     for (int u = 0; u < LQQ; ++u) {
       const int e = threadIdx.x + 256 * u;
-      if (e < NQQ) (&qqs[0][0])[e] = rqq[u];
+      qrow[(e / RG) * RL + (e % RG)] = rqq[u];
     }
     uint32_t wcur[WPS];
 #pragma unroll
@@ -544,13 +545,13 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
     auto do_word = [&](int w, uint32_t word) {
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
-        const int r = 8 * w + 4 * s + g;  // staged individual of this lane's A column
+        const TP* row = qrow + (8 * w + 4 * s + g) * RL;  // staged individual of this lane's A column
         TP X[4];
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
           TP x = (TP)0;
 #pragma unroll
-          for (int k = 0; k < K; ++k) x = hk_fma(qs[r][k], P[h][k], x);
+          for (int k = 0; k < K; ++k) x = hk_fma(row[QS + k], P[h][k], x);
           X[h] = x;
         }
         TP v[4];
@@ -558,7 +559,7 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
         if constexpr (NTILE > 0) {
           TP b[NTA];
 #pragma unroll
-          for (int t = 0; t < NTILE; ++t) b[t] = qqs[r][16 * t + m];
+          for (int t = 0; t < NTILE; ++t) b[t] = row[16 * t + m];
 #pragma unroll
           for (int h = 0; h < 4; ++h)
 #pragma unroll
@@ -567,7 +568,7 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
         if constexpr (NR > 0) {
           TP tq[NRA];
 #pragma unroll
-          for (int n = 0; n < NR; ++n) tq[n] = qqs[r][T0 + n];
+          for (int n = 0; n < NR; ++n) tq[n] = row[T0 + n];
 #pragma unroll
           for (int h = 0; h < 4; ++h)
 #pragma unroll
@@ -575,7 +576,7 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
         }
       }
       if constexpr (F32) {
-        if (++nword == 8) {
+        if (++nword == 16) {
           nword = 0;
 #pragma unroll
           for (int h = 0; h < 4; ++h)
@@ -658,15 +659,14 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
                                               const TP* __restrict__ pp, double* __restrict__ part) {
   using S = CurvShape<K>;
   constexpr int NKK = S::NKK, NT = K + NKK, NTILE = S::NTILE, NR = S::NR, QS = S::QS, T0 = S::TAIL0;
+  constexpr int RG = S::RGQ, RL = S::RLQ;
   constexpr int NTA = NTILE > 0 ? NTILE : 1, NRA = NR > 0 ? NR : 1;
-  constexpr int WPS = 2, R = 2;
-  constexpr int NP = 8 * WPS * 4 * K, NPP = 8 * WPS * 4 * QS;
-  constexpr int LP = (NP + 255) / 256, LPP = (NPP + 255) / 256;
+  constexpr int WPS = 2, R = 2;  // genotype words per LDS stage (16 blocks), row tiles per wave
+  constexpr int LPP = 8 * WPS * RG / 256;
   constexpr bool F32 = sizeof(TP) == 4;
   using M = Mfma<TP>;
   using V4 = typename M::v4;
-  __shared__ TP Pl[8 * WPS][4 * K];
-  __shared__ __attribute__((aligned(16))) TP PPl[8 * WPS][4][QS];
+  __shared__ __attribute__((aligned(16))) TP prow[8 * WPS * RL];  // staged table rows of 16 blocks
   __shared__ __attribute__((aligned(16))) TP lut[128];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4;
@@ -700,24 +700,15 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
       DA[r][n] = 0.0;
     }
   }
-  // registers of the next stage: exactly nw words (the chunk's last stage may hold one word, and p / pp
-  // end at block Jp), the rest zero
-  TP rp[LP], rpp[LPP];
+  // registers of the next stage: the 8 WPS table rows from block 8 jws_ on (the table has PAD_ROWS
+  // rows past Jp, so a chunk's last stage stays inside it)
+  TP rpp[LPP];
   uint32_t wnext[WPS][R] = {};
   auto fetch = [&](int jws_) {
     const int nw_ = min(WPS, jw1 - jws_);
-    const TP* ps = p + (int64_t)K * 4 * 8 * jws_;
-    const TP* pps = pp + (int64_t)4 * 8 * jws_ * QS;
+    const TP* base = pp + (int64_t)jws_ * 8 * RG;
 #pragma unroll
-    for (int u = 0; u < LP; ++u) {
-      const int e = threadIdx.x + 256 * u;
-      rp[u] = e < 8 * nw_ * 4 * K ? ps[e] : (TP)0;
-    }
-#pragma unroll
-    for (int u = 0; u < LPP; ++u) {
-      const int e = threadIdx.x + 256 * u;
-      rpp[u] = e < 8 * nw_ * 4 * QS ? pps[e] : (TP)0;
-    }
+    for (int u = 0; u < LPP; ++u) rpp[u] = base[threadIdx.x + 256 * u];
     if constexpr (PF) {
 #pragma unroll
       for (int w = 0; w < WPS; ++w)
@@ -732,14 +723,9 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
     __syncthreads();
     if (!PF) fetch(jws);
 #pragma unroll
-    for (int u = 0; u < LP; ++u) {
-      const int e = threadIdx.x + 256 * u;
-      if (e < NP) (&Pl[0][0])[e] = rp[u];
-    }
-#pragma unroll
     for (int u = 0; u < LPP; ++u) {
       const int e = threadIdx.x + 256 * u;
-      if (e < NPP) (&PPl[0][0][0])[e] = rpp[u];
+      prow[(e / RG) * RL + (e % RG)] = rpp[u];
     }
     uint32_t wcur[WPS][R];
 #pragma unroll
@@ -751,17 +737,17 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
     auto do_word = [&](int w, const uint32_t (&word)[R]) {
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
-        const int blk = 8 * w + 4 * s + g;  // staged block of this lane's A column
+        const TP* row = prow + (8 * w + 4 * s + g) * RL;  // staged block of this lane's A column
         TP b[4][NTA];
 #pragma unroll
         for (int h = 0; h < 4; ++h)
 #pragma unroll
-          for (int t = 0; t < NTILE; ++t) b[h][t] = PPl[blk][h][16 * t + m];
+          for (int t = 0; t < NTILE; ++t) b[h][t] = row[h * QS + 16 * t + m];
         TP tp[4][NRA];
 #pragma unroll
         for (int h = 0; h < 4; ++h)
 #pragma unroll
-          for (int n = 0; n < NR; ++n) tp[h][n] = PPl[blk][h][T0 + n];
+          for (int n = 0; n < NR; ++n) tp[h][n] = row[h * QS + T0 + n];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           TP X[4];
@@ -769,7 +755,7 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
           for (int h = 0; h < 4; ++h) {
             TP x = (TP)0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) x = hk_fma(qi[r][k], Pl[blk][h * K + k], x);
+            for (int k = 0; k < K; ++k) x = hk_fma(qi[r][k], row[4 * QS + h * K + k], x);
             X[h] = x;
           }
           TP v[4];
@@ -788,7 +774,7 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
         }
       }
       if constexpr (F32) {
-        if (++nword == 8) {
+        if (++nword == 16) {
           nword = 0;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
@@ -996,9 +982,10 @@ static hipError_t qside_k(int mode, const PassGeom& g, const void* q, const void
   else if (mode == MODE_GRAD)
     hipLaunchKernelGGL((k_qside<K, TP, MODE_GRAD>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part);
   else {
-    constexpr int QS = CurvShape<K>::QS;
-    hipLaunchKernelGGL((k_make_outer<K, TP>), dim3((unsigned)((4 * g.Jp * QS + 255) / 256)), dim3(256), 0, s, pp,
-                       4 * g.Jp, 4 * g.Jp, QS, (TP)2, (TP*)g.pp);
+    using S = CurvShape<K>;
+    const int64_t nrow = g.Jp + S::PAD_ROWS;
+    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGQ + 255) / 256)), dim3(256), 0, s, pp,
+                       4 * g.Jp, nrow, 4, S::QS, S::RGQ, (TP*)g.pp);
     const dim3 gmm((unsigned)(g.Ip / 128), (unsigned)g.qchunks);
     if constexpr (K <= 6 && sizeof(TP) == 4)
       hipLaunchKernelGGL((k_qside_mm_pf<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
@@ -1020,9 +1007,10 @@ static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void
     hipLaunchKernelGGL((k_pside_grad<K, TP>), dim3((unsigned)(g.Jp / 256 + (g.Jp % 256 ? 1 : 0)), (unsigned)g.pchunks),
                        block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out);
   else {
-    constexpr int QS = CurvShape<K>::QS;
-    hipLaunchKernelGGL((k_make_outer<K, TP>), dim3((unsigned)((g.Ip * QS + 255) / 256)), dim3(256), 0, s, qq, g.I,
-                       g.Ip, QS, (TP)2, (TP*)g.qq);
+    using S = CurvShape<K>;
+    const int64_t nrow = g.Ip + S::PAD_ROWS;
+    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGP + 255) / 256)), dim3(256), 0, s, qq,
+                       g.I, nrow, 1, S::QS, S::RGP, (TP*)g.qq);
     const dim3 gmm((unsigned)(g.Jp / 64), (unsigned)g.pchunks);
     if constexpr (K <= 6 && sizeof(TP) == 4)
       hipLaunchKernelGGL((k_pside_mm_pf<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
@@ -1059,17 +1047,24 @@ hipError_t launch_pside(int K, int prec, int mode, const PassGeom& g, const void
   HK_K_SWITCH(pside_k, float, mode, g, q, p, out, s)
 }
 
-int outer_stride(int K) {
+template <int K>
+static void stage_sizes_k(int64_t Ip, int64_t Jp, int64_t* nq, int64_t* np) {
+  using S = CurvShape<K>;
+  *nq = (Ip + S::PAD_ROWS) * S::RGP;
+  *np = (Jp + S::PAD_ROWS) * S::RGQ;
+}
+
+void stage_sizes(int K, int64_t Ip, int64_t Jp, int64_t* nq, int64_t* np) {
   switch (K) {
-    case 1: return CurvShape<1>::QS;
-    case 2: return CurvShape<2>::QS;
-    case 3: return CurvShape<3>::QS;
-    case 4: return CurvShape<4>::QS;
-    case 5: return CurvShape<5>::QS;
-    case 6: return CurvShape<6>::QS;
-    case 7: return CurvShape<7>::QS;
-    case 8: return CurvShape<8>::QS;
-    default: return 0;
+    case 1: return stage_sizes_k<1>(Ip, Jp, nq, np);
+    case 2: return stage_sizes_k<2>(Ip, Jp, nq, np);
+    case 3: return stage_sizes_k<3>(Ip, Jp, nq, np);
+    case 4: return stage_sizes_k<4>(Ip, Jp, nq, np);
+    case 5: return stage_sizes_k<5>(Ip, Jp, nq, np);
+    case 6: return stage_sizes_k<6>(Ip, Jp, nq, np);
+    case 7: return stage_sizes_k<7>(Ip, Jp, nq, np);
+    case 8: return stage_sizes_k<8>(Ip, Jp, nq, np);
+    default: *nq = *np = 0;
   }
 }
 
