@@ -66,35 +66,47 @@ def test_pair_loglik_all_genotype_classes(brute):
 
 def _pair_table():
     """The kernels' per-nibble table (hk_pass.hip fill_pair_lut), restated for the checks below:
-    nibble -> (s1_00, s1_01, s1_11, e), (s2_00, s2_01, s2_11, h)."""
+    nibble -> (h s1_00, h s1_01, h s1_11, e), (s2_00, s2_01, s2_11, 2 h s1_01)."""
     def snp_s(c):
         hi, lo = c >> 1, c & 1
-        return [float(hi ^ 1), float(hi ^ lo), float(lo)]
+        return np.array([float(hi ^ 1), float(hi ^ lo), float(lo)])
     t1, t2 = np.zeros((16, 4)), np.zeros((16, 4))
     for nib in range(16):
         c1, c2 = nib & 3, nib >> 2
-        t1[nib, :3] = snp_s(c1)
+        h = 0.5 if (c1 == 2 or c2 == 2) else 1.0
+        t1[nib, :3] = h * snp_s(c1)
         t2[nib, :3] = snp_s(c2)
-        t2[nib, 3] = 0.5 if (c1 == 2 or c2 == 2) else 1.0
+        t2[nib, 3] = 2 * h * snp_s(c1)[1]
         if nib == 5:  # both SNPs missing
             t1[nib] = [0.0, 0.0, 0.0, 1.0]
+            t2[nib, 3] = 0.0
     return t1, t2
 
 
 def _pair_form(X, nib, t1, t2):
-    """PairForm of hk_pass.hip: Y = S1 X S2, L' = <X, Y> + e, and the het factor h."""
+    """PairForm of hk_pass.hip: Y = S1' X S2 and L' = <X, Y> + e."""
     a, b = t1[nib], t2[nib]
     z00, z01 = X[0] * b[0] + X[1] * b[1], X[0] * b[1] + X[1] * b[2]
     z10, z11 = X[2] * b[0] + X[3] * b[1], X[2] * b[1] + X[3] * b[2]
     Y = np.array([a[0] * z00 + a[1] * z10, a[0] * z01 + a[1] * z11, a[1] * z00 + a[2] * z10, a[1] * z01 + a[2] * z11])
-    return Y, float(X @ Y) + a[3], b[3]
+    return Y, float(X @ Y) + a[3]
+
+
+def _pair_lik(X, nib, t1, t2):
+    """pair_lik of hk_pass.hip: the quadratic form G without Y."""
+    a, b = t1[nib], t2[nib]
+    z00, z01 = X[0] * b[0] + X[1] * b[1], X[0] * b[1] + X[1] * b[2]
+    z10, z11 = X[2] * b[0] + X[3] * b[1], X[2] * b[1] + X[3] * b[2]
+    g00, g01, g11 = X[0] * z00 + X[1] * z01, X[0] * z10 + X[1] * z11, X[2] * z10 + X[3] * z11
+    return a[0] * g00 + b[3] * g01 + a[2] * g11 + a[3]
 
 
 def test_pair_table_forms_match_bruteforce(brute):
     """Every nibble (both BED codes) and random mixtures: the table-driven per-pair forms of the kernels
-    equal the ordered-haplotype-pair enumeration -- likelihood L' h (with the reference's 1/2 per het
-    SNP, 1 for a double-missing pair), gradient weights 2 Y_h / L' = c_h / qp_h and curvature weights
-    Y_h / (L' qp_h) = c_h / (2 qp_h^2) (0 for a double-missing pair) -- so no genotype class can drop a term."""
+    equal the ordered-haplotype-pair enumeration -- likelihood L' (with the reference's 1/2 per het SNP,
+    1 for a double-missing pair) in both of its forms, gradient weights 2 Y_h / L' = c_h / qp_h and
+    curvature weights Y_h / (L' qp_h) = c_h / (2 qp_h^2) (0 for a double-missing pair) -- so no genotype
+    class can drop a term."""
     geno = {0: 0, 1: -1, 2: 1, 3: 2}  # BED code -> allele-2 count (-1 missing), loops.jl:1-2
     t1, t2 = _pair_table()
     rng = np.random.default_rng(11)
@@ -102,9 +114,10 @@ def test_pair_table_forms_match_bruteforce(brute):
         X = rng.dirichlet(np.ones(4))
         for c1, c2 in itertools.product(range(4), repeat=2):
             nib = c1 | (c2 << 2)
-            Y, L, h = _pair_form(X, nib, t1, t2)
+            Y, L = _pair_form(X, nib, t1, t2)
             want = brute.pair_loglik(geno[c1], geno[c2], X)
-            assert abs(np.log(L * h) - want) <= 1e-13 * max(1.0, abs(want)), (c1, c2)
+            assert abs(np.log(L) - want) <= 1e-13 * max(1.0, abs(want)), (c1, c2)
+            assert abs(np.log(_pair_lik(X, nib, t1, t2)) - want) <= 1e-13 * max(1.0, abs(want)), (c1, c2)
             c = brute.pair_counts(geno[c1], geno[c2], X)
             assert np.allclose(2 * Y / L, c / X, rtol=1e-13, atol=1e-15), (c1, c2)
             assert np.allclose(Y / (L * X), c / (2 * X * X), rtol=1e-13, atol=1e-15), (c1, c2)
