@@ -52,8 +52,9 @@ struct StepOpts {
 // dq / dp (optional): the raw QP steps before projection, for the step-halving line search.
 hipError_t launch_q_step(int K, int64_t I, const double* res_q, const double* q, double* q_next, const StepOpts& o,
                          int* fails, hipStream_t s, double* dq = nullptr);
+// redo: J + 1 ints of scratch (the list of blocks left to the dense solver).
 hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
-                         int* fails, hipStream_t s, double* dp = nullptr);
+                         int* fails, int* redo, hipStream_t s, double* dp = nullptr);
 hipError_t launch_project_q(int K, int64_t I, double* q, double eps, hipStream_t s);
 hipError_t launch_project_p(int K, int64_t J, double* p, double eps, hipStream_t s);
 hipError_t launch_em_q(int K, int64_t I, int64_t J, const double* res_q, int NT, const int64_t* dm, const double* q,

@@ -186,6 +186,7 @@ __device__ int wave_qp(const double (&Hrow)[N], double g, double lo, double hi, 
     for (int c = 0; c < N; ++c) r = __builtin_fma(Hrow[c], rl(x, c), r);
     double z = solve(fr ? r : 0.0);
     double lam_me = 0.0;  // multiplier of this lane's group
+    bool lone = false;    // this lane is the only free member of its group (its step is exactly 0)
     if constexpr (M > 0) {
       double Y[M];
 #pragma unroll
@@ -196,7 +197,9 @@ __device__ int wave_qp(const double (&Hrow)[N], double g, double lo, double hi, 
       for (int c1 = 0; c1 < M; ++c1) {
         const bool mine = fr && grp == c1;
         rhs[c1] = -wave_sum(mine ? z : 0.0);
-        has[c1] = __ballot(mine) != 0ull;
+        const unsigned long long bm_c = __ballot(mine);
+        has[c1] = bm_c != 0ull;
+        lone = (grp == c1 && __popcll(bm_c) == 1) ? true : lone;
 #pragma unroll
         for (int c2 = 0; c2 < M; ++c2) S[c1][c2] = wave_sum(mine ? Y[c2] : 0.0);
       }
@@ -247,7 +250,7 @@ __device__ int wave_qp(const double (&Hrow)[N], double g, double lo, double hi, 
         lam_me = (grp == c) ? rhs[c] : lam_me;
       }
     }
-    const double d = fr ? -z : 0.0;
+    const double d = (fr && !lone) ? -z : 0.0;
     // Ratio test.
     double tc = 2.0;
     int side = 0;
@@ -331,17 +334,15 @@ __global__ __launch_bounds__(64) void k_q_step(int64_t I, const double* __restri
 }
 
 // ------------------------------------------------------------------------------------------------
-// update_p! row step: one wavefront per SNP block j; variable (l, k) at index l*K + k.
+// update_p! row step, dense: one wavefront per SNP block j; variable (l, k) at index l*K + k.
 // ------------------------------------------------------------------------------------------------
 template <int K, bool PEN>
-__global__ __launch_bounds__(64) void k_p_step(int64_t J, const double* __restrict__ res, const double* __restrict__ p,
-                                               double* __restrict__ p_next, double rho, double eps,
-                                               int* __restrict__ fails, double* __restrict__ dp) {
+__device__ __forceinline__ void p_step_dense(int64_t j, const double* __restrict__ res, const double* __restrict__ p,
+                                             double* __restrict__ p_next, double rho, double eps,
+                                             int* __restrict__ fails, double* __restrict__ dp, double* lds) {
   constexpr bool penalty = PEN;
   constexpr int NT = K + K * (K + 1) / 2;
   constexpr int N = 4 * K;
-  __shared__ double lds[N * N];
-  const int64_t j = blockIdx.x;
   const int lane = threadIdx.x;
   const bool act = lane < N;
   const int l = act ? lane / K : 0, k = act ? lane % K : 0;
@@ -381,6 +382,343 @@ __global__ __launch_bounds__(64) void k_p_step(int64_t J, const double* __restri
     proj_simplex<4>(v, eps);
 #pragma unroll
     for (int h = 0; h < 4; ++h) p_next[lane + (int64_t)K * (4 * j + h)] = v[h];
+  }
+}
+
+// Dense solves of the blocks listed in redo[1 .. redo[0]] (grid-stride over the list).
+template <int K, bool PEN>
+__global__ __launch_bounds__(64) void k_p_step_list(const double* __restrict__ res, const double* __restrict__ p,
+                                                    double* __restrict__ p_next, double rho, double eps,
+                                                    int* __restrict__ fails, double* __restrict__ dp,
+                                                    const int* __restrict__ redo) {
+  __shared__ double lds[4 * K * 4 * K];
+  const int n = redo[0];
+  for (int idx = blockIdx.x; idx < n; idx += gridDim.x)
+    p_step_dense<K, PEN>(redo[1 + idx], res, p, p_next, rho, eps, fails, dp, lds);
+}
+
+// ------------------------------------------------------------------------------------------------
+// update_p! row step, four lanes per QP (block-structured active set)
+//
+// The P-side QP of block j has 4K variables x[l][k] (haplotype l, population k) and curvature
+//     H = blockdiag(A_0, .., A_3) + a (J_4 (x) I_K),     A_l = XtX_p[:, :, 4j + l],
+// a = rho in penalty mode (no equalities) or sigma = the largest diagonal in equality mode
+// (sum_l x[l][k] = 0 for every k; the a-term vanishes on the feasible set).  Lane l of a quad owns
+// block l.  Every active-set step of wave_qp is reproduced with the same ratio test, tie-breaking
+// and multiplier rule, but the Newton system on the free set F is solved through the K-dimensional
+// coupling: with B_F = blockdiag(A_l[F_l, F_l]), C_l = (A_l[F_l, F_l])^-1 zero-padded, y_l = C_l r_l,
+//     penalty:   (B_F + rho E E')^-1 r = y - C t,     (I / rho + sum_l C_l) t = sum_l y_l   (Woodbury)
+//     equality:  d = -(y + C lam),                    (sum_l C_l) lam = -sum_l y_l          (Schur)
+// (sums over the quad by DPP).  A quad with a free block that is not positive definite (a haplotype
+// compatible with too few individuals) is appended to `redo` and solved by the one-wave dense kernel.
+// ------------------------------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ double quad_max(double v) {
+  v = fmax(v, dpp_f<0xB1>(v));
+  return fmax(v, dpp_f<0x4E>(v));
+}
+// (value, index) extremum over a quad with ties to the smaller index; identical on the four lanes.
+template <bool MIN>
+__device__ __forceinline__ void quad_arg(double& v, int& idx) {
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const double ov = s == 0 ? dpp_f<0xB1>(v) : dpp_f<0x4E>(v);
+    const int oi = s == 0 ? dpp_i<0xB1>(idx) : dpp_i<0x4E>(idx);
+    if ((MIN ? ov < v : ov > v) || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
+    }
+  }
+}
+
+// Packed lower-triangle index of the symmetric entry (a, b).
+__host__ __device__ constexpr int pk(int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; }
+
+// Cholesky solve of the K x K system S u = b (S symmetric positive definite, lower triangle used).
+template <int K>
+__device__ __forceinline__ void small_spd_solve(const double (&S)[K][K], double (&u)[K]) {
+  double Lc[K][K], di[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    double s = S[c][c];
+#pragma unroll
+    for (int t = 0; t < c; ++t) s -= Lc[c][t] * Lc[c][t];
+    s = s > 0.0 ? s : 1e-300;
+    const double sq = sqrt(s);
+    di[c] = 1.0 / sq;
+    Lc[c][c] = sq;
+#pragma unroll
+    for (int r = c + 1; r < K; ++r) {
+      double v = S[r][c];
+#pragma unroll
+      for (int t = 0; t < c; ++t) v -= Lc[r][t] * Lc[c][t];
+      Lc[r][c] = v * di[c];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    double v = u[r];
+#pragma unroll
+    for (int t = 0; t < r; ++t) v -= Lc[r][t] * u[t];
+    u[r] = v * di[r];
+  }
+#pragma unroll
+  for (int r = K - 1; r >= 0; --r) {
+    double v = u[r];
+#pragma unroll
+    for (int t = r + 1; t < K; ++t) v -= Lc[t][r] * u[t];
+    u[r] = v * di[r];
+  }
+}
+
+template <int K, bool PEN>
+__global__ __launch_bounds__(256) void k_p_step4(int64_t J, const double* __restrict__ res,
+                                                 const double* __restrict__ p, double* __restrict__ p_next,
+                                                 double rho, double eps, double* __restrict__ dp,
+                                                 int* __restrict__ redo) {
+  constexpr int NT = K + K * (K + 1) / 2;
+  constexpr int maxit = 4 * (4 * K) + 20;  // as wave_qp with N = 4K
+  const int l = threadIdx.x & 3;
+  const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+  const bool valid = j < J;
+  const int64_t jj = valid ? j : J - 1;  // quads past the end mirror the last block and write nothing
+  const double* r = res + (4 * jj + l) * NT;
+  constexpr int NKK = K * (K + 1) / 2;
+  double A[NKK];  // packed lower triangle, A[pk(k, k2)]
+#pragma unroll
+  for (int t = 0; t < NKK; ++t) A[t] = r[K + t];
+  double g[K], lo[K], hi[K], pv[K], x[K];
+  int st[K];  // 0 free, -1 at lower, +1 at upper
+  double gs = 0.0, bm = 0.0, dg = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    pv[k] = p[k + (int64_t)K * (4 * jj + l)];
+    g[k] = -r[k];
+    lo[k] = 0.0 - pv[k];
+    hi[k] = 1.0 - pv[k];
+    x[k] = 0.0;
+    st[k] = lo[k] >= 0.0 ? -1 : (hi[k] <= 0.0 ? 1 : 0);
+    gs = fmax(gs, fabs(g[k]));
+    bm = fmax(bm, fmax(fabs(lo[k]), fabs(hi[k])));
+    dg = fmax(dg, A[pk(k, k)]);
+  }
+  const double a = PEN ? rho : quad_max(dg);
+  double hm = a;  // the entries a of the off-diagonal blocks
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int k2 = 0; k2 <= k; ++k2) hm = fmax(hm, fabs(A[pk(k, k2)] + (k == k2 ? a : 0.0)));
+  hm = quad_max(hm);
+  gs = quad_max(gs);
+  bm = quad_max(bm);
+  const double tol = 1e-11 * (gs + hm * bm) + 1e-300;
+  // pivots below this share of the block's largest diagonal send the QP to the dense solver
+  const double piv_min = 1e-6 * dg + 1e-300;
+
+  bool done = !valid, dense = false;
+  double lam[K];  // equality-mode multipliers of the last Newton step
+#pragma unroll
+  for (int k = 0; k < K; ++k) lam[k] = 0.0;
+  for (int it = 0; it < maxit; ++it) {
+    if (__ballot(!done) == 0ull) break;
+    if (done) continue;
+    bool fr[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) fr[k] = st[k] == 0;
+    // masked block Cholesky: A on F_l x F_l, identity elsewhere
+    double L[K][K], di[K];
+    bool pd = true;
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      double s = fr[c] ? A[pk(c, c)] : 1.0;
+#pragma unroll
+      for (int t = 0; t < c; ++t) s -= L[c][t] * L[c][t];
+      const bool ok = !fr[c] || s > piv_min;
+      pd = pd && ok;
+      s = ok ? s : 1.0;
+      const double sq = sqrt(s);
+      di[c] = 1.0 / sq;
+      L[c][c] = sq;
+#pragma unroll
+      for (int rr = c + 1; rr < K; ++rr) {
+        double v = (fr[rr] && fr[c]) ? A[pk(rr, c)] : 0.0;
+#pragma unroll
+        for (int t = 0; t < c; ++t) v -= L[rr][t] * L[c][t];
+        L[rr][c] = v * di[c];
+      }
+    }
+    if (quad_max(pd ? 0.0 : 1.0) > 0.0) {
+      dense = true;
+      done = true;
+      continue;
+    }
+    // C = (A_l[F, F])^-1, zero outside F (packed lower triangle): column c solves L L' C[:, c] = e_c
+    double C[NKK];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      double u[K];
+#pragma unroll
+      for (int rr = 0; rr < K; ++rr) {
+        double v = rr == c ? 1.0 : 0.0;
+#pragma unroll
+        for (int t = 0; t < rr; ++t) v -= L[rr][t] * u[t];
+        u[rr] = v * di[rr];
+      }
+#pragma unroll
+      for (int rr = K - 1; rr >= 0; --rr) {
+        double v = u[rr];
+#pragma unroll
+        for (int t = rr + 1; t < K; ++t) v -= L[t][rr] * u[t];
+        u[rr] = v * di[rr];
+      }
+#pragma unroll
+      for (int rr = c; rr < K; ++rr) C[pk(rr, c)] = (fr[rr] && fr[c]) ? u[rr] : 0.0;
+    }
+    // gradient at x: r = A x + a sum_l x + g
+    double xs[K], gr[K], y[K], ys[K], Cs[K][K];  // Cs: lower triangle used
+#pragma unroll
+    for (int k = 0; k < K; ++k) xs[k] = quad_sum<double>(x[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = g[k];
+#pragma unroll
+      for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(A[pk(k, k2)], x[k2], s);
+      gr[k] = __builtin_fma(a, xs[k], s);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0.0;
+#pragma unroll
+      for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(C[pk(k, k2)], fr[k2] ? gr[k2] : 0.0, s);
+      y[k] = s;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      ys[k] = quad_sum<double>(y[k]);
+#pragma unroll
+      for (int k2 = 0; k2 <= k; ++k2) Cs[k][k2] = quad_sum<double>(C[pk(k, k2)]);
+    }
+    double d[K];
+    if constexpr (PEN) {
+      double t[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        t[k] = ys[k];
+        Cs[k][k] += 1.0 / rho;
+      }
+      small_spd_solve<K>(Cs, t);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double s = y[k];
+#pragma unroll
+        for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(-C[pk(k, k2)], t[k2], s);
+        d[k] = fr[k] ? -s : 0.0;
+      }
+    } else {
+      // groups without a free member are already satisfied: identity rows, zero multiplier
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const bool has = Cs[k][k] > 0.0;
+        lam[k] = has ? -ys[k] : 0.0;
+        if (!has) {
+#pragma unroll
+          for (int k2 = 0; k2 < K; ++k2) {
+            if (k2 <= k) Cs[k][k2] = k2 == k ? 1.0 : 0.0;
+            else Cs[k2][k] = 0.0;
+          }
+        }
+      }
+      small_spd_solve<K>(Cs, lam);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double s = y[k];
+#pragma unroll
+        for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(C[pk(k, k2)], lam[k2], s);
+        // the lone free member of a group cannot move: its step is exactly 0
+        const double nfree = quad_sum<double>(fr[k] ? 1.0 : 0.0);
+        d[k] = (fr[k] && nfree > 1.5) ? -s : 0.0;
+      }
+    }
+    // ratio test (smallest step to a bound; ties to the smaller variable index l K + k)
+    double tmin = 2.0;
+    int imin = l * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double tc = 2.0;
+      if (fr[k]) {
+        if (d[k] < 0.0 && x[k] + d[k] < lo[k]) tc = (lo[k] - x[k]) / d[k];
+        else if (d[k] > 0.0 && x[k] + d[k] > hi[k]) tc = (hi[k] - x[k]) / d[k];
+      }
+      if (tc < tmin) {
+        tmin = tc;
+        imin = l * K + k;
+      }
+    }
+    quad_arg<true>(tmin, imin);
+    const bool blocked = tmin < 1.0;
+    const double alpha = blocked ? fmax(tmin, 0.0) : 1.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (fr[k]) {
+        const double v = x[k] + alpha * d[k];
+        x[k] = v < lo[k] ? lo[k] : (v > hi[k] ? hi[k] : v);
+      }
+    if (blocked) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (imin == l * K + k) {
+          const int side = d[k] < 0.0 ? -1 : 1;
+          x[k] = side < 0 ? lo[k] : hi[k];
+          st[k] = side;
+        }
+      continue;
+    }
+    // at the minimiser of the working set: bound multipliers, release the most violated
+#pragma unroll
+    for (int k = 0; k < K; ++k) xs[k] = quad_sum<double>(x[k]);
+    double vmax = -1.0;
+    int iv = l * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = g[k];
+#pragma unroll
+      for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(A[pk(k, k2)], x[k2], s);
+      s = __builtin_fma(a, xs[k], s);
+      if constexpr (!PEN) s += lam[k];
+      const double v = st[k] != 0 ? (st[k] < 0 ? -s : s) : -1.0;
+      if (v > vmax) {
+        vmax = v;
+        iv = l * K + k;
+      }
+    }
+    quad_arg<false>(vmax, iv);
+    if (!(vmax > tol)) {
+      done = true;
+      continue;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (iv == l * K + k) st[k] = 0;
+  }
+  if (!valid) return;
+  if (dense) {
+    if (l == 0) redo[1 + atomicAdd(redo, 1)] = (int)j;
+    return;
+  }
+  if (dp) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) dp[k + (int64_t)K * (4 * j + l)] = x[k];
+  }
+  // p_next = p + dp, then project_p! on each (k, j) 4-vector (the quad's four lanes)
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double nv = pv[k] + x[k];
+    double v[4] = {quad_bcast<0>(nv), quad_bcast<1>(nv), quad_bcast<2>(nv), quad_bcast<3>(nv)};
+    proj_simplex<4>(v, eps);
+    p_next[k + (int64_t)K * (4 * j + l)] = l == 0 ? v[0] : (l == 1 ? v[1] : (l == 2 ? v[2] : v[3]));
   }
 }
 
@@ -638,16 +976,24 @@ hipError_t launch_q_step(int K, int64_t I, const double* res_q, const double* q,
   });
 }
 
+// Four lanes per block QP (k_p_step4); blocks whose free curvature is not positive definite go through
+// redo to the dense one-wave solver, whose grid strides over the list (empty in the usual case).
 hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
-                         int* fails, hipStream_t s, double* dp) {
+                         int* fails, int* redo, hipStream_t s, double* dp) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, redo, 0);
+  const unsigned nq = (unsigned)((4 * J + 255) / 256), nd = (unsigned)(J < 1024 ? J : 1024);
   return with_k(K, [&](auto kc) {
     constexpr int KT = decltype(kc)::value;
-    if (o.penalty)
-      hipLaunchKernelGGL((k_p_step<KT, true>), dim3((unsigned)J), dim3(64), 0, s, J, res_p, p, p_next, o.rho, o.eps,
-                         fails, dp);
-    else
-      hipLaunchKernelGGL((k_p_step<KT, false>), dim3((unsigned)J), dim3(64), 0, s, J, res_p, p, p_next, o.rho, o.eps,
-                         fails, dp);
+    if (o.penalty) {
+      hipLaunchKernelGGL((k_p_step4<KT, true>), dim3(nq), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp, redo);
+      hipLaunchKernelGGL((k_p_step_list<KT, true>), dim3(nd), dim3(64), 0, s, res_p, p, p_next, o.rho, o.eps, fails,
+                         dp, redo);
+    } else {
+      hipLaunchKernelGGL((k_p_step4<KT, false>), dim3(nq), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp,
+                         redo);
+      hipLaunchKernelGGL((k_p_step_list<KT, false>), dim3(nd), dim3(64), 0, s, res_p, p, p_next, o.rho, o.eps, fails,
+                         dp, redo);
+    }
   });
 }
 

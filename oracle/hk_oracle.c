@@ -496,6 +496,14 @@ int hko_qp_solve(int n, const double* H, const double* g, const double* lo, cons
     }
     for (int i = 0; i < n; ++i) d[i] = 0.0;
     for (int a = 0; a < nf; ++a) d[fidx[a]] = -z[a];
+    /* The lone free member of an equality group cannot move (its group sum is fixed): its step is
+     * exactly 0, which keeps round-off from re-blocking it at a bound it was just released from. */
+    for (int c = 0; c < m; ++c) {
+      int cnt = 0, last = -1;
+      for (int a = 0; a < nf; ++a)
+        if (grp[fidx[a]] == c) { ++cnt; last = fidx[a]; }
+      if (cnt == 1) d[last] = 0.0;
+    }
     /* Ratio test. */
     double alpha = 1.0;
     int blk = -1, bside = 0;
