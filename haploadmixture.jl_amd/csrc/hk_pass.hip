@@ -185,12 +185,20 @@ __device__ __forceinline__ TP pair_lik(const TP (&X)[4], const TP* __restrict__ 
 }
 
 // Curvature weights v_h = Y_h / (L X_h) (the factor 2 of c_h / qp_h^2 is in the outer-product tables).
+// Two reciprocals per pair: v_0, v_1 share 1 / (L' X_0 X_1) and v_2, v_3 share 1 / (L' X_2 X_3) (in FP32
+// range for every X_h >= ~3e-10, far below the 1e-5 floor the projections keep).
 template <typename TP>
 __device__ __forceinline__ void pair_v(const TP (&X)[4], const TP* __restrict__ lut, unsigned nib, TP (&v)[4]) {
   const PairForm<TP> f(X, lut, nib);
-#pragma unroll
-  for (int h = 0; h < 4; ++h) v[h] = f.Y[h] * hk_rcp(f.L * X[h]);
+  const TP r01 = hk_rcp(f.L * X[0] * X[1]), r23 = hk_rcp(f.L * X[2] * X[3]);
+  v[0] = f.Y[0] * X[1] * r01;
+  v[1] = f.Y[1] * X[0] * r01;
+  v[2] = f.Y[2] * X[3] * r23;
+  v[3] = f.Y[3] * X[2] * r23;
 }
+
+// Genotype nibble at bit position sh of a word (one bit-field extract).
+__device__ __forceinline__ unsigned nib_at(uint32_t word, unsigned sh) { return __builtin_amdgcn_ubfe(word, sh, 4); }
 
 // -----------------------------------------------------------------------------------------------
 // Q side (individual-parallel): log-likelihood and EM / gradient sums
@@ -581,7 +589,7 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
           X[h] = x;
         }
         TP v[4];
-        pair_v<TP>(X, lut, (word >> (4 * (4 * s + g))) & 15u, v);
+        pair_v<TP>(X, lut, nib_at(word, 4 * (4 * s + g)), v);
         if constexpr (NTILE > 0) {
           TP b[NTA];
 #pragma unroll
@@ -785,7 +793,7 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
             X[h] = x;
           }
           TP v[4];
-          pair_v<TP>(X, lut, (word[r] >> (4 * (4 * s + g))) & 15u, v);
+          pair_v<TP>(X, lut, nib_at(word[r], 4 * (4 * s + g)), v);
 #pragma unroll
           for (int h = 0; h < 4; ++h)
 #pragma unroll

@@ -88,6 +88,24 @@ def test_row_steps_match_oracle(I, J, K, oracle):
         assert rel(e.p_step(q_pass=oq, penalty=False), op2) <= 1e-8
 
 
+@pytest.mark.parametrize("K", [3, 6])
+def test_p_step_four_lane_path_matches_oracle(K, oracle):
+    """P-side row QPs at a size where every block's curvature is well conditioned, so the four-lanes-
+    per-block solver (not its dense fallback) takes them; penalty and equality modes, parameters
+    from a few fit iterations so that many bounds are active."""
+    I, J = 1500, 400
+    print(f"p step four-lane path I={I} J={J} K={K}", flush=True)
+    s, q, p = case(I, J, K, miss=0.02)
+    q, p, _, _, _ = oracle.fit(s.bed, I, J, q, p, em_iters=3, n_iter=2, rtol=0.0)
+    q, p = np.asfortranarray(q), np.asfortranarray(p)
+    with _eng(s, q, p) as e:
+        XtX, Xtz = oracle.update_p(s.bed, I, J, q, p)
+        for pen in (True, False):
+            op, fails = oracle.p_step(XtX, Xtz, p, penalty=pen)
+            assert fails == 0
+            assert rel(e.p_step(q_pass=q, penalty=pen), op) <= 1e-8, pen
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_fit_matches_oracle_fit(prec, oracle):
     """init_em! + admixture_qn! on the device vs the CPU restatement of the same loop."""
