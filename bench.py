@@ -264,11 +264,13 @@ def main():
     breakdown["EM (ms)"] = round(sum(x["t_em_ms"] for x in runs) / a.steps, 2)
     breakdown["ZAL+proj (ms)"] = round(sum(x["t_qn_ms"] for x in runs) / a.steps, 2)
 
+    # HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC runs (scaled
+    # FETCH_SIZE + WRITE_SIZE), keyed by workload and pass
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        t = json.load(open(tpath)).get(a.workload)
-        if t and prec == "mixed" and "pside" in name:
+    if os.path.exists(tpath) and prec == "mixed":
+        t = json.load(open(tpath)).get(a.workload, {}).get(name)
+        if t:
             traffic = t["hbm_bytes_per_launch"]
     headline = a.workload == "mixed_I20k_J200k_K6" and prec == "mixed"
     out = {

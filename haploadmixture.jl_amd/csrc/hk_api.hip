@@ -199,6 +199,7 @@ static int run_qpass(hk_ctx* c, int mode, const double* q, const double* p, Time
                      double* ll_out = nullptr) {
   const void *qq, *pp;
   CHK(pass_params(c, q, p, &qq, &pp));
+  if (mode == MODE_CURV) HIPC(c, launch_curv_tables(c->K, c->prec, 0, c->g, qq, pp, c->s));
   if (tm) tm->begin(cat);
   HIPC(c, launch_qside(c->K, c->prec, mode, c->g, qq, pp, c->part_q, c->s));
   if (tm) tm->end();
@@ -214,6 +215,7 @@ static int run_ppass(hk_ctx* c, int mode, const double* q, const double* p, Time
   const void *qq, *pp;
   CHK(pass_params(c, q, p, &qq, &pp));
   const int NT = nacc(c->K, mode);
+  if (mode == MODE_CURV) HIPC(c, launch_curv_tables(c->K, c->prec, 1, c->g, qq, pp, c->s));
   if (c->g.pchunks == 1) {
     if (tm) tm->begin(cat);
     HIPC(c, launch_pside(c->K, c->prec, mode, c->g, qq, pp, c->res_p, c->s));
@@ -993,8 +995,13 @@ extern "C" int hk_time_pass(hk_ctx* c, int kind, int reps, double* ms) {
     const void *qq, *pp;
     rc = pass_params(c, qof(c, x), pof(c, x), &qq, &pp);
     if (rc != HK_OK) break;
+    hipError_t e = hipSuccess;
+    if (kind == 1 || kind == 2) e = launch_curv_tables(c->K, c->prec, kind == 1 ? 0 : 1, c->g, qq, pp, c->s);
+    if (e != hipSuccess) {
+      rc = fail(c, HK_EHIP, "timing failed: %s", hipGetErrorString(e));
+      break;
+    }
     (void)hipEventRecord(a, c->s);
-    hipError_t e;
     if (kind == 0) e = launch_qside(c->K, c->prec, MODE_LL, c->g, qq, pp, c->part_q, c->s);
     else if (kind == 1) e = launch_qside(c->K, c->prec, MODE_CURV, c->g, qq, pp, c->part_q, c->s);
     else if (kind == 2)

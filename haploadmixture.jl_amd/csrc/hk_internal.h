@@ -28,6 +28,10 @@ void stage_sizes(int K, int64_t Ip, int64_t Jp, int64_t* nq, int64_t* np);
 hipError_t launch_repack(const uint8_t* raw, int64_t bps, const PassGeom& g, hipStream_t s);
 hipError_t launch_count_dm(const PassGeom& g, int64_t* dm, hipStream_t s);
 hipError_t launch_decode(const PassGeom& g, uint8_t* codes, hipStream_t s);
+// MODE_CURV passes read staging tables built from the same (q, p) by launch_curv_tables (side 0: the
+// Q-side pass, side 1: the P-side pass), which must be enqueued first.
+hipError_t launch_curv_tables(int K, int prec, int side, const PassGeom& g, const void* q, const void* p,
+                              hipStream_t s);
 hipError_t launch_qside(int K, int prec, int mode, const PassGeom& g, const void* q, const void* p, double* part,
                         hipStream_t s);
 hipError_t launch_pside(int K, int prec, int mode, const PassGeom& g, const void* q, const void* p, double* out,

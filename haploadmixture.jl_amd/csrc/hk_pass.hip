@@ -1016,10 +1016,6 @@ static hipError_t qside_k(int mode, const PassGeom& g, const void* q, const void
   else if (mode == MODE_GRAD)
     hipLaunchKernelGGL((k_qside<K, TP, MODE_GRAD>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part);
   else {
-    using S = CurvShape<K>;
-    const int64_t nrow = g.Jp + S::PAD_ROWS;
-    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGQ + 255) / 256)), dim3(256), 0, s, pp,
-                       4 * g.Jp, nrow, 4, S::QS, S::RGQ, (TP*)g.pp);
     const dim3 gmm((unsigned)(g.Ip / 128), (unsigned)g.qchunks);
     if constexpr (K <= 6 && sizeof(TP) == 4)
       hipLaunchKernelGGL((k_qside_mm_pf<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
@@ -1041,10 +1037,6 @@ static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void
     hipLaunchKernelGGL((k_pside_grad<K, TP>), dim3((unsigned)(g.Jp / 256 + (g.Jp % 256 ? 1 : 0)), (unsigned)g.pchunks),
                        block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out);
   else {
-    using S = CurvShape<K>;
-    const int64_t nrow = g.Ip + S::PAD_ROWS;
-    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGP + 255) / 256)), dim3(256), 0, s, qq,
-                       g.I, nrow, 1, S::QS, S::RGP, (TP*)g.qq);
     const dim3 gmm((unsigned)(g.Jp / 64), (unsigned)g.pchunks);
     if constexpr (K <= 6 && sizeof(TP) == 4)
       hipLaunchKernelGGL((k_pside_mm_pf<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
@@ -1052,6 +1044,23 @@ static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void
     else
       hipLaunchKernelGGL((k_pside_mm<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
                          (const TP*)g.qq, out);
+  }
+  return hipGetLastError();
+}
+
+// Staging tables of the curvature passes from the pass parameters: the Q-side pass reads the p table
+// (rows per block), the P-side pass the q table (rows per individual).
+template <int K, typename TP>
+static hipError_t tables_k(int side, const PassGeom& g, const void* x, hipStream_t s) {
+  using S = CurvShape<K>;
+  if (side == 0) {
+    const int64_t nrow = g.Jp + S::PAD_ROWS;
+    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGQ + 255) / 256)), dim3(256), 0, s,
+                       (const TP*)x, 4 * g.Jp, nrow, 4, S::QS, S::RGQ, (TP*)g.pp);
+  } else {
+    const int64_t nrow = g.Ip + S::PAD_ROWS;
+    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGP + 255) / 256)), dim3(256), 0, s,
+                       (const TP*)x, g.I, nrow, 1, S::QS, S::RGP, (TP*)g.qq);
   }
   return hipGetLastError();
 }
@@ -1073,6 +1082,13 @@ hipError_t launch_qside(int K, int prec, int mode, const PassGeom& g, const void
                         hipStream_t s) {
   if (prec == PREC_FP64) { HK_K_SWITCH(qside_k, double, mode, g, q, p, part, s) }
   HK_K_SWITCH(qside_k, float, mode, g, q, p, part, s)
+}
+
+hipError_t launch_curv_tables(int K, int prec, int side, const PassGeom& g, const void* q, const void* p,
+                              hipStream_t s) {
+  const void* x = side == 0 ? p : q;
+  if (prec == PREC_FP64) { HK_K_SWITCH(tables_k, double, side, g, x, s) }
+  HK_K_SWITCH(tables_k, float, side, g, x, s)
 }
 
 hipError_t launch_pside(int K, int prec, int mode, const PassGeom& g, const void* q, const void* p, double* out,
