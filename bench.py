@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-7)
     ap.add_argument("--max-iter", type=int, default=1000)
     ap.add_argument("--no-fp64-ref", action="store_true", help="skip the plain FP64 GPU fit")
+    ap.add_argument("--no-fp32acc-ref", action="store_true",
+                    help="skip the fit with the reference's T2 = Float32 semantics (FP32 accumulation)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-blocks", type=int, default=1000)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -319,6 +321,20 @@ def main():
                            "speedup_mixed_over_fp64": round(r64["seconds"] / r["seconds"], 3)}
         out["time_to_1e-6_of_fp64_ll_s"] = round(float(tt[hit[0]]), 3) if hit.size else None
         log(f"fp64: ll={r64['ll']:.10g} iters={r64['iters']} {r64['seconds']:.2f}s; gap {gap:.2e}")
+        if not a.no_fp32acc_ref:
+            log("FP32-accumulation fit (the reference's T2 = Float32 semantics) on the same data")
+            e32 = HipAdmixData(data.bed, I, J, K, precision="fp32acc")
+            e32.set_params(q0, p0)
+            r32 = e32.fit(rtol=a.rtol, n_iter=a.max_iter, profile=0, verbose=0)
+            e32.close()
+            out["fp32_accumulation_gpu"] = {
+                "seconds": round(r32["seconds"], 3), "ll": r32["ll"], "iters": r32["iters"],
+                "converged": bool(r32["converged"]),
+                "rel_ll_gap_vs_fp64": abs(r32["ll"] - r64["ll"]) / abs(r64["ll"]),
+                "note": "FP32 products and FP32 gradient/curvature sums (reference T2 = Float32); "
+                        "the MIXED engine adds FP64 accumulation"}
+            log(f"fp32 accumulation: ll={r32['ll']:.10g} iters={r32['iters']} gap "
+                f"{out['fp32_accumulation_gpu']['rel_ll_gap_vs_fp64']:.2e}")
     if not a.no_cpu_baseline:
         js = min(a.cpu_sample_blocks, J)
         log(f"CPU baseline: oracle on {js} blocks")

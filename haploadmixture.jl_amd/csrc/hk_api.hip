@@ -1,6 +1,7 @@
 // hk_api.hip — C ABI of libhaplo_hip.so: context, parameter transfer, the reference's GPU runner entry
 // points and the device-resident QN/SQP fit loop.  Host code; the kernels live in hk_pass.hip and
 // hk_step.hip.
+#include <limits.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -94,8 +95,10 @@ static void plan_chunks(hk_ctx* c) {
   PassGeom& g = c->g;
   constexpr int64_t kTargetWG = 8192, kMinWords = 16;
   const int64_t wgq = g.Ip / 256, wgp = 4 * g.Jp / 256;
-  int64_t qc = c->qchunks_req > 0 ? c->qchunks_req : (kTargetWG + wgq - 1) / wgq;
-  int64_t pc = c->pchunks_req > 0 ? c->pchunks_req : (kTargetWG + wgp - 1) / wgp;
+  // FP32 accumulation (the reference's Float32 sums) runs each sum over the whole i / j range
+  const int64_t auto_q = g.acc32 ? 1 : (kTargetWG + wgq - 1) / wgq, auto_p = g.acc32 ? 1 : (kTargetWG + wgp - 1) / wgp;
+  int64_t qc = c->qchunks_req > 0 ? c->qchunks_req : auto_q;
+  int64_t pc = c->pchunks_req > 0 ? c->pchunks_req : auto_p;
   if (c->qchunks_req <= 0 && qc > g.J8 / kMinWords) qc = g.J8 / kMinWords;
   if (c->pchunks_req <= 0 && pc > g.I8 / kMinWords) pc = g.I8 / kMinWords;
   if (qc < 1) qc = 1;
@@ -307,12 +310,23 @@ static int init_fit_buffers(hk_ctx* c) {
   return HK_OK;
 }
 
+static bool valid_precision(int p) { return p == HK_PREC_FP64 || p == HK_PREC_MIXED || p == HK_PREC_FP32; }
+
+// Kernel precision (FP64 or FP32 per-pair arithmetic) and accumulation semantics of an ABI precision.
+static void set_precision(hk_ctx* c, int precision) {
+  c->prec = precision == HK_PREC_FP64 ? PREC_FP64 : PREC_MIXED;
+  const bool acc32 = precision == HK_PREC_FP32;
+  c->g.acc32 = acc32 ? 1 : 0;
+  c->g.flush_curv = acc32 ? INT_MAX : 16;
+  c->g.flush_grad = acc32 ? INT_MAX : 8;
+}
+
 int hk_ctx_create(hk_ctx** out, int64_t I, int64_t J, int K, const uint8_t* bed, int64_t bps, int64_t n_snp,
                   int precision) {
   if (!out) return fail(nullptr, HK_EINVAL, "out is NULL");
   *out = nullptr;
   if (I <= 0 || J <= 0 || K < 1 || K > HK_MAX_K || !bed || bps < (I + 3) / 4 || n_snp < 2 * J ||
-      (precision != HK_PREC_FP64 && precision != HK_PREC_MIXED))
+      !valid_precision(precision))
     return fail(nullptr, HK_EINVAL,
                 "invalid arguments: I=%lld J=%lld K=%d bytes_per_snp=%lld n_snp=%lld precision=%d (need K in 1..%d, "
                 "bytes_per_snp >= ceil(I/4), n_snp >= 2J)",
@@ -323,7 +337,7 @@ int hk_ctx_create(hk_ctx** out, int64_t I, int64_t J, int K, const uint8_t* bed,
   c->I = I;
   c->J = J;
   c->K = K;
-  c->prec = precision;
+  set_precision(c, precision);
   int rc = HK_OK;
   hipError_t e = hipGetDevice(&c->dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
@@ -367,7 +381,7 @@ int hk_ctx_create(hk_ctx** out, int64_t I, int64_t J, int K, const uint8_t* bed,
 int hk_ctx_create_shared(hk_ctx** out, hk_ctx* base, int K, int precision) {
   if (!out || !base) return fail(nullptr, HK_EINVAL, "null pointer");
   *out = nullptr;
-  if (K < 1 || K > HK_MAX_K || (precision != HK_PREC_FP64 && precision != HK_PREC_MIXED))
+  if (K < 1 || K > HK_MAX_K || !valid_precision(precision))
     return fail(nullptr, HK_EINVAL, "invalid K=%d or precision=%d", K, precision);
   hipError_t e = hipSetDevice(base->dev);
   if (e != hipSuccess) return fail(nullptr, HK_EHIP, "hipSetDevice: %s", hipGetErrorString(e));
@@ -375,10 +389,10 @@ int hk_ctx_create_shared(hk_ctx** out, hk_ctx* base, int K, int precision) {
   c->I = base->I;
   c->J = base->J;
   c->K = K;
-  c->prec = precision;
   c->dev = base->dev;
   c->owns_geno = false;
   c->g = base->g;
+  set_precision(c, precision);
   c->gq = base->gq;
   c->gp = base->gp;
   c->dm = base->dm;

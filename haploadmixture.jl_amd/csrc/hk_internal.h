@@ -17,6 +17,9 @@ struct PassGeom {
   const uint32_t* gp;  // [I8][Jp]
   int qchunks, qwpc;   // Q side: number of j chunks, words per chunk
   int pchunks, pwpc;   // P side: number of i chunks, words per chunk
+  int flush_curv;      // MIXED: genotype words per FP32 partial of the curvature passes (then FP64)
+  int flush_grad;      // MIXED: genotype words per FP32 partial of the gradient (EM) passes
+  int acc32;           // reference Float32 semantics: no FP64 flushes, chunk partials summed in FP32
   void* qq;            // staging table of the P-side curvature pass (TP; rows per individual)
   void* pp;            // staging table of the Q-side curvature pass (TP; rows per block)
 };

@@ -216,7 +216,7 @@ struct NAcc {
 template <int K, typename TP, int MODE>
 __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
                                                int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
-                                               double* __restrict__ part) {
+                                               double* __restrict__ part, int flush) {
   constexpr int NT = NAcc<K, MODE>::v;
   constexpr bool F32 = sizeof(TP) == 4;
   __shared__ __attribute__((aligned(16))) TP lut[128];
@@ -235,8 +235,8 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = 0.0;
   double llm = 1.0;
-  int lle = 0;
-  TP pa[NT];  // FP32 partials (MIXED), flushed to acc every 8 words
+  int lle = 0, nfl = 0;
+  TP pa[NT];  // FP32 partials (MIXED), flushed to acc every `flush` words
 #pragma unroll
   for (int t = 0; t < NT; ++t) pa[t] = (TP)0;
 
@@ -281,7 +281,8 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
       llm = frexp(llm, &e);
       lle += e;
     } else if constexpr (F32) {
-      if (((jw - jw0) & 7) == 7) {
+      if (++nfl == flush) {
+        nfl = 0;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           acc[t] += (double)pa[t];
@@ -309,7 +310,7 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
 template <int K, typename TP>
 __global__ __launch_bounds__(256) void k_pside_grad(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8,
                                                     int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
-                                                    double* __restrict__ out) {
+                                                    double* __restrict__ out, int flush) {
   constexpr bool F32 = sizeof(TP) == 4;
   __shared__ __attribute__((aligned(16))) TP lut[128];
   fill_pair_lut<TP>(lut);
@@ -326,6 +327,7 @@ __global__ __launch_bounds__(256) void k_pside_grad(const uint32_t* __restrict__
     for (int k = 0; k < K; ++k) P[h][k] = p[k + (int64_t)K * (4 * j + h)];
   double acc[4][K];
   TP pa[4][K];
+  int nfl = 0;
 #pragma unroll
   for (int h = 0; h < 4; ++h)
 #pragma unroll
@@ -361,7 +363,8 @@ __global__ __launch_bounds__(256) void k_pside_grad(const uint32_t* __restrict__
       }
     }
     if constexpr (F32) {
-      if (((iw - iw0) & 7) == 7) {
+      if (++nfl == flush) {
+        nfl = 0;
 #pragma unroll
         for (int h = 0; h < 4; ++h)
 #pragma unroll
@@ -505,7 +508,7 @@ __device__ __forceinline__ void tail_flush_p(float (&A)[4][NR], double (&DA)[NR]
 template <int K, typename TP, bool PF>
 __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8, int wpc,
                                               const TP* __restrict__ q, const TP* __restrict__ p,
-                                              const TP* __restrict__ qq, double* __restrict__ out) {
+                                              const TP* __restrict__ qq, double* __restrict__ out, int flush) {
   using S = CurvShape<K>;
   constexpr int NKK = S::NKK, NT = K + NKK, NTILE = S::NTILE, NR = S::NR, QS = S::QS, T0 = S::TAIL0;
   constexpr int RG = S::RGP, RL = S::RLP;
@@ -610,7 +613,7 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
         }
       }
       if constexpr (F32) {
-        if (++nword == 16) {
+        if (++nword == flush) {
           nword = 0;
 #pragma unroll
           for (int h = 0; h < 4; ++h)
@@ -673,15 +676,15 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
 template <int K, typename TP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pside_mm_pf(
     const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8, int wpc, const TP* __restrict__ q,
-    const TP* __restrict__ p, const TP* __restrict__ qq, double* __restrict__ out) {
-  pside_mm_body<K, TP, true>(gp, Jp, I, I8, wpc, q, p, qq, out);
+    const TP* __restrict__ p, const TP* __restrict__ qq, double* __restrict__ out, int flush) {
+  pside_mm_body<K, TP, true>(gp, Jp, I, I8, wpc, q, p, qq, out, flush);
 }
 // waves_per_eu(2) keeps K = 8 at two waves per SIMD (and K = 7 at three) without spills.
 template <int K, typename TP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pside_mm(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8,
                                                   int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
-                                                  const TP* __restrict__ qq, double* __restrict__ out) {
-  pside_mm_body<K, TP, false>(gp, Jp, I, I8, wpc, q, p, qq, out);
+                                                  const TP* __restrict__ qq, double* __restrict__ out, int flush) {
+  pside_mm_body<K, TP, false>(gp, Jp, I, I8, wpc, q, p, qq, out, flush);
 }
 
 // Q side: a wave owns 2 x 16 individuals (two A row tiles); each MFMA step takes 4 blocks (A columns /
@@ -690,7 +693,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 template <int K, typename TP, bool PF>
 __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8, int wpc,
                                               const TP* __restrict__ q, const TP* __restrict__ p,
-                                              const TP* __restrict__ pp, double* __restrict__ part) {
+                                              const TP* __restrict__ pp, double* __restrict__ part, int flush) {
   using S = CurvShape<K>;
   constexpr int NKK = S::NKK, NT = K + NKK, NTILE = S::NTILE, NR = S::NR, QS = S::QS, T0 = S::TAIL0;
   constexpr int RG = S::RGQ, RL = S::RLQ;
@@ -808,7 +811,7 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
         }
       }
       if constexpr (F32) {
-        if (++nword == 16) {
+        if (++nword == flush) {
           nword = 0;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
@@ -864,14 +867,14 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
 template <int K, typename TP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_qside_mm_pf(
     const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8, int wpc, const TP* __restrict__ q,
-    const TP* __restrict__ p, const TP* __restrict__ pp, double* __restrict__ part) {
-  qside_mm_body<K, TP, true>(gq, Ip, I, J8, wpc, q, p, pp, part);
+    const TP* __restrict__ p, const TP* __restrict__ pp, double* __restrict__ part, int flush) {
+  qside_mm_body<K, TP, true>(gq, Ip, I, J8, wpc, q, p, pp, part, flush);
 }
 template <int K, typename TP>
 __global__ __launch_bounds__(256) void k_qside_mm(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
                                                   int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
-                                                  const TP* __restrict__ pp, double* __restrict__ part) {
-  qside_mm_body<K, TP, false>(gq, Ip, I, J8, wpc, q, p, pp, part);
+                                                  const TP* __restrict__ pp, double* __restrict__ part, int flush) {
+  qside_mm_body<K, TP, false>(gq, Ip, I, J8, wpc, q, p, pp, part, flush);
 }
 
 // -----------------------------------------------------------------------------------------------
@@ -896,12 +899,20 @@ __device__ __forceinline__ double chunk_sum(const double* __restrict__ part, int
 }
 
 // part[c][t][Ip] -> res[i][NT] for i < I, slots t0 + blockIdx.y (thread per (i, t)).
+// acc32: the chunk partials are summed in FP32 (the reference's Float32 accumulators).
 __global__ void k_reduce_q(const double* __restrict__ part, int nchunk, int NT, int t0, int64_t Ip, int64_t I,
-                           double* __restrict__ res) {
+                           double* __restrict__ res, int acc32) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int t = t0 + blockIdx.y;
   if (i >= I) return;
-  res[i * NT + t] = chunk_sum(part, nchunk, (int64_t)NT * Ip, (int64_t)t * Ip + i);
+  const int64_t stride = (int64_t)NT * Ip, off = (int64_t)t * Ip + i;
+  if (acc32) {
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += (float)part[(int64_t)c * stride + off];
+    res[i * NT + t] = s;
+  } else {
+    res[i * NT + t] = chunk_sum(part, nchunk, stride, off);
+  }
 }
 
 // Curvature pass: Xtz = XtX q (FP64 q) into slots 0..K-1 of res[i][NT], after k_reduce_q filled the XtX slots.
@@ -941,12 +952,18 @@ __global__ void k_finish_p(int K, int64_t J, const double* __restrict__ p, doubl
 
 // part[c][4Jp][NT] -> res[jh][NT] for jh < 4J
 __global__ void k_reduce_p(const double* __restrict__ part, int nchunk, int NT, int64_t Jp, int64_t J,
-                           double* __restrict__ res) {
+                           double* __restrict__ res, int acc32) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= 4 * J * NT) return;
-  double s = 0.0;
-  for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * 4 * Jp * NT + idx];
-  res[idx] = s;
+  if (acc32) {
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += (float)part[(int64_t)c * 4 * Jp * NT + idx];
+    res[idx] = s;
+  } else {
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * 4 * Jp * NT + idx];
+    res[idx] = s;
+  }
 }
 
 // Log-likelihood, stage 1: rows[i] = sum_c part[c][i]; bpart[b] = sum of the rows of workgroup b (tree).
@@ -1012,17 +1029,19 @@ static hipError_t qside_k(int mode, const PassGeom& g, const void* q, const void
   const TP* qq = (const TP*)q;
   const TP* pp = (const TP*)p;
   if (mode == MODE_LL)
-    hipLaunchKernelGGL((k_qside<K, TP, MODE_LL>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part);
+    hipLaunchKernelGGL((k_qside<K, TP, MODE_LL>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part,
+                       g.flush_grad);
   else if (mode == MODE_GRAD)
-    hipLaunchKernelGGL((k_qside<K, TP, MODE_GRAD>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part);
+    hipLaunchKernelGGL((k_qside<K, TP, MODE_GRAD>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part,
+                       g.flush_grad);
   else {
     const dim3 gmm((unsigned)(g.Ip / 128), (unsigned)g.qchunks);
     if constexpr (K <= 6 && sizeof(TP) == 4)
       hipLaunchKernelGGL((k_qside_mm_pf<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
-                         (const TP*)g.pp, part);
+                         (const TP*)g.pp, part, g.flush_curv);
     else
       hipLaunchKernelGGL((k_qside_mm<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
-                         (const TP*)g.pp, part);
+                         (const TP*)g.pp, part, g.flush_curv);
   }
   return hipGetLastError();
 }
@@ -1035,15 +1054,15 @@ static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void
   const TP* pp = (const TP*)p;
   if (mode == MODE_GRAD)
     hipLaunchKernelGGL((k_pside_grad<K, TP>), dim3((unsigned)(g.Jp / 256 + (g.Jp % 256 ? 1 : 0)), (unsigned)g.pchunks),
-                       block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out);
+                       block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out, g.flush_grad);
   else {
     const dim3 gmm((unsigned)(g.Jp / 64), (unsigned)g.pchunks);
     if constexpr (K <= 6 && sizeof(TP) == 4)
       hipLaunchKernelGGL((k_pside_mm_pf<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
-                         (const TP*)g.qq, out);
+                         (const TP*)g.qq, out, g.flush_curv);
     else
       hipLaunchKernelGGL((k_pside_mm<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
-                         (const TP*)g.qq, out);
+                         (const TP*)g.qq, out, g.flush_curv);
   }
   return hipGetLastError();
 }
@@ -1142,7 +1161,7 @@ hipError_t launch_decode(const PassGeom& g, uint8_t* codes, hipStream_t s) {
 
 hipError_t launch_reduce_q(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s) {
   hipLaunchKernelGGL(k_reduce_q, dim3((unsigned)((g.I + 255) / 256), (unsigned)NT), dim3(256), 0, s, part, nchunk, NT, 0,
-                     g.Ip, g.I, res);
+                     g.Ip, g.I, res, g.acc32);
   return hipGetLastError();
 }
 
@@ -1150,7 +1169,8 @@ hipError_t launch_reduce_q_curv(const double* part, int nchunk, int K, const Pas
                                hipStream_t s) {
   const int NT = K + K * (K + 1) / 2;
   const unsigned nb = (unsigned)((g.I + 255) / 256);
-  hipLaunchKernelGGL(k_reduce_q, dim3(nb, (unsigned)(NT - K)), dim3(256), 0, s, part, nchunk, NT, K, g.Ip, g.I, res);
+  hipLaunchKernelGGL(k_reduce_q, dim3(nb, (unsigned)(NT - K)), dim3(256), 0, s, part, nchunk, NT, K, g.Ip, g.I, res,
+                     g.acc32);
   hipLaunchKernelGGL(k_xtz_q, dim3(nb), dim3(256), 0, s, K, g.I, q, res);
   return hipGetLastError();
 }
@@ -1162,7 +1182,8 @@ hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* re
 
 hipError_t launch_reduce_p(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s) {
   int64_t n = 4 * g.J * NT;
-  hipLaunchKernelGGL(k_reduce_p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nchunk, NT, g.Jp, g.J, res);
+  hipLaunchKernelGGL(k_reduce_p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nchunk, NT, g.Jp, g.J, res,
+                     g.acc32);
   return hipGetLastError();
 }
 

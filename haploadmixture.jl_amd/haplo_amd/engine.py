@@ -15,7 +15,7 @@ import numpy as np
 from . import _native
 from ._native import FitOpts, FitResult, HipError, ptr
 
-PRECISIONS = {"fp64": 0, "mixed": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp32acc": 2}
 
 
 def _F(a, shape):

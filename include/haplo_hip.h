@@ -39,7 +39,10 @@ enum hk_status {
 
 enum hk_precision {
   HK_PREC_FP64 = 0,  /* every per-pair product and every sum in FP64 (reference T = T2 = Float64) */
-  HK_PREC_MIXED = 1  /* FP32 per-pair products, FP64 accumulation, FP64 log-likelihood, QP and QN */
+  HK_PREC_MIXED = 1, /* FP32 per-pair products, FP64 accumulation, FP64 log-likelihood, QP and QN */
+  HK_PREC_FP32 = 2   /* the reference's mixed mode T2 = Float32 (driver.jl:149, algorithms_inner.jl:115-130):
+                        FP32 per-pair products AND FP32 gradient / curvature accumulation; log-likelihood
+                        sum, QP and QN in FP64.  A comparison point for HK_PREC_MIXED. */
 };
 
 #define HK_MAX_K 8

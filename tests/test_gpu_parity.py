@@ -68,6 +68,29 @@ def test_mixed_passes_match_oracle(I, J, K, oracle):
         assert rel(XtX, oX) <= 2e-5 and rel(Xtz, oz) <= 2e-5
 
 
+def test_fp32_accumulation_mode(oracle):
+    """The reference's T2 = Float32 semantics (HK_PREC_FP32): FP32 products and FP32 gradient / curvature
+    sums, each over the whole i or j range.  Its passes stay within FP32-accumulation error of the FP64
+    oracle, and on the P-side curvature (a sum over 20,000 individuals) the MIXED engine -- FP64
+    accumulation of the same FP32 products -- is clearly closer, which is the point of the comparison."""
+    I, J, K = 20000, 64, 4
+    print(f"fp32 accumulation I={I} J={J} K={K}", flush=True)
+    s, q, p = case(I, J, K, miss=0.02)
+    oX, oz = oracle.update_p(s.bed, I, J, q, p)
+    oXq, ozq = oracle.update_q(s.bed, I, J, q, p)
+    oll = oracle.loglik(s.bed, I, J, q, p)
+    errs = {}
+    for prec in ("fp32acc", "mixed"):
+        with _eng(s, q, p, prec) as e:
+            assert abs(e.loglikelihood() - oll) <= 2e-6 * abs(oll)
+            XtX, Xtz = e.update_p()
+            XtXq, Xtzq = e.update_q()
+            errs[prec] = rel(XtX, oX)
+            assert max(rel(Xtz, oz), rel(XtXq, oXq), rel(Xtzq, ozq)) <= (1e-3 if prec == "fp32acc" else 2e-5)
+    assert errs["fp32acc"] <= 1e-3 and errs["mixed"] <= 2e-5, errs
+    assert errs["mixed"] * 4 <= errs["fp32acc"], errs
+
+
 @pytest.mark.parametrize("I,J,K", [(13, 7, 2), (50, 33, 4), (301, 130, 5), (257, 65, 6), (77, 90, 8)])
 def test_row_steps_match_oracle(I, J, K, oracle):
     """update_q!/update_p! QP + step + projection on the device vs the CPU restatement."""
