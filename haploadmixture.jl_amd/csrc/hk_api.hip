@@ -584,6 +584,7 @@ void hk_fit_opts_default(hk_fit_opts* o) {
   o->use_graph = 1;
   o->em_fix_q = 0;
   o->guarantee_increase = 0;
+  o->qn_mode = 0;
 }
 
 }  // extern "C"
@@ -680,7 +681,7 @@ static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm, double 
   else HIPC(c, launch_copy_f64(np, pof(c, x), pof(c, xn2), c->s));
   CHK(run_qpass(c, MODE_LL, qof(c, xn2), pof(c, xn2), tm, T_LL, c->d_ll + 0));
   if (tm) tm->begin(T_QN);
-  HIPC(c, launch_zal(N, o->Qn, c->d_iter, c->U, c->V, x, xn, xn2, xt, c->zal_part, c->zal_coef, c->s));
+  HIPC(c, launch_zal(N, o->Qn, c->d_iter, c->U, c->V, x, xn, xn2, xt, c->zal_part, c->zal_coef, c->s, o->qn_mode));
   HIPC(c, launch_project_p(K, J, pof(c, xt), o->eps, c->s));
   HIPC(c, launch_project_q(K, I, qof(c, xt), o->eps, c->s));
   if (tm) tm->end();
@@ -695,6 +696,7 @@ static unsigned long long graph_key(const hk_ctx* c, const hk_fit_opts* o) {
   auto mix = [&](unsigned long long v) { k = (k ^ v) * 1099511628211ull; };
   mix((unsigned)o->fix_q); mix((unsigned)o->fix_p); mix((unsigned)o->q_penalty); mix((unsigned)o->p_penalty);
   mix((unsigned)o->Qn); mix(*(const unsigned long long*)&o->rho); mix(*(const unsigned long long*)&o->eps);
+  mix((unsigned)o->qn_mode);
   mix((unsigned)c->g.qchunks); mix((unsigned)c->g.pchunks);
   return k;
 }
@@ -792,7 +794,7 @@ struct FitRun {
 static int fit_prepare(FitRun& f, double ll_start) {
   hk_ctx* c = f.c;
   const hk_fit_opts* o = f.o;
-  if (o->Qn < 1 || o->Qn > 4 || o->n_iter < 0 || o->em_iters < 0)
+  if (o->Qn < 1 || o->Qn > 4 || o->n_iter < 0 || o->em_iters < 0 || o->qn_mode < 0 || o->qn_mode > 1)
     return fail(c, HK_EINVAL, "invalid fit options (Qn in 1..4)");
   memset(f.res, 0, sizeof(*f.res));
   f.t0 = std::chrono::steady_clock::now();

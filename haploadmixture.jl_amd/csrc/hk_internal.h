@@ -69,8 +69,9 @@ hipError_t launch_em_q(int K, int64_t I, int64_t J, const double* res_q, int NT,
 hipError_t launch_em_p(int K, int64_t J, const double* res_p, int NT, const double* p, double* p_next, double eps,
                        hipStream_t s);
 hipError_t launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
+// Quasi-Newton extrapolation of admixture_qn! (mode 0: ZAL, 1: LBQN); dots_part holds 512 x 64 doubles.
 hipError_t launch_zal(int64_t N, int Qn, const int* d_iter, double* U, double* V, const double* x0, const double* x1,
-                      const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s);
+                      const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s, int mode);
 hipError_t launch_accept(int64_t N, double* ll, const double* x_basic, const double* x_qn, double* x, int* d_iter,
                          hipStream_t s);
 hipError_t launch_copy_f64(int64_t n, const double* src, double* dst, hipStream_t s);

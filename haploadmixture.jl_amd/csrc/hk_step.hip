@@ -810,7 +810,7 @@ __global__ void k_axpy(int64_t n, const double* __restrict__ a, const double* __
 // x_out = x1 - V (U'U - U'V)^{-1} U'(x0 - x1).  Qn <= 4.
 // ------------------------------------------------------------------------------------------------
 #define ZAL_QMAX 4
-#define ZAL_NDOT (ZAL_QMAX * ZAL_QMAX * 2 + ZAL_QMAX)
+#define ZAL_NDOT (ZAL_QMAX * ZAL_QMAX * 3 + ZAL_QMAX * 2)  // U'U, U'V, V'V, U'r, V'r
 #define ZAL_NB 512
 
 // The QN iteration i (1-based) is read from device memory so that an iteration can be replayed from a
@@ -843,14 +843,17 @@ __global__ __launch_bounds__(256) void k_zal_dots(int64_t N, const int* __restri
       ua[a] = (a < ncols) ? (a == col ? u : U[t + N * a]) : 0.0;
       va[a] = (a < ncols) ? (a == col ? v : V[t + N * a]) : 0.0;
     }
+    constexpr int Q2 = ZAL_QMAX * ZAL_QMAX;
 #pragma unroll
     for (int a = 0; a < ZAL_QMAX; ++a) {
 #pragma unroll
       for (int b = 0; b < ZAL_QMAX; ++b) {
         acc[a * ZAL_QMAX + b] = __builtin_fma(ua[a], ua[b], acc[a * ZAL_QMAX + b]);
-        acc[ZAL_QMAX * ZAL_QMAX + a * ZAL_QMAX + b] = __builtin_fma(ua[a], va[b], acc[ZAL_QMAX * ZAL_QMAX + a * ZAL_QMAX + b]);
+        acc[Q2 + a * ZAL_QMAX + b] = __builtin_fma(ua[a], va[b], acc[Q2 + a * ZAL_QMAX + b]);
+        acc[2 * Q2 + a * ZAL_QMAX + b] = __builtin_fma(va[a], va[b], acc[2 * Q2 + a * ZAL_QMAX + b]);
       }
-      acc[2 * ZAL_QMAX * ZAL_QMAX + a] = __builtin_fma(ua[a], rr, acc[2 * ZAL_QMAX * ZAL_QMAX + a]);
+      acc[3 * Q2 + a] = __builtin_fma(ua[a], rr, acc[3 * Q2 + a]);
+      acc[3 * Q2 + ZAL_QMAX + a] = __builtin_fma(va[a], rr, acc[3 * Q2 + ZAL_QMAX + a]);
     }
   }
   for (int d = 0; d < ZAL_NDOT; ++d) {
@@ -860,8 +863,14 @@ __global__ __launch_bounds__(256) void k_zal_dots(int64_t N, const int* __restri
   }
 }
 
+// Coefficients of the quasi-Newton point x_qn = F(x) - V c (r = x - F(x), U / V the secant columns):
+//   mode 0 (ZAL, Zhou-Alexander-Lange):     (U'U - U'V) c = U' r
+//   mode 1 (LBQN, limited-memory Broyden):  (Y'Y) c = -Y' r,  Y = V - U
+// Both are multisecant Broyden updates from H0 = -I of the inverse Jacobian of G(x) = F(x) - x: ZAL the
+// first ("good") kind, LBQN the second ("bad") kind, H = -I + (U + Y)(Y'Y)^-1 Y', which satisfies the
+// secants H Y = U; x_qn = x - H G(x).
 __global__ void k_zal_solve(int nb, const int* __restrict__ d_iter, int Qn, const double* __restrict__ part,
-                            double* __restrict__ coef) {
+                            double* __restrict__ coef, int mode) {
   __shared__ double dots[ZAL_NDOT];
   int col, ncols;
   zal_cols(d_iter, Qn, col, ncols);
@@ -876,9 +885,15 @@ __global__ void k_zal_solve(int nb, const int* __restrict__ d_iter, int Qn, cons
   if (threadIdx.x != 0) return;
   const int n = ncols;
   double A[ZAL_QMAX * ZAL_QMAX], y[ZAL_QMAX];
+  constexpr int Q2 = ZAL_QMAX * ZAL_QMAX;
   for (int a = 0; a < n; ++a) {
-    y[a] = dots[2 * ZAL_QMAX * ZAL_QMAX + a];
-    for (int b = 0; b < n; ++b) A[a + n * b] = dots[a * ZAL_QMAX + b] - dots[ZAL_QMAX * ZAL_QMAX + a * ZAL_QMAX + b];
+    const double ur = dots[3 * Q2 + a], vr = dots[3 * Q2 + ZAL_QMAX + a];
+    y[a] = mode == 0 ? ur : ur - vr;
+    for (int b = 0; b < n; ++b) {
+      const double uu = dots[a * ZAL_QMAX + b], uv = dots[Q2 + a * ZAL_QMAX + b];
+      const double vu = dots[Q2 + b * ZAL_QMAX + a], vv = dots[2 * Q2 + a * ZAL_QMAX + b];
+      A[a + n * b] = mode == 0 ? uu - uv : vv - uv - vu + uu;
+    }
   }
   int rc = 0;
   // LU with partial pivoting (same elimination order as the CPU restatement)
@@ -1065,10 +1080,10 @@ hipError_t launch_axpy_step(int64_t n, const double* a, const double* d, double 
 }
 
 hipError_t launch_zal(int64_t N, int Qn, const int* d_iter, double* U, double* V, const double* x0, const double* x1,
-                      const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s) {
-  if (Qn < 1 || Qn > ZAL_QMAX) return hipErrorInvalidValue;
+                      const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s, int mode) {
+  if (Qn < 1 || Qn > ZAL_QMAX || mode < 0 || mode > 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_zal_dots, dim3(ZAL_NB), dim3(256), 0, s, N, d_iter, Qn, U, V, x0, x1, x2, dots_part);
-  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(64), 0, s, ZAL_NB, d_iter, Qn, dots_part, coef);
+  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(64), 0, s, ZAL_NB, d_iter, Qn, dots_part, coef, mode);
   hipLaunchKernelGGL(k_zal_apply, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, d_iter, Qn, V, x1, coef, x_out);
   return hipGetLastError();
 }

@@ -43,8 +43,12 @@ class AdmixData2:
 
 def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iters=5, T=np.float64,
                     T2=np.float64, Q=3, use_gpu=True, verbose=False, fix_q=False, fix_p=False,
-                    init_q=None, init_p=None, engine_opts=None, warm_start=False, guarantee_increase=False):
-    """src/driver.jl:133-201.  guarantee_increase is admixture_qn!'s keyword (algorithms_outer.jl:41)."""
+                    init_q=None, init_p=None, engine_opts=None, warm_start=False, guarantee_increase=False,
+                    mode="ZAL"):
+    """src/driver.jl:133-201.  guarantee_increase and mode (:ZAL / :LBQN) are admixture_qn!'s keywords
+    (algorithms_outer.jl:39-41)."""
+    if mode not in ("ZAL", "LBQN"):
+        raise ValueError("mode must be 'ZAL' or 'LBQN'")
     if not use_gpu:
         raise NotImplementedError("this library is the GPU engine; use_gpu=False has no implementation here")
     if T is not np.float64:
@@ -62,7 +66,7 @@ def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iter
         d.doublemissing = eng.double_missing()
         eng.set_params(d.q, d.p)
         base = dict(Qn=Q, fix_q=int(fix_q), fix_p=int(fix_p), verbose=int(verbose),
-                    guarantee_increase=int(guarantee_increase))
+                    guarantee_increase=int(guarantee_increase), qn_mode=int(mode == "LBQN"))
         if engine_opts:
             base.update(engine_opts)
         ll_start = math.nan

@@ -110,6 +110,8 @@ typedef struct {
   int em_fix_q;          /* EM warm-up updates P only (em_p!, algorithms_inner.jl:40): keeps a given Q */
   int guarantee_increase; /* step halving of every Q/P step until the log-likelihood does not drop
                              (update_q!/update_p!, algorithms_inner.jl:170-199,314-343); eager path */
+  int qn_mode;           /* admixture_qn!(mode=...) (algorithms_outer.jl:39,128-141): 0 = :ZAL (default),
+                            1 = :LBQN, limited-memory Broyden of the second kind on the same secants     */
 } hk_fit_opts;
 
 typedef struct {

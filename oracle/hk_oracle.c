@@ -706,8 +706,24 @@ static int lu_solve(double* A, int n, double* b) {
 
 /* U[:, col] = x1 - x0, V[:, col] = x2 - x1; x_out = x1 - V (U'U - U'V)^{-1} U'(x0 - x1) over the
  * first ncols columns. */
+/* Quasi-Newton point of admixture_qn! from x0 = x, x1 = F(x), x2 = F(F(x)) (algorithms_outer.jl:128-141).
+ * Secant columns (update_UV!'s cycling): U[:, col] = x1 - x0, V[:, col] = x2 - x1, col = (iter-1) mod Qn,
+ * using the first min(iter, Qn) columns.  With r = x0 - x1:
+ *   mode 0, ZAL (Zhou, Alexander & Lange 2011):  x_out = x1 - V c,  (U'U - U'V) c = U' r
+ *   mode 1, LBQN (limited-memory Broyden, Agarwal & Xu):  x_out = x1 - V c,  (Y'Y) c = -Y' r,  Y = V - U
+ * Both are multisecant Broyden updates from H0 = -I of the inverse Jacobian of G(x) = F(x) - x (ZAL of the
+ * first kind; LBQN of the second kind, H = -I + (U + Y)(Y'Y)^-1 Y', H Y = U) with x_out = x0 - H G(x0).
+ * OpenADMIXTURE's update_UV_LBQN!/update_QN_LBQN! are not available; mode 1 is the published method. */
+int hko_qn(int64_t N, int Qn, int iter, int mode, double* U, double* V, const double* x0, const double* x1,
+           const double* x2, double* x_out);
+
 int hko_zal(int64_t N, int Qn, int iter, double* U, double* V, const double* x0, const double* x1,
             const double* x2, double* x_out) {
+  return hko_qn(N, Qn, iter, 0, U, V, x0, x1, x2, x_out);
+}
+
+int hko_qn(int64_t N, int Qn, int iter, int mode, double* U, double* V, const double* x0, const double* x1,
+           const double* x2, double* x_out) {
   const int col = (iter - 1) % Qn;
   const int ncols = iter < Qn ? iter : Qn;
 #pragma omp parallel for schedule(static)
@@ -715,27 +731,32 @@ int hko_zal(int64_t N, int Qn, int iter, double* U, double* V, const double* x0,
     U[t + N * col] = x1[t] - x0[t];
     V[t + N * col] = x2[t] - x1[t];
   }
-  double UtU[64], UtV[64], Utr[8];
-  for (int a = 0; a < ncols; ++a) {
-    Utr[a] = 0.0;
-    for (int b = 0; b < ncols; ++b) { UtU[a + ncols * b] = 0.0; UtV[a + ncols * b] = 0.0; }
-  }
+  double UtU[64], UtV[64], VtV[64], Utr[8], Vtr[8];
   for (int a = 0; a < ncols; ++a) {
     const double* ua = U + N * a;
-    double s = 0.0;
-    for (int64_t t = 0; t < N; ++t) s += ua[t] * (x0[t] - x1[t]);
+    const double* va = V + N * a;
+    double s = 0.0, sv = 0.0;
+    for (int64_t t = 0; t < N; ++t) { s += ua[t] * (x0[t] - x1[t]); sv += va[t] * (x0[t] - x1[t]); }
     Utr[a] = s;
+    Vtr[a] = sv;
     for (int b = 0; b < ncols; ++b) {
       const double* ub = U + N * b;
       const double* vb = V + N * b;
-      double su = 0.0, sv = 0.0;
-      for (int64_t t = 0; t < N; ++t) { su += ua[t] * ub[t]; sv += ua[t] * vb[t]; }
+      double su = 0.0, suv = 0.0, svv = 0.0;
+      for (int64_t t = 0; t < N; ++t) { su += ua[t] * ub[t]; suv += ua[t] * vb[t]; svv += va[t] * vb[t]; }
       UtU[a + ncols * b] = su;
-      UtV[a + ncols * b] = sv;
+      UtV[a + ncols * b] = suv;
+      VtV[a + ncols * b] = svv;
     }
   }
   double A[64];
-  for (int t = 0; t < ncols * ncols; ++t) A[t] = UtU[t] - UtV[t];
+  for (int a = 0; a < ncols; ++a)
+    for (int b = 0; b < ncols; ++b) {
+      const int t = a + ncols * b;
+      A[t] = mode == 0 ? UtU[t] - UtV[t] : VtV[t] - UtV[t] - UtV[b + ncols * a] + UtU[t];
+    }
+  if (mode != 0)
+    for (int a = 0; a < ncols; ++a) Utr[a] -= Vtr[a];
   int rc = lu_solve(A, ncols, Utr);
 #pragma omp parallel for schedule(static)
   for (int64_t t = 0; t < N; ++t) {
@@ -762,6 +783,7 @@ typedef struct {
   double rho;        /* 1e7 */
   double eps;        /* pseudocount 1e-5 */
   int guarantee_increase; /* admixture_qn!(...; guarantee_increase), algorithms_outer.jl:41 */
+  int qn_mode;            /* admixture_qn!(...; mode): 0 = :ZAL, 1 = :LBQN (algorithms_outer.jl:128-141) */
 } hko_fit_opts;
 
 /* Step-halving loop of update_q! / update_p! with guarantee_increase (algorithms_inner.jl:170-199 and
@@ -862,7 +884,7 @@ int hko_fit(const uint8_t* bed, int64_t bps, int64_t I, int64_t J, int K, const 
       if (gi) halve_step(bed, bps, I, J, K, 0, p_next, dp, p_next2, q_next2, p_next2, &ll_tmp, o->eps);
     } else memcpy(p_next2, p, sizeof(double) * K * 4 * J);
     const double ll_basic = hko_loglik(bed, bps, I, J, K, q_next2, p_next2);
-    hko_zal(N, o->Qn, i, U, V, x, x_next, x_next2, x_tmp);
+    hko_qn(N, o->Qn, i, o->qn_mode, U, V, x, x_next, x_next2, x_tmp);
     hko_project_p(p_tmp, K, J, o->eps);
     hko_project_q(q_tmp, K, I, o->eps);
     const double ll_qn = hko_loglik(bed, bps, I, J, K, q_tmp, p_tmp);

@@ -46,6 +46,7 @@ class FitOpts(ctypes.Structure):
         ("rho", _dbl),
         ("eps", _dbl),
         ("guarantee_increase", _int),
+        ("qn_mode", _int),
     ]
 
 
@@ -84,6 +85,8 @@ def lib():
         L.hko_em_normalize.argtypes = [_int, _i64, _i64, _i64p, _f64p_f, _f64p_f, _int, _int, _dbl]
         L.hko_zal.argtypes = [_i64, _int, _int, _f64p_f, _f64p_f, _f64p, _f64p, _f64p, _f64p]
         L.hko_zal.restype = _int
+        L.hko_qn.argtypes = [_i64, _int, _int, _int, _f64p_f, _f64p_f, _f64p, _f64p, _f64p, _f64p]
+        L.hko_qn.restype = _int
         L.hko_fit.argtypes = [_u8p, _i64, _i64, _i64, _int, ctypes.POINTER(FitOpts), _f64p_f,
                               ctypes.POINTER(_dbl), _f64p]
         L.hko_fit.restype = _int
@@ -235,15 +238,24 @@ def zal(U, V, x0, x1, x2, it, Qn=3):
     return out, rc
 
 
+def qn_step(U, V, x0, x1, x2, it, Qn=3, mode=0):
+    """Quasi-Newton point of admixture_qn! (mode 0: ZAL, 1: LBQN); U, V are updated in place."""
+    N = x0.size
+    out = np.zeros(N)
+    rc = lib().hko_qn(N, Qn, it, mode, U, V, np.ascontiguousarray(x0), np.ascontiguousarray(x1),
+                      np.ascontiguousarray(x2), out)
+    return out, rc
+
+
 def fit(bed, I, J, q0, p0, em_iters=5, n_iter=1000, rtol=1e-7, Qn=3, fix_q=False, fix_p=False,
-        q_penalty=False, p_penalty=True, rho=1e7, eps=1e-5, guarantee_increase=False):
+        q_penalty=False, p_penalty=True, rho=1e7, eps=1e-5, guarantee_increase=False, qn_mode=0):
     """init_em! + admixture_qn! (algorithms_outer.jl:1-172), with the step halving of update_q! /
     update_p! when guarantee_increase (algorithms_inner.jl:170-199,314-343).
     Returns (q, p, ll, iters, trace)."""
     K = q0.shape[0]
     x = np.asfortranarray(np.concatenate([q0, p0], axis=1))
     o = FitOpts(em_iters, n_iter, rtol, Qn, int(fix_q), int(fix_p), int(q_penalty), int(p_penalty), rho, eps,
-                int(guarantee_increase))
+                int(guarantee_increase), int(qn_mode))
     ll = _dbl(0.0)
     trace = np.zeros(max(n_iter, 1))
     iters = lib().hko_fit(bed, _bps(I), I, J, K, ctypes.byref(o), x, ctypes.byref(ll), trace)

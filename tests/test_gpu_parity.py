@@ -160,8 +160,8 @@ def test_fit_first_iterations_track_oracle(oracle):
 
 
 @pytest.mark.parametrize("opts", [dict(fix_q=True), dict(fix_p=True), dict(q_penalty=True),
-                                  dict(p_penalty=False), dict(guarantee_increase=True)],
-                         ids=["fix_q", "fix_p", "q_penalty", "p_equality", "guarantee_increase"])
+                                  dict(p_penalty=False), dict(guarantee_increase=True), dict(qn_mode=1)],
+                         ids=["fix_q", "fix_p", "q_penalty", "p_equality", "guarantee_increase", "lbqn"])
 def test_fit_modes_track_oracle(opts, oracle):
     """admixture_qn! modes (fix_q / fix_p: algorithms_outer.jl:71-116; penalty variants: :58-64 and
     algorithms_inner.jl:158-168,295-343; guarantee_increase: algorithms_inner.jl:170-199,314-343),

@@ -246,6 +246,48 @@ def test_zal_closed_form(oracle):
         assert rel(out, want) <= 1e-10
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["ZAL", "LBQN"])
+def test_quasi_newton_exact_on_affine_maps(oracle, mode):
+    """Both quasi-Newton modes are multisecant Broyden updates from H0 = -I of the inverse Jacobian of
+    G(x) = F(x) - x.  For an affine contraction F(x) = A x + b the secants satisfy V = A U exactly, so once
+    the U columns span the space the update lands on the fixed point (I - A)^-1 b -- for ZAL through
+    (U'(I - A)U)^-1, for LBQN through ((A - I)U)'((A - I)U) -- whatever the iteration point.  A dropped
+    term, a sign or a transposed product breaks this."""
+    rng = np.random.default_rng(5)
+    n = 3
+    A = rng.normal(size=(n, n))
+    A *= 0.6 / np.max(np.abs(np.linalg.eigvals(A)))
+    b = rng.normal(size=n)
+    xstar = np.linalg.solve(np.eye(n) - A, b)
+    U = np.zeros((n, 3), order="F")
+    V = np.zeros((n, 3), order="F")
+    for it in (1, 2, 3):
+        x0 = rng.normal(size=n)
+        x1 = A @ x0 + b
+        x2 = A @ x1 + b
+        out, rc = oracle.qn_step(U, V, x0, x1, x2, it, Qn=3, mode=mode)
+        assert rc == 0
+    assert np.max(np.abs(out - xstar)) <= 1e-10 * max(1.0, np.max(np.abs(xstar)))
+    # with fewer secants than dimensions the step is not exact (the test above is not vacuous)
+    U1 = np.zeros((n, 3), order="F")
+    V1 = np.zeros((n, 3), order="F")
+    x0 = rng.normal(size=n)
+    out1, _ = oracle.qn_step(U1, V1, x0, A @ x0 + b, A @ (A @ x0 + b) + b, 1, Qn=3, mode=mode)
+    assert np.max(np.abs(out1 - xstar)) > 1e-6
+
+
+def test_lbqn_fit_reaches_the_zal_optimum(oracle):
+    """admixture_qn!(mode=:LBQN) on the CPU restatement: the log-likelihood never decreases across
+    iterations (the accept rule keeps the better of the plain and the extrapolated point) and the fit
+    converges to the same optimum as the default ZAL mode."""
+    I, J, K = 120, 90, 3
+    s, q, p = case(I, J, K, fst=0.1, miss=0.02)
+    _, _, ll_zal, _, _ = oracle.fit(s.bed, I, J, q, p, rtol=1e-10, n_iter=500)
+    _, _, ll_lb, it_lb, tr = oracle.fit(s.bed, I, J, q, p, rtol=1e-10, n_iter=500, qn_mode=1)
+    assert np.all(np.diff(tr) >= -1e-9 * abs(tr[-1]))
+    assert abs(ll_lb - ll_zal) <= 1e-7 * abs(ll_zal), (ll_lb, ll_zal, it_lb)
+
+
 def test_oracle_fit_monotone_and_converges(oracle):
     """init_em! + admixture_qn! on a small simulated set: accepted ll never decreases, converges."""
     I, J, K = 60, 80, 3
