@@ -84,4 +84,10 @@ hipError_t launch_copy16(const void* src, void* dst, int64_t n16, hipStream_t s)
 int project_features(const PassGeom& g, int n_blocks, uint64_t seed, float* F_host, hipStream_t s, char* err,
                      size_t errlen);
 
+// SKFR marker pre-selection (hk_skfr.hip) on the genotype streams of g; returns 0, -1 (bad argument), -2 (HIP
+// error) or -3 (out of device memory) with err filled.
+int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, int tries, uint64_t seed,
+             const int32_t* init, int32_t* clusters, int32_t* aims, int* n_aims, double* wss, int* iters,
+             hipStream_t s, char* err, size_t errlen);
+
 }  // namespace hk

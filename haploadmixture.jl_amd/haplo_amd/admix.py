@@ -4,7 +4,8 @@ Mirrors src/driver.jl:38-201 (keyword names, defaults and control flow) with the
 the MI355X.  Differences, each forced by the build:
   * ``rng`` is an integer seed of the build's counter RNG (the reference takes a Julia AbstractRNG);
   * ``use_gpu=False`` is rejected: the CPU path of the reference is not part of this library;
-  * ``sparsity`` (SKFR AIM pre-selection, driver.jl:84-131) is not provided;
+  * SKFR labels and markers are 0-based (``skfr`` module), and a vector of sparsities runs each
+    sparsity as its own seeded search (the reference's sparsekmeans_path warm-starts them);
   * ``T2=np.float32`` selects the MIXED engine (FP32 per-pair products, FP64 accumulation), the
     analogue of the reference's ``T2=Float32`` GPU arrays (driver.jl:149, cuda/structs.jl:19-30).
 Addition: ``warm_start=True`` fills the ``init_q`` hook (driver.jl:56,184-185) with the learned
@@ -17,7 +18,7 @@ import math
 
 import numpy as np
 
-from . import plink, warmstart
+from . import plink, skfr, warmstart
 from .engine import HipAdmixData
 from .sim import init_params
 
@@ -106,6 +107,38 @@ def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iter
     return d
 
 
+def _filter_SKFR(filename, K, sparsity, *, rng=1, prefix=None, tries=10, max_inner_iter=50, ftn=None):
+    """src/driver.jl:84-131: SKFR on every individual and SNP of the fileset, then des.bed/.bim/.fam with
+    the selected SNPs, des = f"{prefix}_{K}_{2 * sparsity}aims".  Returns (des + ".bed", clusters, aims);
+    for a decreasing list of sparsities, lists of each (des = f"{prefix}_{K}_{s}aims", as the reference
+    names them)."""
+    if not str(filename).endswith(".bed"):
+        raise ValueError("filename should end with .bed")
+    prefix = str(filename)[:-4] if prefix is None else prefix
+    ftn = skfr.sparsekmeans1 if ftn is None else ftn
+    many = not np.isscalar(sparsity)
+    sparsities = [int(s) for s in (sparsity if many else [sparsity])]
+    if many and sparsities != sorted(sparsities, reverse=True):
+        raise ValueError("sparsities should be decreasing")
+    fs = plink.PlinkFileset(str(filename))
+    outs, clusters, aims = [], [], []
+    with skfr.ImputedSnpMatrix(fs, K, rng=rng, blocksize=2) as ism:
+        for s in sparsities:
+            if tries == 1:
+                cl, _, am, _, _ = ftn(ism, s, max_iter=max_inner_iter, squares=False)
+            else:
+                cl, _, am, _, _, _ = skfr.sparsekmeans_repeat(ism, s, iter=tries, max_inner_iter=max_inner_iter,
+                                                              ftn=ftn)
+            des = f"{prefix}_{K}_{s if many else 2 * s}aims"
+            print(des)
+            outs.append(plink.filter(str(filename), np.ones(fs.n_individuals, dtype=bool), np.sort(am), des))
+            clusters.append(cl)
+            aims.append(am)
+    if many:
+        return outs, clusters, aims
+    return outs[0], clusters[0], aims[0]
+
+
 def run_admixture(filename, I, J, K, *, rng=1, sparsity=None, prefix=None, skfr_tries=1, skfr_max_inner_iter=50,
                   skfr_mode="global", admix_n_iter=1000, admix_rtol=1e-7, admix_n_em_iters=5, T=np.float64,
                   T2=np.float64, Q=3, use_gpu=True, verbose=False, progress_bar=False, fix_q=False, fix_p=False,
@@ -115,9 +148,18 @@ def run_admixture(filename, I, J, K, *, rng=1, sparsity=None, prefix=None, skfr_
         raise ValueError("filename should end with .bed")
     print(f"Using {filename} as input.")
     if sparsity is not None:
-        raise NotImplementedError("SKFR AIM pre-selection (sparsity=...) is not part of this engine")
-    d = _admixture_base(filename, I, J, K, n_iter=admix_n_iter, rtol=admix_rtol, rng=rng,
+        if skfr_mode == "global":
+            ftn = skfr.sparsekmeans1
+        elif skfr_mode == "local":
+            ftn = skfr.sparsekmeans2
+        else:
+            raise ValueError("skfr_mode can only be 'global' or 'local'")
+        admix_input, clusters, aims = _filter_SKFR(filename, K, sparsity, rng=rng, prefix=prefix, tries=skfr_tries,
+                                                   max_inner_iter=skfr_max_inner_iter, ftn=ftn)
+    else:
+        admix_input, clusters, aims = filename, None, None
+    d = _admixture_base(admix_input, I, J, K, n_iter=admix_n_iter, rtol=admix_rtol, rng=rng,
                         em_iters=admix_n_em_iters, T=T, T2=T2, Q=Q, use_gpu=use_gpu, verbose=verbose,
                         fix_q=fix_q, fix_p=fix_p, init_q=init_q, init_p=init_p, warm_start=warm_start,
                         guarantee_increase=guarantee_increase)
-    return d, None, None
+    return d, clusters, aims

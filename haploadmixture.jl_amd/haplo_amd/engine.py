@@ -203,6 +203,29 @@ class HipAdmixData:
     def sync(self):
         self._check(self._L.hk_sync(self._h))
 
+    # -- SKFR marker pre-selection ------------------------------------------------------------------
+    def skfr(self, k: int, sparsity: int, mode: str = "global", max_iter: int = 50, tries: int = 1, seed: int = 1,
+             init=None) -> dict:
+        """hk_skfr on this context's J blocks: sparse k-means with feature ranking (mode "global" =
+        sparsekmeans1, "local" = sparsekmeans2).  Returns clusters (I,), aims (0-based SNP indices,
+        ascending), wss and iters of the best of `tries` runs."""
+        if mode not in ("global", "local"):
+            raise ValueError("mode must be 'global' or 'local'")
+        clusters = np.zeros(self.I, dtype=np.int32)
+        aims = np.zeros(2 * max(1, min(self.J, int(k) * int(sparsity))), dtype=np.int32)
+        n = ctypes.c_int()
+        wss = ctypes.c_double()
+        iters = ctypes.c_int()
+        ini = None
+        if init is not None:
+            ini = np.ascontiguousarray(init, dtype=np.int32)
+            if ini.shape != (int(k),):
+                raise ValueError(f"init must hold k={k} individual indices")
+        self._check(self._L.hk_skfr(self._h, int(k), int(sparsity), int(mode == "local"), int(max_iter), int(tries),
+                                    int(seed) & ((1 << 64) - 1), ptr(ini), ptr(clusters), ptr(aims), ctypes.byref(n),
+                                    ctypes.byref(wss), ctypes.byref(iters)))
+        return dict(clusters=clusters, aims=aims[: n.value].copy(), wss=wss.value, iters=iters.value)
+
 
 def fit_batch(engines, opts_list, ll_start=None):
     """hk_fit_batch: fits of several contexts in lockstep; returns (results, seconds_at_finish)."""

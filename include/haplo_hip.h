@@ -186,6 +186,19 @@ int hk_mlp_grad(hk_mlp* m, int64_t row0, int64_t n, int match, float* loss, int*
 /* One Adam step (bias-corrected, step >= 1) on the stored gradients; refreshes the BF16 weights. */
 int hk_mlp_adam(hk_mlp* m, float lr, float beta1, float beta2, float eps, int step);
 
+/* ---- SKFR marker pre-selection: _filter_SKFR (src/driver.jl:84-131) over SparseKmeansFeatureRanking's
+ * sparsekmeans1 (mode 0, skfr_mode = :global) / sparsekmeans2 (mode 1, :local) and sparsekmeans_repeat
+ * (tries > 1) on the context's J SNP blocks (blocksize = 2: SNPs 2b and 2b+1 form block b).
+ * Sparse k-means with feature ranking on standardised allele-2 dosages: k clusters (1..HK_MAX_K, <= I),
+ * `sparsity` blocks kept (1..J; :local keeps that many per cluster, markers = their union), at most
+ * max_iter (skfr_max_inner_iter) reassignments per try; the try with the smallest within-cluster sum of
+ * squares over the kept SNPs wins.  seed drives the k-means++ starts; init (k individual indices, or NULL)
+ * replaces them in the first try.  Outputs: clusters (I labels 0..k-1), aims (0-based SNP indices,
+ * ascending; room for 2 * min(J, k * sparsity)), n_aims, and (each may be NULL) wss and iters of the
+ * returned try.  Runs synchronously on the context stream; fit parameters are untouched. */
+int hk_skfr(hk_ctx* ctx, int k, int sparsity, int mode, int max_iter, int tries, uint64_t seed,
+            const int32_t* init, int32_t* clusters, int32_t* aims, int* n_aims, double* wss, int* iters);
+
 /* Calibration: copy `bytes` device-to-device `reps` times with a plain streaming kernel (k_copy16, 16 B per
  * lane) on the context stream; gbps = (read + write bytes) / time.  Used to scale HBM byte counters. */
 int hk_copy_bench(hk_ctx* ctx, int64_t bytes, int reps, double* gbps);

@@ -93,8 +93,10 @@ def test_run_admixture_argument_errors(tmp_path):
     plink.write_bed(f, s.bed, 16, 8)
     with pytest.raises(ValueError):
         haplo_amd.run_admixture(f[:-4] + ".txt", 16, 4, 2)
-    with pytest.raises(NotImplementedError):
-        haplo_amd.run_admixture(f, 16, 4, 2, sparsity=10)
+    with pytest.raises(ValueError):
+        haplo_amd.run_admixture(f, 16, 4, 2, sparsity=2, skfr_mode="other")
+    with pytest.raises(ValueError):
+        haplo_amd._filter_SKFR(f, 2, [2, 3])
     with pytest.raises(NotImplementedError):
         haplo_amd.run_admixture(f, 16, 4, 2, use_gpu=False)
 
