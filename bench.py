@@ -40,6 +40,10 @@ WORKLOADS = {
     "select_I3k_J20k_K2-8x4": dict(I=3000, J=20000, K=5, fst=0.05, miss=0.01, seed=4, prec="mixed"),
     # learned warm start: held-out datasets of the training population structure (world seed 2024)
     "warmstart_I10k_J100k_K5": dict(I=10000, J=100000, K=5, fst=0.05, miss=0.01, seed=4242, prec="mixed"),
+    # SKFR marker pre-selection (run_admixture(sparsity=...)) on the default fit dataset: k = 6, 10,000 of
+    # 200,000 blocks kept (:global), at most 50 reassignments, one try
+    "skfr_I20k_J200k_K6_s10k": dict(I=20000, J=200000, K=6, fst=0.05, miss=0.02, seed=3, prec="mixed",
+                                    sparsity=10000, sample_I=2000, sample_J=20000),
 }
 PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}  # MI355X FP32 / FP64 vector peaks (dense)
 
@@ -194,6 +198,78 @@ def warmstart_bench(a, w, prec):
     print(json.dumps(out), flush=True)
 
 
+def skfr_bench(a, w):
+    """SKFR on the full dataset (wall time per call, genotypes resident in HBM), the device time of its
+    kernel classes (HIP events on the engine stream), and the CPU restatement against the GPU on a bounded
+    sample (first sample_I individuals x first sample_J blocks, sparsity scaled with J)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import skfr_oracle
+
+    from haplo_amd import plink
+
+    I, J, K, s = w["I"], w["J"], w["K"], w["sparsity"]
+    log(f"generating {a.workload}: I={I} J={J} Fst={w['fst']} miss={w['miss']}")
+    data = simulate(I, J, K, fst=w["fst"], miss=w["miss"], seed=w["seed"], truth=False)
+    eng = HipAdmixData(data.bed, I, J, K, precision="mixed")
+    log("warm-up (kernel loading)")
+    eng.skfr(K, s, max_iter=2, seed=1)
+    eng.sync()
+    runs = []
+    for st in range(a.steps):
+        t0 = time.perf_counter()
+        r = eng.skfr(K, s, max_iter=50, seed=1, profile=True)
+        runs.append((time.perf_counter() - t0, r))
+        log(f"step {st}: {runs[-1][0]:.3f}s, {r['iters']} iterations, wss {r['wss']:.6g}")
+    wall = float(np.median([t for t, _ in runs]))
+    r = runs[-1][1]
+    pr = r["profile"]
+    eng.close()
+    # the center update reads the block-major stream once: 0.5 B per (individual, block) pair, plus its
+    # center (k x 2J) and score (J) writes
+    cen_ms = pr["centers_ms"] / max(pr["centers_launches"], 1)
+    cen_bytes = 0.5 * I * J + 8.0 * (2 * K * J + J)
+    dist_ms = pr["dist_ms"] / max(pr["dist_launches"], 1)
+    roof = {"kernel": "center update (k_sk_member + k_sk_counts + k_sk_centers)", "bound": "hbm", "achieved": round(cen_bytes / (cen_ms * 1e-3) / 1e9, 1),
+            "peak": 8000.0, "unit": "GB/s", "frac": None, "traffic": None,
+            "avg_launch_ms": round(cen_ms, 4), "algorithmic_bytes_per_launch": cen_bytes}
+    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+    # bounded CPU sample: the NumPy restatement and the GPU on the same sub-problem
+    Is, Js = w["sample_I"], w["sample_J"]
+    ss = max(1, s * Js // J)
+    bps = (I + 3) // 4
+    codes = plink.unpack_codes(data.bed, I, np.arange(2 * Js))[:, :Is]
+    D = skfr_oracle.dosages(codes)
+    log(f"CPU sample: I={Is} J={Js} sparsity={ss}")
+    t0 = time.perf_counter()
+    ref = skfr_oracle.skfr(D, K, ss, max_iter=50, seed=1)
+    t_cpu = time.perf_counter() - t0
+    with HipAdmixData(np.ascontiguousarray(data.bed[: 2 * Js * bps]), Is, Js, K, bytes_per_snp=bps,
+                      n_snp=2 * Js) as es:
+        es.skfr(K, ss, max_iter=2, seed=1)
+        t0 = time.perf_counter()
+        gs = es.skfr(K, ss, max_iter=50, seed=1)
+        t_gpu_s = time.perf_counter() - t0
+    same = bool(np.array_equal(gs["clusters"], ref["clusters"]) and np.array_equal(gs["aims"], ref["aims"]))
+    log(f"sample: CPU {t_cpu:.2f}s, GPU {t_gpu_s:.4f}s, identical labels/markers: {same}")
+    out = {
+        "metric": f"SKFR marker selection time (s), I=20k J=200k k={K} sparsity={s}",
+        "value": round(wall, 4), "unit": "s", "n_gpus": 1, "steps": a.steps, "warmup": 1,
+        "ms_per_step": round(1000 * wall, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded generator, K=6 populations)",
+        "config": {"workload": a.workload, "I": I, "J": J, "k": K, "sparsity": s, "mode": "global", "tries": 1,
+                   "max_iter": 50},
+        "iters": r["iters"], "wss": r["wss"],
+        "device_ms": {k2: (round(v, 3) if isinstance(v, float) else v) for k2, v in pr.items()},
+        "dist_avg_launch_ms": round(dist_ms, 4),
+        "roofline": roof,
+        "cpu_baseline": {"value": round(t_cpu, 3), "unit": "s", "cores": 1, "kind": "port",
+                         "sample": f"first {Is} individuals x first {Js} blocks, sparsity {ss}, k={K}: the NumPy "
+                                   f"restatement (single-threaded Python loop over SNP columns)",
+                         "gpu_same_sample_s": round(t_gpu_s, 4), "identical_result": same},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +297,8 @@ def main():
         return select_bench(a, w, prec)
     if a.workload.startswith("warmstart"):
         return warmstart_bench(a, w, prec)
+    if a.workload.startswith("skfr"):
+        return skfr_bench(a, w)
     I, J, K = w["I"], w["J"], w["K"]
     log(f"generating {a.workload}: I={I} J={J} K={K} Fst={w['fst']} miss={w['miss']}")
     t0 = time.perf_counter()

@@ -1068,10 +1068,11 @@ extern "C" int hk_features(hk_ctx* c, int n_blocks, uint64_t seed, float* F) {
 }
 
 extern "C" int hk_skfr(hk_ctx* c, int k, int sparsity, int mode, int max_iter, int tries, uint64_t seed,
-                       const int32_t* init, int32_t* clusters, int32_t* aims, int* n_aims, double* wss, int* iters) {
+                       const int32_t* init, int32_t* clusters, int32_t* aims, int* n_aims, double* wss, int* iters,
+                       double* prof) {
   if (!c || !clusters || !aims || !n_aims) return fail(c, HK_EINVAL, "null pointer");
   HIPC(c, hipSetDevice(c->dev));
   const int rc = skfr_run(c->g, k, sparsity, mode, max_iter, tries, seed, init, clusters, aims, n_aims, wss, iters,
-                          c->s, c->err, sizeof c->err);
+                          prof, c->s, c->err, sizeof c->err);
   return rc;
 }

@@ -88,6 +88,6 @@ int project_features(const PassGeom& g, int n_blocks, uint64_t seed, float* F_ho
 // error) or -3 (out of device memory) with err filled.
 int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, int tries, uint64_t seed,
              const int32_t* init, int32_t* clusters, int32_t* aims, int* n_aims, double* wss, int* iters,
-             hipStream_t s, char* err, size_t errlen);
+             double* prof, hipStream_t s, char* err, size_t errlen);
 
 }  // namespace hk

@@ -18,8 +18,12 @@
 //          reassign i to argmin_c sum over the kept SNPs of theta_cj^2 - 2 z_ij theta_cj (smaller c on a
 //          tie); stop when no label changes.
 //   tries  the try with the smallest within-cluster sum of squares over the kept SNPs wins.
+//   output labels, and the kept blocks' SNP pairs in decreasing order of importance (docs/src/index.md:287):
+//          the score sum_c n_c (theta_c,2b^2 + theta_c,2b+1^2) of the last center update, ties to the
+//          smaller block.
 // All arithmetic is FP64 with contraction off and every sum runs in a fixed order (SNPs ascending within
 // a chunk of SK_CHUNK blocks, chunks ascending), so a CPU restatement reproduces labels and markers exactly.
+// The three standardised values of each SNP, (d - m_j) / s_j for d = 0, 1, 2, are tabulated once.
 //
 // Layout: the fit's genotype streams, gq [J8][Ip] (8 blocks of one individual per word) for the
 // individual-parallel kernels and gp [I8][Jp] (8 individuals of one block per word) for the
@@ -32,6 +36,7 @@
 #include <math.h>
 #include <stdio.h>
 
+#include <algorithm>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -42,37 +47,40 @@ namespace {
 
 constexpr int SK_MAXC = HK_MAXK;  // largest number of clusters
 constexpr int SK_CHUNK = 1024;    // blocks per partial sum of the all-SNP distance kernel
+constexpr int SK_CW = 128;        // genotype words (1024 individuals) per partial of the center counts
+constexpr int SK_ACH_MIN = 256;   // kept blocks per partial of the reassignment (at least; <= 64 partials)
 
-// Allele-2 dosage of a PLINK code (0b00 -> 0, 0b10 -> 1, 0b11 -> 2); -1 for missing (0b01).
-__device__ __forceinline__ int dosage(unsigned code) { return code == 0u ? 0 : (code == 2u ? 1 : (code == 3u ? 2 : -1)); }
-
-// Standardised dosage (0 for a constant SNP); `missing` for a missing entry.
-__device__ __forceinline__ double zval(int d, double m, double s, double missing) {
-  return d < 0 ? missing : (s > 0.0 ? ((double)d - m) / s : 0.0);
+// Standardised value of a PLINK code at SNP column jj from the table ztab [2J][4]; `missing` for code 0b01.
+__device__ __forceinline__ double zval(const double* __restrict__ ztab, int64_t jj, unsigned code, double missing) {
+  const int d = code == 0u ? 0 : (code == 2u ? 1 : 2);
+  return code == 1u ? missing : ztab[jj * 4 + d];
 }
 
 // Per-SNP mean and standard deviation over the observed entries (lane = block).  Integer sums, so the
 // order does not matter; mean = s / n, var = ss / n - mean^2.
 __global__ void k_sk_stats(const uint32_t* __restrict__ gp, int64_t Jp, int64_t J, int64_t I8,
-                           double* __restrict__ mean, double* __restrict__ sd) {
+                           double* __restrict__ mean, double* __restrict__ sd, double* __restrict__ ztab) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= J) return;
   int64_t n[2] = {0, 0}, s[2] = {0, 0}, ss[2] = {0, 0};
-  for (int64_t iw = 0; iw < I8; ++iw) {
-    const uint32_t w = gp[iw * Jp + j];
+  // integer counts from popcounts (padding individuals decode as missing): with lo / hi the code bits of
+  // an allele, dosage = hi + (hi & lo), dosage^2 = hi + 3 (hi & lo), missing = lo & ~hi
+  constexpr uint32_t LOW = 0x11111111u;
+  int64_t iw = 0;
+  for (; iw < I8; iw += 4) {
+    uint32_t w[4];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const unsigned nib = (w >> (4 * t)) & 15u;
+    for (int u = 0; u < 4; ++u) w[u] = iw + u < I8 ? gp[(iw + u) * Jp + j] : 0x55555555u;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
-        const int d = dosage((nib >> (2 * a)) & 3u);  // padding individuals decode as missing
-        if (d >= 0) {
-          n[a] += 1;
-          s[a] += d;
-          ss[a] += d * d;
-        }
+        const uint32_t lo = (w[u] >> (2 * a)) & LOW, hi = (w[u] >> (2 * a + 1)) & LOW;
+        const int nh = __builtin_popcount(hi), nhl = __builtin_popcount(hi & lo);
+        n[a] += 8 - __builtin_popcount(lo & ~hi);
+        s[a] += nh + nhl;
+        ss[a] += nh + 3 * nhl;
       }
-    }
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a) {
@@ -81,27 +89,28 @@ __global__ void k_sk_stats(const uint32_t* __restrict__ gp, int64_t Jp, int64_t 
       m = (double)s[a] / (double)n[a];
       v = (double)ss[a] / (double)n[a] - m * m;
     }
+    const double sdv = v > 0.0 ? sqrt(v) : 0.0;
     mean[2 * j + a] = m;
-    sd[2 * j + a] = v > 0.0 ? sqrt(v) : 0.0;
+    sd[2 * j + a] = sdv;
+    for (int d = 0; d < 3; ++d) ztab[(2 * j + a) * 4 + d] = sdv > 0.0 ? ((double)d - m) / sdv : 0.0;
+    ztab[(2 * j + a) * 4 + 3] = 0.0;
   }
 }
 
 // Row of individual r as a center, missing -> 0: row[2j + a].
 __global__ void k_sk_row(const uint32_t* __restrict__ gq, int64_t Ip, int64_t J, int64_t r,
-                         const double* __restrict__ mean, const double* __restrict__ sd, double* __restrict__ row) {
+                         const double* __restrict__ ztab, double* __restrict__ row) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= J) return;
   const unsigned nib = (gq[(j >> 3) * Ip + r] >> (4 * (j & 7))) & 15u;
-  for (int a = 0; a < 2; ++a)
-    row[2 * j + a] = zval(dosage((nib >> (2 * a)) & 3u), mean[2 * j + a], sd[2 * j + a], 0.0);
+  for (int a = 0; a < 2; ++a) row[2 * j + a] = zval(ztab, 2 * j + a, (nib >> (2 * a)) & 3u, 0.0);
 }
 
 // Squared distances of every individual to nc center rows over the blocks of chunk blockIdx.y (missing z ->
 // 0): part[(ch * nc + c) * I + i], SNPs ascending.  Grid: (ceil(I / 256), ceil(J / SK_CHUNK)).
 __global__ void __launch_bounds__(256) k_sk_dist(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J,
-                                                 int nc, const double* __restrict__ mean,
-                                                 const double* __restrict__ sd, const double* __restrict__ rows,
-                                                 double* __restrict__ part) {
+                                                 int nc, const double* __restrict__ ztab,
+                                                 const double* __restrict__ rows, double* __restrict__ part) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= I) return;
   const int64_t ch = blockIdx.y;
@@ -112,13 +121,28 @@ __global__ void __launch_bounds__(256) k_sk_dist(const uint32_t* __restrict__ gq
   for (int c = 0; c < SK_MAXC; ++c) acc[c] = 0.0;
   for (int64_t jw = j0 >> 3; jw * 8 < j1; ++jw) {
     const uint32_t w = gq[jw * Ip + i];
-    const int tn = j1 - jw * 8 < 8 ? (int)(j1 - jw * 8) : 8;
+    if (jw * 8 + 8 <= j1) {
+      // whole word: the 16 table values are fetched together, then added in SNP order
+      double z[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) z[u] = zval(ztab, 16 * jw + u, (w >> (2 * u)) & 3u, 0.0);
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+#pragma unroll
+        for (int c = 0; c < SK_MAXC; ++c)
+          if (c < nc) {
+            const double e = z[u] - rows[(int64_t)c * 2 * J + 16 * jw + u];
+            acc[c] = acc[c] + e * e;
+          }
+      continue;
+    }
+    const int tn = (int)(j1 - jw * 8);
     for (int t = 0; t < tn; ++t) {
       const int64_t j = jw * 8 + t;
       const unsigned nib = (w >> (4 * t)) & 15u;
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
-        const double z = zval(dosage((nib >> (2 * a)) & 3u), mean[2 * j + a], sd[2 * j + a], 0.0);
+        const double z = zval(ztab, 2 * j + a, (nib >> (2 * a)) & 3u, 0.0);
 #pragma unroll
         for (int c = 0; c < SK_MAXC; ++c)
           if (c < nc) {
@@ -161,58 +185,128 @@ __global__ void k_sk_dist_finish(const double* __restrict__ part, int nch, int n
   labels[i] = best;
 }
 
-// Cluster centers over the observed entries (lane = block) and block scores.  theta [k][2J];
-// score [J] (:global) or [k][J] (:local).  Cluster sizes come from the host.
-__global__ void __launch_bounds__(256) k_sk_centers(const uint32_t* __restrict__ gp, int64_t Jp, int64_t J, int64_t I,
-                                                    int k, const int* __restrict__ labels,
-                                                    const int64_t* __restrict__ csize, const double* __restrict__ mean,
-                                                    const double* __restrict__ sd, double* __restrict__ theta,
-                                                    double* __restrict__ score, int local) {
+// Cluster membership of the 8 individuals of word iw as nibble masks: memb[iw * 8 + c] has 0xF at the
+// nibbles of the members of cluster c (padding individuals belong to none).
+__global__ void k_sk_member(const int* __restrict__ labels, int64_t I, int64_t nw, uint32_t* __restrict__ memb) {
+  const int64_t iw = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (iw >= nw) return;
+  uint32_t M[SK_MAXC];
+#pragma unroll
+  for (int c = 0; c < SK_MAXC; ++c) M[c] = 0u;
+  for (int t = 0; t < 8; ++t) {
+    const int64_t i = iw * 8 + t;
+    if (i >= I) break;
+    const int lab = labels[i];
+#pragma unroll
+    for (int c = 0; c < SK_MAXC; ++c)
+      if (c == lab) M[c] |= 0xFu << (4 * t);
+  }
+#pragma unroll
+  for (int c = 0; c < SK_MAXC; ++c) memb[iw * SK_MAXC + c] = M[c];
+}
+
+// Per-cluster allele-2 dosage sums and missing counts of block j over the words of individual chunk
+// blockIdx.y (SK_CW words), from popcounts: with lo / hi the low / high code bits of an allele at each
+// individual's nibble, dosage = hi + (hi & lo) and missing = lo & ~hi.  Integer partials:
+// ps / pm [chunk][k][2][J].
+__global__ void __launch_bounds__(256) k_sk_counts(const uint32_t* __restrict__ gp, int64_t Jp, int64_t J, int64_t nw,
+                                                   int k, const uint32_t* __restrict__ memb, int* __restrict__ ps,
+                                                   int* __restrict__ pm) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= J) return;
-  int n[SK_MAXC][2], s[SK_MAXC][2];
+  const int64_t ch = blockIdx.y;
+  const int64_t w0 = ch * SK_CW;
+  const int64_t w1 = w0 + SK_CW < nw ? w0 + SK_CW : nw;
+  unsigned sa[SK_MAXC][2], ma[SK_MAXC][2];
 #pragma unroll
-  for (int c = 0; c < SK_MAXC; ++c) n[c][0] = n[c][1] = s[c][0] = s[c][1] = 0;
-  const int64_t I8 = (I + 7) >> 3;
-  for (int64_t iw = 0; iw < I8; ++iw) {
-    const uint32_t w = gp[iw * Jp + j];
-    const int tn = I - iw * 8 < 8 ? (int)(I - iw * 8) : 8;
-    for (int t = 0; t < tn; ++t) {
-      const int lab = labels[iw * 8 + t];
-      const unsigned nib = (w >> (4 * t)) & 15u;
-      // observed count and dosage of each allele of the pair: code 0 -> 0, 2 -> 1, 3 -> 2, 1 missing
+  for (int c = 0; c < SK_MAXC; ++c) sa[c][0] = sa[c][1] = ma[c][0] = ma[c][1] = 0u;
+  constexpr uint32_t LOW = 0x11111111u;
+  int64_t iw = w0;
+  for (; iw + 4 <= w1; iw += 4) {
+    uint32_t w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = gp[(iw + u) * Jp + j];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
-        const unsigned code = (nib >> (2 * a)) & 3u;
-        const int ob = code != 1u ? 1 : 0;
-        const int d = code == 3u ? 2 : (code == 2u ? 1 : 0);
+        const uint32_t lo = (w[u] >> (2 * a)) & LOW, hi = (w[u] >> (2 * a + 1)) & LOW;
+        const uint32_t v = hi | ((hi & lo) << 1), mis = lo & ~hi;
 #pragma unroll
         for (int c = 0; c < SK_MAXC; ++c)
-          if (c < k && c == lab) {
-            n[c][a] += ob;
-            s[c][a] += d;
+          if (c < k) {
+            const uint32_t M = memb[(iw + u) * SK_MAXC + c];
+            sa[c][a] += __builtin_popcount(v & M);
+            ma[c][a] += __builtin_popcount(mis & M);
           }
       }
     }
   }
-  double g = 0.0;
+  for (; iw < w1; ++iw) {
+    const uint32_t w = gp[iw * Jp + j];
 #pragma unroll
-  for (int c = 0; c < SK_MAXC; ++c) {
-    if (c >= k) break;
+    for (int a = 0; a < 2; ++a) {
+      const uint32_t lo = (w >> (2 * a)) & LOW, hi = (w >> (2 * a + 1)) & LOW;
+      const uint32_t v = hi | ((hi & lo) << 1), mis = lo & ~hi;
+#pragma unroll
+      for (int c = 0; c < SK_MAXC; ++c)
+        if (c < k) {
+          const uint32_t M = memb[iw * SK_MAXC + c];
+          sa[c][a] += __builtin_popcount(v & M);
+          ma[c][a] += __builtin_popcount(mis & M);
+        }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < SK_MAXC; ++c)
+    if (c < k)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int64_t o = ((ch * k + c) * 2 + a) * J + j;
+        ps[o] = (int)sa[c][a];
+        pm[o] = (int)ma[c][a];
+      }
+}
+
+// Cluster centers over the observed entries (lane = block) from the integer partials, and block scores.
+// theta [k][2J]; score rows [k][J] per cluster (:local only) and the row over all clusters at
+// score + gr * J (gr = 0 for :global, k for :local).  Cluster sizes come from the host.
+__global__ void k_sk_centers(const int* __restrict__ ps, const int* __restrict__ pm, int nch, int64_t J, int k,
+                             const int64_t* __restrict__ csize, const double* __restrict__ mean,
+                             const double* __restrict__ sd, double* __restrict__ theta, double* __restrict__ score,
+                             int local) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= J) return;
+  double g = 0.0;
+  for (int c = 0; c < k; ++c) {
     double sc = 0.0;
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
+      int64_t S = 0, Mi = 0;
+      for (int ch = 0; ch < nch; ++ch) {
+        const int64_t o = (((int64_t)ch * k + c) * 2 + a) * J + j;
+        S += ps[o];
+        Mi += pm[o];
+      }
+      const int64_t n = csize[c] - Mi;  // observed entries among the members
       const double m = mean[2 * j + a], sdv = sd[2 * j + a];
       double th = 0.0;
-      if (n[c][a] > 0 && sdv > 0.0) th = ((double)s[c][a] / (double)n[c][a] - m) / sdv;
+      if (n > 0 && sdv > 0.0) th = ((double)S / (double)n - m) / sdv;
       theta[(int64_t)c * 2 * J + 2 * j + a] = th;
       sc = sc + th * th;
     }
     const double v = (double)csize[c] * sc;
     if (local) score[(int64_t)c * J + j] = v;
-    else g = g + v;
+    g = g + v;
   }
-  if (!local) score[j] = g;
+  score[(int64_t)(local ? k : 0) * J + j] = g;
+}
+
+// Sort key of the final ranking: the all-cluster score of a kept block, -1 for the others.
+__global__ void k_sk_rank_key(int64_t J, const int* __restrict__ flags, const double* __restrict__ gscore,
+                              double* __restrict__ key) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < J) key[b] = flags[b] ? gscore[b] : -1.0;
 }
 
 // Bit c of sel[b] for the first s blocks of a score-sorted list.
@@ -244,59 +338,127 @@ __global__ void k_sk_zero(int64_t J, int k, int local, const unsigned* __restric
   flags[b] = m != 0u ? 1 : 0;
 }
 
-// Reassignment over the kept blocks (ascending list, nblocks[0] of them): cost_c = sum (theta_cj^2 - 2 z theta_cj)
-// with a missing z = theta of the current cluster; wss[i] = sum over the kept SNPs of (z - theta_new)^2 (a
-// missing entry contributes 0); *changed counts the individuals that move.
-__global__ void __launch_bounds__(256) k_sk_assign(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J,
-                                                   int k, const int* __restrict__ blocks,
-                                                   const int* __restrict__ nblocks, const double* __restrict__ mean,
-                                                   const double* __restrict__ sd, const double* __restrict__ theta,
-                                                   const int* __restrict__ labels, int* __restrict__ labels_out,
-                                                   double* __restrict__ wss, int* __restrict__ changed) {
+// Reassignment cost over chunk blockIdx.y of the kept-block list (ascending, nblocks[0] entries, `ach` per
+// chunk): pc[(ch * k + c) * I + i] = sum over the chunk's SNPs of theta_cj^2 - 2 z theta_cj, a missing z
+// taken as theta of the current cluster.
+__global__ void __launch_bounds__(256) k_sk_cost(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J,
+                                                 int k, const int* __restrict__ blocks, const int* __restrict__ nblocks,
+                                                 int ach, const double* __restrict__ ztab,
+                                                 const double* __restrict__ theta, const int* __restrict__ labels,
+                                                 double* __restrict__ pc) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= I) return;
-  const int cur = labels[i];
   const int nb = *nblocks;
+  const int t0 = (int)blockIdx.y * ach;
+  if (t0 >= nb) return;
+  const int t1 = t0 + ach < nb ? t0 + ach : nb;
+  const double* thc = theta + (int64_t)labels[i] * 2 * J;
   double cost[SK_MAXC];
 #pragma unroll
   for (int c = 0; c < SK_MAXC; ++c) cost[c] = 0.0;
-  for (int t = 0; t < nb; ++t) {
-    const int64_t b = blocks[t];
-    const unsigned nib = (gq[(b >> 3) * Ip + i] >> (4 * (b & 7))) & 15u;
+  for (int t = t0; t < t1; t += 4) {
+    // four kept blocks per step: genotype words and table values fetched together, added in list order
+    const int tn = t1 - t < 4 ? t1 - t : 4;
+    int64_t b[4];
+    unsigned nib[4];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int64_t jj = 2 * b + a;
-      const double z = zval(dosage((nib >> (2 * a)) & 3u), mean[jj], sd[jj], theta[(int64_t)cur * 2 * J + jj]);
+    for (int u = 0; u < 4; ++u) b[u] = blocks[u < tn ? t + u : t];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) nib[u] = (gq[(b[u] >> 3) * Ip + i] >> (4 * (b[u] & 7))) & 15u;
+    double z2[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t jj = 2 * b[u >> 1] + (u & 1);
+      z2[u] = 2.0 * zval(ztab, jj, (nib[u >> 1] >> (2 * (u & 1))) & 3u, thc[jj]);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if ((u >> 1) >= tn) break;
+      const int64_t jj = 2 * b[u >> 1] + (u & 1);
 #pragma unroll
       for (int c = 0; c < SK_MAXC; ++c)
         if (c < k) {
           const double th = theta[(int64_t)c * 2 * J + jj];
-          cost[c] = cost[c] + (th * th - 2.0 * z * th);
+          cost[c] = cost[c] + (th * th - z2[u] * th);
         }
     }
   }
+#pragma unroll
+  for (int c = 0; c < SK_MAXC; ++c)
+    if (c < k) pc[((int64_t)blockIdx.y * k + c) * I + i] = cost[c];
+}
+
+// Chunk costs summed in chunk order, argmin (smaller c on a tie) -> labels_out; *changed counts movers.
+__global__ void k_sk_pick(const double* __restrict__ pc, const int* __restrict__ nblocks, int ach, int64_t I, int k,
+                          const int* __restrict__ labels, int* __restrict__ labels_out, int* __restrict__ changed) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I) return;
+  const int nch = (*nblocks + ach - 1) / ach;
+  double tot[SK_MAXC];
+#pragma unroll
+  for (int c = 0; c < SK_MAXC; ++c) tot[c] = 0.0;
+  for (int ch = 0; ch < nch; ++ch)
+#pragma unroll
+    for (int c = 0; c < SK_MAXC; ++c)
+      if (c < k) tot[c] = tot[c] + pc[((int64_t)ch * k + c) * I + i];
   int best = 0;
-  double bv = cost[0];
+  double bv = tot[0];
 #pragma unroll
   for (int c = 1; c < SK_MAXC; ++c)
-    if (c < k && cost[c] < bv) {
-      bv = cost[c];
+    if (c < k && tot[c] < bv) {
+      bv = tot[c];
       best = c;
     }
   labels_out[i] = best;
-  if (best != cur) atomicAdd(changed, 1);
+  if (best != labels[i]) atomicAdd(changed, 1);
+}
+
+// Within-cluster sum of squares of the new labels over chunk blockIdx.y of the kept list (a missing entry
+// contributes 0): pw[ch * I + i].
+__global__ void __launch_bounds__(256) k_sk_wss(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J,
+                                                const int* __restrict__ blocks, const int* __restrict__ nblocks,
+                                                int ach, const double* __restrict__ ztab,
+                                                const double* __restrict__ theta, const int* __restrict__ labels,
+                                                double* __restrict__ pw) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I) return;
+  const int nb = *nblocks;
+  const int t0 = (int)blockIdx.y * ach;
+  if (t0 >= nb) return;
+  const int t1 = t0 + ach < nb ? t0 + ach : nb;
+  const double* thb = theta + (int64_t)labels[i] * 2 * J;
   double w = 0.0;
-  for (int t = 0; t < nb; ++t) {
-    const int64_t b = blocks[t];
-    const unsigned nib = (gq[(b >> 3) * Ip + i] >> (4 * (b & 7))) & 15u;
+  for (int t = t0; t < t1; t += 4) {
+    const int tn = t1 - t < 4 ? t1 - t : 4;
+    int64_t b[4];
+    unsigned nib[4];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int64_t jj = 2 * b + a;
-      const double th = theta[(int64_t)best * 2 * J + jj];
-      const double e = zval(dosage((nib >> (2 * a)) & 3u), mean[jj], sd[jj], th) - th;
-      w = w + e * e;
+    for (int u = 0; u < 4; ++u) b[u] = blocks[u < tn ? t + u : t];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) nib[u] = (gq[(b[u] >> 3) * Ip + i] >> (4 * (b[u] & 7))) & 15u;
+    double e[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t jj = 2 * b[u >> 1] + (u & 1);
+      const double th = thb[jj];
+      e[u] = zval(ztab, jj, (nib[u >> 1] >> (2 * (u & 1))) & 3u, th) - th;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if ((u >> 1) >= tn) break;
+      w = w + e[u] * e[u];
     }
   }
+  pw[(int64_t)blockIdx.y * I + i] = w;
+}
+
+__global__ void k_sk_wss_sum(const double* __restrict__ pw, const int* __restrict__ nblocks, int ach, int64_t I,
+                             double* __restrict__ wss) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I) return;
+  const int nch = (*nblocks + ach - 1) / ach;
+  double w = 0.0;
+  for (int ch = 0; ch < nch; ++ch) w = w + pw[(int64_t)ch * I + i];
   wss[i] = w;
 }
 
@@ -334,6 +496,41 @@ struct DevBuf {
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+// Device time of one class of launches: start()/stop() bracket a launch on the stream, collect() after a
+// stream synchronisation adds the bracketed time.  Inactive (no events) when profiling is off.
+struct LaunchTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  bool pending = false;
+  double ms = 0.0;
+  int n = 0;
+  hipError_t init(bool on) {
+    if (!on) return hipSuccess;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    return e;
+  }
+  ~LaunchTimer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  void start(hipStream_t st) {
+    if (a) (void)hipEventRecord(a, st);
+  }
+  void stop(hipStream_t st) {
+    if (b) {
+      (void)hipEventRecord(b, st);
+      pending = true;
+    }
+  }
+  void collect() {
+    if (!pending) return;
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, a, b) == hipSuccess) ms += t;
+    ++n;
+    pending = false;
+  }
+};
+
 }  // namespace
 
 #define SKC(x)                                            \
@@ -347,7 +544,7 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, int tries, uint64_t seed,
              const int32_t* init, int32_t* clusters_out, int32_t* aims_out, int* n_aims, double* wss_out,
-             int* iters_out, hipStream_t st, char* err, size_t errlen) {
+             int* iters_out, double* prof, hipStream_t st, char* err, size_t errlen) {
   const int64_t I = g.I, J = g.J;
   if (k < 1 || k > SK_MAXC || k > I || sparsity < 1 || sparsity > J || max_iter < 1 || tries < 1 ||
       (local != 0 && local != 1)) {
@@ -363,21 +560,36 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
         return -1;
       }
   const int nch = (int)((J + SK_CHUNK - 1) / SK_CHUNK);
-  const int nscore = local ? k : 1;
+  const int nsel = local ? k : 1;                 // score rows that select blocks
+  const int64_t nw = (I + 7) / 8;                 // genotype words holding individuals
+  const int nic = (int)((nw + SK_CW - 1) / SK_CW);  // individual chunks of the center counts
+  const int64_t maxnb = local ? (J < (int64_t)k * sparsity ? J : (int64_t)k * sparsity) : sparsity;
+  const int ach = (int)std::max<int64_t>(SK_ACH_MIN, (maxnb + 63) / 64);  // kept blocks per cost partial
+  const int nach = (int)((maxnb + ach - 1) / ach);
+  const int64_t gr = local ? k : 0;               // row of the all-cluster score
   DevBuf db;
-  double *mean, *sd, *rows, *part, *d2, *theta, *score, *score_sorted, *wss;
-  int *labels, *labels2, *idx, *idx_sorted, *flags, *blocks, *nblocks, *changed;
+  double *mean, *sd, *ztab, *rows, *part, *d2, *theta, *score, *score_sorted, *key, *wss;
+  double *pc, *pw;
+  int *labels, *labels2, *idx, *idx_sorted, *flags, *blocks, *nblocks, *changed, *ps, *pm;
   unsigned* sel;
+  uint32_t* memb;
   int64_t* csize;
   SKC(db.get(&mean, 2 * J));
   SKC(db.get(&sd, 2 * J));
+  SKC(db.get(&ztab, 8 * J));
   SKC(db.get(&rows, (size_t)k * 2 * J));
   SKC(db.get(&part, (size_t)nch * k * I));
   SKC(db.get(&d2, I));
   SKC(db.get(&theta, (size_t)k * 2 * J));
-  SKC(db.get(&score, (size_t)nscore * J));
+  SKC(db.get(&score, (size_t)(gr + 1) * J));
   SKC(db.get(&score_sorted, J));
+  SKC(db.get(&key, J));
   SKC(db.get(&wss, I));
+  SKC(db.get(&pc, (size_t)nach * k * I));
+  SKC(db.get(&pw, (size_t)nach * I));
+  SKC(db.get(&ps, (size_t)nic * k * 2 * J));
+  SKC(db.get(&pm, (size_t)nic * k * 2 * J));
+  SKC(db.get(&memb, (size_t)nw * SK_MAXC));
   SKC(db.get(&labels, I));
   SKC(db.get(&labels2, I));
   SKC(db.get(&idx, J));
@@ -396,10 +608,26 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
   char* tmp = nullptr;
   SKC(db.get(&tmp, tmp_bytes));
 
-  hipLaunchKernelGGL(k_sk_stats, dim3(nblk(J)), dim3(256), 0, st, g.gp, g.Jp, J, g.I8, mean, sd);
+
+  // prof: device ms and launch counts of the center update (k_sk_member + k_sk_counts + k_sk_centers), the
+  // all-SNP distances (k_sk_dist) and the reassignment (k_sk_cost + k_sk_pick + k_sk_wss + k_sk_wss_sum),
+  // then the whole call's ms
+  LaunchTimer t_cen, t_dist, t_asg, t_all;
+  SKC(t_cen.init(prof != nullptr));
+  SKC(t_dist.init(prof != nullptr));
+  SKC(t_asg.init(prof != nullptr));
+  SKC(t_all.init(prof != nullptr));
+  auto sync = [&]() -> hipError_t {
+    const hipError_t e = hipStreamSynchronize(st);
+    t_cen.collect();
+    t_dist.collect();
+    t_asg.collect();
+    return e;
+  };
+  t_all.start(st);
+  hipLaunchKernelGGL(k_sk_stats, dim3(nblk(J)), dim3(256), 0, st, g.gp, g.Jp, J, g.I8, mean, sd, ztab);
   hipLaunchKernelGGL(k_sk_iota, dim3(nblk(J)), dim3(256), 0, st, idx, J);
   SKC(hipGetLastError());
-
   SplitMix rng{seed};
   std::vector<double> hd(I);
   std::vector<int> hlab(I), best_lab(I), best_blocks;
@@ -420,7 +648,7 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
         if (r >= I) r = I - 1;
       } else {
         SKC(hipMemcpyAsync(hd.data(), d2, sizeof(double) * I, hipMemcpyDeviceToHost, st));
-        SKC(hipStreamSynchronize(st));
+        SKC(sync());
         double tot = 0.0;
         for (int64_t i = 0; i < I; ++i) tot = tot + hd[i];
         const double u = rng.uniform() * tot;
@@ -435,15 +663,21 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
         }
       }
       double* row = rows + (int64_t)c * 2 * J;
-      hipLaunchKernelGGL(k_sk_row, dim3(nblk(J)), dim3(256), 0, st, g.gq, g.Ip, J, r, mean, sd, row);
+      hipLaunchKernelGGL(k_sk_row, dim3(nblk(J)), dim3(256), 0, st, g.gq, g.Ip, J, r, (const double*)ztab, row);
       if (!given && c + 1 < k) {
-        hipLaunchKernelGGL(k_sk_dist, gdist, dim3(256), 0, st, g.gq, g.Ip, I, J, 1, mean, sd, (const double*)row, part);
+        t_dist.start(st);
+        hipLaunchKernelGGL(k_sk_dist, gdist, dim3(256), 0, st, g.gq, g.Ip, I, J, 1, (const double*)ztab,
+                           (const double*)row, part);
+        t_dist.stop(st);
         hipLaunchKernelGGL(k_sk_dist_finish, dim3(nblk(I)), dim3(256), 0, st, (const double*)part, nch, 1, I, 0,
                            c == 0 ? 1 : 0, d2, labels);
       }
       SKC(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_sk_dist, gdist, dim3(256), 0, st, g.gq, g.Ip, I, J, k, mean, sd, (const double*)rows, part);
+    t_dist.start(st);
+    hipLaunchKernelGGL(k_sk_dist, gdist, dim3(256), 0, st, g.gq, g.Ip, I, J, k, (const double*)ztab,
+                       (const double*)rows, part);
+    t_dist.stop(st);
     hipLaunchKernelGGL(k_sk_dist_finish, dim3(nblk(I)), dim3(256), 0, st, (const double*)part, nch, k, I, 1, 0, d2,
                        labels);
     SKC(hipGetLastError());
@@ -451,14 +685,19 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
     double wsum = 0.0;
     for (it = 1; it <= max_iter; ++it) {
       SKC(hipMemcpyAsync(hlab.data(), labels, sizeof(int) * I, hipMemcpyDeviceToHost, st));
-      SKC(hipStreamSynchronize(st));
+      SKC(sync());
       for (int c = 0; c < k; ++c) cs[c] = 0;
       for (int64_t i = 0; i < I; ++i) cs[hlab[i]] += 1;
       SKC(hipMemcpyAsync(csize, cs.data(), sizeof(int64_t) * k, hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(k_sk_centers, dim3(nblk(J)), dim3(256), 0, st, g.gp, g.Jp, J, I, k, (const int*)labels,
-                         (const int64_t*)csize, mean, sd, theta, score, local);
+      t_cen.start(st);
+      hipLaunchKernelGGL(k_sk_member, dim3(nblk(nw)), dim3(256), 0, st, (const int*)labels, I, nw, memb);
+      hipLaunchKernelGGL(k_sk_counts, dim3(nblk(J), (unsigned)nic), dim3(256), 0, st, g.gp, g.Jp, J, nw, k,
+                         (const uint32_t*)memb, ps, pm);
+      hipLaunchKernelGGL(k_sk_centers, dim3(nblk(J)), dim3(256), 0, st, (const int*)ps, (const int*)pm, nic, J, k,
+                         (const int64_t*)csize, (const double*)mean, (const double*)sd, theta, score, local);
+      t_cen.stop(st);
       hipLaunchKernelGGL(k_sk_clear, dim3(nblk(J)), dim3(256), 0, st, sel, J, changed);
-      for (int c = 0; c < nscore; ++c) {
+      for (int c = 0; c < nsel; ++c) {
         size_t tb = tmp_bytes;
         SKC(hipcub::DeviceRadixSort::SortPairsDescending((void*)tmp, tb, score + (int64_t)c * J, score_sorted, idx,
                                                           idx_sorted, (int)J, 0, 64, st));
@@ -470,13 +709,22 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
         size_t tb = tmp_bytes;
         SKC(hipcub::DeviceSelect::Flagged((void*)tmp, tb, idx, flags, blocks, nblocks, (int)J, st));
       }
-      hipLaunchKernelGGL(k_sk_assign, dim3(nblk(I)), dim3(256), 0, st, g.gq, g.Ip, I, J, k, (const int*)blocks,
-                         (const int*)nblocks, mean, sd, (const double*)theta, (const int*)labels, labels2, wss,
-                         changed);
+      t_asg.start(st);
+      hipLaunchKernelGGL(k_sk_cost, dim3(nblk(I), (unsigned)nach), dim3(256), 0, st, g.gq, g.Ip, I, J, k,
+                         (const int*)blocks, (const int*)nblocks, ach, (const double*)ztab, (const double*)theta,
+                         (const int*)labels, pc);
+      hipLaunchKernelGGL(k_sk_pick, dim3(nblk(I)), dim3(256), 0, st, (const double*)pc, (const int*)nblocks, ach, I, k,
+                         (const int*)labels, labels2, changed);
+      hipLaunchKernelGGL(k_sk_wss, dim3(nblk(I), (unsigned)nach), dim3(256), 0, st, g.gq, g.Ip, I, J,
+                         (const int*)blocks, (const int*)nblocks, ach, (const double*)ztab, (const double*)theta,
+                         (const int*)labels2, pw);
+      hipLaunchKernelGGL(k_sk_wss_sum, dim3(nblk(I)), dim3(256), 0, st, (const double*)pw, (const int*)nblocks, ach, I,
+                         wss);
+      t_asg.stop(st);
       SKC(hipGetLastError());
       SKC(hipMemcpyAsync(&moved, changed, sizeof(int), hipMemcpyDeviceToHost, st));
       SKC(hipMemcpyAsync(hd.data(), wss, sizeof(double) * I, hipMemcpyDeviceToHost, st));
-      SKC(hipStreamSynchronize(st));
+      SKC(sync());
       std::swap(labels, labels2);
       wsum = 0.0;
       for (int64_t i = 0; i < I; ++i) wsum = wsum + hd[i];
@@ -487,13 +735,19 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
       best_wss = wsum;
       best_iters = it;
       int nb = 0;
+      // kept blocks ranked by the all-cluster score of the last center update
+      hipLaunchKernelGGL(k_sk_rank_key, dim3(nblk(J)), dim3(256), 0, st, J, (const int*)flags,
+                         (const double*)(score + gr * J), key);
+      size_t tb = tmp_bytes;
+      SKC(hipcub::DeviceRadixSort::SortPairsDescending((void*)tmp, tb, key, score_sorted, idx, idx_sorted, (int)J, 0,
+                                                        64, st));
       SKC(hipMemcpyAsync(best_lab.data(), labels, sizeof(int) * I, hipMemcpyDeviceToHost, st));
       SKC(hipMemcpyAsync(&nb, nblocks, sizeof(int), hipMemcpyDeviceToHost, st));
-      SKC(hipStreamSynchronize(st));
+      SKC(sync());
       best_blocks.resize(nb);
       if (nb > 0) {
-        SKC(hipMemcpyAsync(best_blocks.data(), blocks, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
-        SKC(hipStreamSynchronize(st));
+        SKC(hipMemcpyAsync(best_blocks.data(), idx_sorted, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+        SKC(sync());
       }
     }
   }
@@ -506,6 +760,13 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
   *n_aims = 2 * na;
   if (wss_out) *wss_out = best_wss;
   if (iters_out) *iters_out = best_iters;
+  if (prof) {
+    t_all.stop(st);
+    SKC(sync());
+    t_all.collect();
+    const double v[8] = {t_cen.ms, (double)t_cen.n, t_dist.ms, (double)t_dist.n, t_asg.ms, (double)t_asg.n, t_all.ms, 0.0};
+    for (int t = 0; t < 8; ++t) prof[t] = v[t];
+  }
   return 0;
 }
 

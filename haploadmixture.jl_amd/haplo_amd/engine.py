@@ -205,10 +205,10 @@ class HipAdmixData:
 
     # -- SKFR marker pre-selection ------------------------------------------------------------------
     def skfr(self, k: int, sparsity: int, mode: str = "global", max_iter: int = 50, tries: int = 1, seed: int = 1,
-             init=None) -> dict:
+             init=None, profile: bool = False) -> dict:
         """hk_skfr on this context's J blocks: sparse k-means with feature ranking (mode "global" =
-        sparsekmeans1, "local" = sparsekmeans2).  Returns clusters (I,), aims (0-based SNP indices,
-        ascending), wss and iters of the best of `tries` runs."""
+        sparsekmeans1, "local" = sparsekmeans2).  Returns clusters (I,), aims (0-based SNP indices, most
+        important block first), wss and iters of the best of `tries` runs."""
         if mode not in ("global", "local"):
             raise ValueError("mode must be 'global' or 'local'")
         clusters = np.zeros(self.I, dtype=np.int32)
@@ -217,14 +217,20 @@ class HipAdmixData:
         wss = ctypes.c_double()
         iters = ctypes.c_int()
         ini = None
+        prof = np.zeros(8) if profile else None
         if init is not None:
             ini = np.ascontiguousarray(init, dtype=np.int32)
             if ini.shape != (int(k),):
                 raise ValueError(f"init must hold k={k} individual indices")
         self._check(self._L.hk_skfr(self._h, int(k), int(sparsity), int(mode == "local"), int(max_iter), int(tries),
                                     int(seed) & ((1 << 64) - 1), ptr(ini), ptr(clusters), ptr(aims), ctypes.byref(n),
-                                    ctypes.byref(wss), ctypes.byref(iters)))
-        return dict(clusters=clusters, aims=aims[: n.value].copy(), wss=wss.value, iters=iters.value)
+                                    ctypes.byref(wss), ctypes.byref(iters), ptr(prof)))
+        out = dict(clusters=clusters, aims=aims[: n.value].copy(), wss=wss.value, iters=iters.value)
+        if profile:
+            out["profile"] = {"centers_ms": prof[0], "centers_launches": int(prof[1]), "dist_ms": prof[2],
+                              "dist_launches": int(prof[3]), "assign_ms": prof[4], "assign_launches": int(prof[5]),
+                              "device_ms": prof[6]}
+        return out
 
 
 def fit_batch(engines, opts_list, ll_start=None):

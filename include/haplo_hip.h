@@ -193,11 +193,15 @@ int hk_mlp_adam(hk_mlp* m, float lr, float beta1, float beta2, float eps, int st
  * `sparsity` blocks kept (1..J; :local keeps that many per cluster, markers = their union), at most
  * max_iter (skfr_max_inner_iter) reassignments per try; the try with the smallest within-cluster sum of
  * squares over the kept SNPs wins.  seed drives the k-means++ starts; init (k individual indices, or NULL)
- * replaces them in the first try.  Outputs: clusters (I labels 0..k-1), aims (0-based SNP indices,
- * ascending; room for 2 * min(J, k * sparsity)), n_aims, and (each may be NULL) wss and iters of the
- * returned try.  Runs synchronously on the context stream; fit parameters are untouched. */
+ * replaces them in the first try.  Outputs: clusters (I labels 0..k-1), aims (0-based SNP indices of the
+ * kept blocks, pairs 2b, 2b+1, most important block first as in docs/src/index.md:287, ranked by
+ * sum_c n_c |theta_c,b|^2 of the last center update; room for 2 * min(J, k * sparsity)), n_aims, and
+ * (each may be NULL) wss and iters of the returned try, and prof[8]: device ms and launch counts of the
+ * center-update, distance and reassignment kernels, then the whole call's device ms (HIP events; NULL =
+ * no timing).  Runs synchronously on the context stream; fit parameters are untouched. */
 int hk_skfr(hk_ctx* ctx, int k, int sparsity, int mode, int max_iter, int tries, uint64_t seed,
-            const int32_t* init, int32_t* clusters, int32_t* aims, int* n_aims, double* wss, int* iters);
+            const int32_t* init, int32_t* clusters, int32_t* aims, int* n_aims, double* wss, int* iters,
+            double* prof);
 
 /* Calibration: copy `bytes` device-to-device `reps` times with a plain streaming kernel (k_copy16, 16 B per
  * lane) on the context stream; gbps = (read + write bytes) / time.  Used to scale HBM byte counters. */

@@ -18,7 +18,9 @@ operation order as the device engine, so labels and markers agree exactly:
 * ``skfr``: center update (mean of the observed z per cluster), block scores
   sum_c n_c (theta_c,2b^2 + theta_c,2b+1^2) (global) or per cluster (local), stable top-s, theta = 0
   off the kept blocks, reassignment by sum_kept (theta^2 - 2 z theta) with a missing z = the center of
-  the current cluster, stop when no label moves; tries keep the smallest within-cluster sum of squares.
+  the current cluster (summed in partials of ``assign_chunk`` kept blocks), stop when no label moves; tries keep the smallest within-cluster sum of squares;
+  markers are the kept blocks' SNP pairs in decreasing order of the all-cluster score of the last center
+  update (ties to the smaller block), as ``aims`` is documented (docs/src/index.md:287).
 
 Parity against SparseKmeansFeatureRanking.jl itself is unpinned (its source and its RNG are not here).
 The restatement is pinned by tests/test_oracle_skfr.py: with every block kept and complete data it is
@@ -32,6 +34,7 @@ from __future__ import annotations
 import numpy as np
 
 CHUNK = 1024          # blocks per partial sum of the all-SNP distances (device: SK_CHUNK)
+ACH_MIN = 256         # kept blocks per partial of the reassignment sums, at least (device: SK_ACH_MIN)
 MAX_CLUSTERS = 8
 _MASK = (1 << 64) - 1
 
@@ -153,34 +156,46 @@ def top_blocks(score_row: np.ndarray, s: int) -> np.ndarray:
     return np.argsort(-score_row, kind="stable")[:s]
 
 
-def assign(Z: np.ndarray, theta: np.ndarray, labels: np.ndarray, blocks: np.ndarray):
-    """(new labels, per-individual WSS) over the kept blocks (ascending)."""
+def assign_chunk(maxnb: int) -> int:
+    """Kept blocks per partial of the reassignment sums: at least ACH_MIN, at most 64 partials."""
+    return max(ACH_MIN, -(-maxnb // 64))
+
+
+def assign(Z: np.ndarray, theta: np.ndarray, labels: np.ndarray, blocks: np.ndarray, ach: int):
+    """(new labels, per-individual WSS) over the kept blocks (ascending), summed in the device order:
+    blocks in list order within partials of `ach` blocks, partials added in order."""
     I = Z.shape[0]
     k = theta.shape[0]
     cost = np.zeros((I, k))
-    for b in blocks:
-        for a in (0, 1):
-            col = 2 * int(b) + a
-            z = Z[:, col]
-            z = np.where(np.isnan(z), theta[labels, col], z)
-            th = theta[:, col]
-            cost = cost + (th * th - 2.0 * z[:, None] * th)
+    for t0 in range(0, len(blocks), ach):
+        acc = np.zeros((I, k))
+        for b in blocks[t0:t0 + ach]:
+            for a in (0, 1):
+                col = 2 * int(b) + a
+                z = Z[:, col]
+                z2 = 2.0 * np.where(np.isnan(z), theta[labels, col], z)
+                th = theta[:, col]
+                acc = acc + (th * th - z2[:, None] * th)
+        cost = cost + acc
     best = np.argmin(cost, axis=1)
     w = np.zeros(I)
-    for b in blocks:
-        for a in (0, 1):
-            col = 2 * int(b) + a
-            z = Z[:, col]
-            th = theta[best, col]
-            e = np.where(np.isnan(z), th, z) - th
-            w = w + e * e
+    for t0 in range(0, len(blocks), ach):
+        acc = np.zeros(I)
+        for b in blocks[t0:t0 + ach]:
+            for a in (0, 1):
+                col = 2 * int(b) + a
+                z = Z[:, col]
+                th = theta[best, col]
+                e = np.where(np.isnan(z), th, z) - th
+                acc = acc + e * e
+        w = w + acc
     return best.astype(np.int32), w
 
 
 def skfr(D: np.ndarray, k: int, sparsity: int, *, mode: str = "global", max_iter: int = 50, tries: int = 1,
          seed: int = 1, init=None, return_history: bool = False) -> dict:
     """SKFR on the dosage matrix D (I x 2J, int8, -1 missing).  Returns clusters (I,), aims (0-based SNP
-    indices, ascending), wss, iters (and the per-try results when return_history)."""
+    indices, most important block first), wss, iters (and the per-try results when return_history)."""
     I, S = D.shape
     if S % 2:
         raise ValueError("D must hold whole SNP blocks (an even number of columns)")
@@ -190,6 +205,7 @@ def skfr(D: np.ndarray, k: int, sparsity: int, *, mode: str = "global", max_iter
     if mode not in ("global", "local"):
         raise ValueError("mode must be 'global' or 'local'")
     local = mode == "local"
+    ach = assign_chunk(min(J, k * sparsity) if local else sparsity)
     m, sd, Z = standardise(D)
     Z0 = np.where(np.isnan(Z), 0.0, Z)
     rng = SplitMix64(seed)
@@ -203,6 +219,7 @@ def skfr(D: np.ndarray, k: int, sparsity: int, *, mode: str = "global", max_iter
             csize = np.bincount(labels, minlength=k)
             theta = centers(D, labels, k, m, sd)
             sc = scores(theta, csize, local)
+            gscore = scores(theta, csize, False)[0]
             keep = np.zeros((k, J), dtype=bool)
             for c in range(sc.shape[0]):
                 keep[c, top_blocks(sc[c], sparsity)] = True
@@ -212,13 +229,15 @@ def skfr(D: np.ndarray, k: int, sparsity: int, *, mode: str = "global", max_iter
                 theta[c, 0::2][~keep[c]] = 0.0
                 theta[c, 1::2][~keep[c]] = 0.0
             blocks = np.nonzero(keep.any(axis=0))[0]
-            new, w = assign(Z, theta, labels, blocks)
+            new, w = assign(Z, theta, labels, blocks, ach)
             moved = int((new != labels).sum())
             labels = new
             wsum = float(np.cumsum(w)[-1])
             if moved == 0:
                 break
-        res = dict(clusters=labels.copy(), blocks=blocks.copy(), wss=wsum, iters=it, starts=starts)
+        key = np.where(keep.any(axis=0), gscore, -1.0)
+        ranked = np.argsort(-key, kind="stable")[: blocks.size]
+        res = dict(clusters=labels.copy(), blocks=ranked, wss=wsum, iters=it, starts=starts)
         history.append(res)
         if tr == 0 or wsum < best["wss"]:
             best = res

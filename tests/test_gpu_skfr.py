@@ -43,6 +43,8 @@ CASES = [
     (300, 120, 3, 8, "local", 3, 0.02, 6, None),
     (123, 45, 8, 1, "local", 1, 0.1, 3, None),
     (500, 2100, 6, 300, "local", 1, 0.05, 9, None),
+    (400, 3000, 4, 700, "global", 1, 0.02, 13, None),   # three reassignment partials
+    (2500, 300, 3, 30, "global", 2, 0.03, 14, None),     # three center-count partials
 ]
 
 
@@ -77,8 +79,11 @@ def test_skfr_fixed_point_at_scale():
         th = np.where((n > 0) & (sd > 0), (sm / np.maximum(n, 1) - m) / np.where(sd > 0, sd, 1), 0.0)
     size = oh.sum(axis=0)
     score = (size[:, None] * (th[:, 0::2] ** 2 + th[:, 1::2] ** 2)).sum(axis=0)
-    kept = np.unique(r["aims"] // 2)
-    assert kept.size == s and np.array_equal(r["aims"], np.stack([2 * kept, 2 * kept + 1], 1).ravel())
+    order = r["aims"][0::2] // 2
+    assert np.array_equal(r["aims"], np.stack([2 * order, 2 * order + 1], 1).ravel())
+    kept = np.unique(order)
+    assert kept.size == s
+    assert np.all(np.diff(score[order]) <= 1e-9 * score[order[0]])   # most important block first
     rest = np.setdiff1d(np.arange(J), kept)
     assert score[kept].min() >= score[rest].max() - 1e-9 * score[kept].min()
     cols = np.stack([2 * kept, 2 * kept + 1], 1).ravel()

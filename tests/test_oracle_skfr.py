@@ -54,7 +54,7 @@ def test_all_blocks_kept_is_lloyd_kmeans():
     km = sk.KMeans(n_clusters=3, init=Z[init], n_init=1, algorithm="lloyd", tol=0.0, max_iter=300).fit(Z)
     assert np.array_equal(r["clusters"], km.labels_)
     assert r["wss"] == pytest.approx(km.inertia_, rel=1e-9)
-    assert np.array_equal(r["aims"], np.arange(300))
+    assert np.array_equal(np.sort(r["aims"]), np.arange(300))
 
 
 def _bruteforce_fixed_point(D, res, k, s, local):
@@ -78,8 +78,14 @@ def _bruteforce_fixed_point(D, res, k, s, local):
             vals = [int(D[i, j]) for i in range(I) if labels[i] == c and D[i, j] >= 0]
             if vals and sd[j] > 0:
                 theta[c][j] = (sum(vals) / len(vals) - mean[j]) / sd[j]
-    kept = sorted(set(int(a) // 2 for a in res["aims"]))
+    aims = [int(a) for a in res["aims"]]
+    assert aims[1::2] == [a + 1 for a in aims[0::2]] and all(a % 2 == 0 for a in aims[0::2])
+    order = [a // 2 for a in aims[0::2]]
+    kept = sorted(order)
     score = [[size[c] * (theta[c][2 * b] ** 2 + theta[c][2 * b + 1] ** 2) for b in range(J)] for c in range(k)]
+    total = [sum(score[c][b] for c in range(k)) for b in range(J)]
+    # markers come most important first
+    assert all(total[x] >= total[y] - 1e-12 * max(1.0, total[x]) for x, y in zip(order, order[1:]))
     rows = score if local else [[sum(score[c][b] for c in range(k)) for b in range(J)]]
     keep = [[False] * J for _ in range(k)]
     for ci, row in enumerate(rows):
