@@ -16,7 +16,8 @@
 //          n_c (theta_c,2b^2 + theta_c,2b+1^2) (:local);
 //          keep the `sparsity` best blocks (stable: the smaller block wins a tie), theta = 0 elsewhere;
 //          reassign i to argmin_c sum over the kept SNPs of theta_cj^2 - 2 z_ij theta_cj (smaller c on a
-//          tie); stop when no label changes.
+//          tie; evaluated per partial of the kept list as sum theta^2 - sum (2 z) theta); stop when no
+//          label changes.
 //   tries  the try with the smallest within-cluster sum of squares over the kept SNPs wins.
 //   output labels, and the kept blocks' SNP pairs in decreasing order of importance (docs/src/index.md:287):
 //          the score sum_c n_c (theta_c,2b^2 + theta_c,2b+1^2) of the last center update, ties to the
@@ -47,6 +48,7 @@ namespace {
 
 constexpr int SK_MAXC = HK_MAXK;  // largest number of clusters
 constexpr int SK_CHUNK = 1024;    // blocks per partial sum of the all-SNP distance kernel
+constexpr int SK_SUB = 256;       // blocks (or kept-list entries) per LDS stage
 constexpr int SK_CW = 128;        // genotype words (1024 individuals) per partial of the center counts
 constexpr int SK_ACH_MIN = 256;   // kept blocks per partial of the reassignment (at least; <= 64 partials)
 
@@ -107,54 +109,78 @@ __global__ void k_sk_row(const uint32_t* __restrict__ gq, int64_t Ip, int64_t J,
 }
 
 // Squared distances of every individual to nc center rows over the blocks of chunk blockIdx.y (missing z ->
-// 0): part[(ch * nc + c) * I + i], SNPs ascending.  Grid: (ceil(I / 256), ceil(J / SK_CHUNK)).
+// 0): part[(ch * nc + c) * I + i], SNPs ascending.  Grid: (ceil(I / 256), ceil(J / SK_CHUNK)); dynamic LDS
+// SK_DIST_LDS bytes.  The chunk is staged dist_sub(nc) blocks at a time as a table of the squared
+// differences e2[c][snp][code] = (z(code) - row_c,snp)^2 (code 0b01 = missing: z = 0), computed with the
+// same operations as the element-wise form, so each element costs one LDS gather and one add per center.
+constexpr size_t SK_DIST_LDS = 16384;  // bytes: 2048 doubles of e2 table per stage
+__host__ __device__ constexpr int dist_sub(int nc) { return nc <= 1 ? 256 : (nc <= 2 ? 128 : (nc <= 4 ? 64 : 32)); }
+
+template <int NC>
 __global__ void __launch_bounds__(256) k_sk_dist(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J,
-                                                 int nc, const double* __restrict__ ztab,
-                                                 const double* __restrict__ rows, double* __restrict__ part) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= I) return;
+                                                 const double* __restrict__ ztab, const double* __restrict__ rows,
+                                                 double* __restrict__ part) {
+  extern __shared__ double sk_lds[];
+  double* e2 = sk_lds;  // [NC][2 sub][4]
+  constexpr int nc = NC;
+  constexpr int sub = dist_sub(NC), snps = 2 * sub, NWQ = sub / 8;
+  const int tid = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+  const bool act = i < I;
   const int64_t ch = blockIdx.y;
   const int64_t j0 = ch * SK_CHUNK;
   const int64_t j1 = j0 + SK_CHUNK < J ? j0 + SK_CHUNK : J;
-  double acc[SK_MAXC];
+  double acc[NC];
 #pragma unroll
-  for (int c = 0; c < SK_MAXC; ++c) acc[c] = 0.0;
-  for (int64_t jw = j0 >> 3; jw * 8 < j1; ++jw) {
-    const uint32_t w = gq[jw * Ip + i];
-    if (jw * 8 + 8 <= j1) {
-      // whole word: the 16 table values are fetched together, then added in SNP order
-      double z[16];
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+  for (int64_t s0 = j0; s0 < j1; s0 += sub) {
+    const int64_t s1 = s0 + sub < j1 ? s0 + sub : j1;
+    const int ns = (int)(2 * (s1 - s0));              // SNPs of this stage
+    const int64_t w0 = s0 >> 3, nwq = ((s1 + 7) >> 3) - w0;  // s0 is a multiple of 8; nwq <= NWQ
+    // the stage's genotype words are requested before the table is built, so their latency overlaps it
+    uint32_t wv[NWQ];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) z[u] = zval(ztab, 16 * jw + u, (w >> (2 * u)) & 3u, 0.0);
+    for (int q = 0; q < NWQ; ++q) wv[q] = (act && q < nwq) ? gq[(w0 + q) * Ip + i] : 0u;
+    __syncthreads();  // the previous stage has been consumed
+    for (int c = 0; c < nc; ++c)
+      for (int r = tid; r < 4 * ns; r += 256) {  // r = 4 snp + code
+        const int code = r & 3;
+        const int64_t col = 2 * s0 + (r >> 2);
+        const double z = code == 1 ? 0.0 : ztab[col * 4 + (code == 0 ? 0 : code - 1)];
+        const double e = z - rows[(int64_t)c * 2 * J + col];
+        e2[c * 4 * snps + r] = e * e;
+      }
+    __syncthreads();
+    if (!act) continue;
+    if (ns == snps) {  // whole stage: every word full
 #pragma unroll
-      for (int u = 0; u < 16; ++u)
+      for (int q = 0; q < NWQ; ++q) {
+        const uint32_t w = wv[q];
 #pragma unroll
-        for (int c = 0; c < SK_MAXC; ++c)
-          if (c < nc) {
-            const double e = z[u] - rows[(int64_t)c * 2 * J + 16 * jw + u];
-            acc[c] = acc[c] + e * e;
-          }
-      continue;
-    }
-    const int tn = (int)(j1 - jw * 8);
-    for (int t = 0; t < tn; ++t) {
-      const int64_t j = jw * 8 + t;
-      const unsigned nib = (w >> (4 * t)) & 15u;
+        for (int u = 0; u < 16; ++u) {
+          const int r = 4 * (16 * q + u) + (int)((w >> (2 * u)) & 3u);
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const double z = zval(ztab, 2 * j + a, (nib >> (2 * a)) & 3u, 0.0);
+          for (int c = 0; c < NC; ++c) acc[c] = acc[c] + e2[c * 4 * snps + r];
+        }
+      }
+    } else {  // the last, partial stage of the SNP range
+      for (int q = 0; q < nwq; ++q) {
+        uint32_t w = 0u;
 #pragma unroll
-        for (int c = 0; c < SK_MAXC; ++c)
-          if (c < nc) {
-            const double e = z - rows[(int64_t)c * 2 * J + 2 * j + a];
-            acc[c] = acc[c] + e * e;
-          }
+        for (int x = 0; x < NWQ; ++x)
+          if (x == q) w = wv[x];
+        const int nu = ns - 16 * q < 16 ? ns - 16 * q : 16;
+        for (int u = 0; u < nu; ++u) {
+          const int r = 4 * (16 * q + u) + (int)((w >> (2 * u)) & 3u);
+#pragma unroll
+          for (int c = 0; c < NC; ++c) acc[c] = acc[c] + e2[c * 4 * snps + r];
+        }
       }
     }
   }
+  if (!act) return;
 #pragma unroll
-  for (int c = 0; c < SK_MAXC; ++c)
-    if (c < nc) part[(ch * nc + c) * I + i] = acc[c];
+  for (int c = 0; c < NC; ++c) part[(ch * NC + c) * I + i] = acc[c];
 }
 
 // Chunk partials summed in chunk order.  nearest == 0 (nc = 1): d2[i] = first ? d : min(d2[i], d);
@@ -209,9 +235,11 @@ __global__ void k_sk_member(const int* __restrict__ labels, int64_t I, int64_t n
 // blockIdx.y (SK_CW words), from popcounts: with lo / hi the low / high code bits of an allele at each
 // individual's nibble, dosage = hi + (hi & lo) and missing = lo & ~hi.  Integer partials:
 // ps / pm [chunk][k][2][J].
+template <int KC>
 __global__ void __launch_bounds__(256) k_sk_counts(const uint32_t* __restrict__ gp, int64_t Jp, int64_t J, int64_t nw,
-                                                   int k, const uint32_t* __restrict__ memb, int* __restrict__ ps,
+                                                   const uint32_t* __restrict__ memb, int* __restrict__ ps,
                                                    int* __restrict__ pm) {
+  constexpr int k = KC;
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= J) return;
   const int64_t ch = blockIdx.y;
@@ -338,14 +366,62 @@ __global__ void k_sk_zero(int64_t J, int k, int local, const unsigned* __restric
   flags[b] = m != 0u ? 1 : 0;
 }
 
-// Reassignment cost over chunk blockIdx.y of the kept-block list (ascending, nblocks[0] entries, `ach` per
-// chunk): pc[(ch * k + c) * I + i] = sum over the chunk's SNPs of theta_cj^2 - 2 z theta_cj, a missing z
-// taken as theta of the current cluster.
+// LDS bytes of the kept-list staging of k_sk_cost / k_sk_wss: value table (8 SK_SUB) + theta (2 SK_SUB k).
+__host__ __device__ constexpr size_t kept_lds_bytes(int k) { return (size_t)(8 * SK_SUB + 2 * SK_SUB * k) * 8; }
+
+// Stage SK_SUB kept-list entries [u0, u0 + nu) in LDS: zt[8 t + 4 a + d] = value table of SNP 2 b_t + a,
+// th[c * 2 SK_SUB + 2 t + a] = theta of cluster c there.  Called by all threads between barriers.
+__device__ __forceinline__ void stage_kept(const int* __restrict__ blocks, int u0, int nu, int64_t J, int k,
+                                           const double* __restrict__ ztab, const double* __restrict__ theta,
+                                           double* zt, double* th) {
+  const int tid = threadIdx.x;
+  for (int x = tid; x < 8 * nu; x += 256) zt[x] = ztab[8 * (int64_t)blocks[u0 + (x >> 3)] + (x & 7)];
+  for (int x = tid; x < 2 * nu * k; x += 256) {
+    const int c = x / (2 * nu), r = x - c * 2 * nu;
+    th[c * 2 * SK_SUB + r] = theta[(int64_t)c * 2 * J + 2 * (int64_t)blocks[u0 + (r >> 1)] + (r & 1)];
+  }
+}
+
+// Per-chunk constant of the reassignment cost: ca[ch * k + c] = sum over the SNPs of kept-list chunk ch
+// (list order) of theta_cj^2.  One thread per (chunk, cluster).
+__global__ void k_sk_chunk_a(const int* __restrict__ blocks, const int* __restrict__ nblocks, int ach, int nach,
+                             int64_t J, int k, const double* __restrict__ theta, double* __restrict__ ca) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nach * k) return;
+  const int ch = x / k, c = x - ch * k;
+  const int nb = *nblocks;
+  const int t0 = ch * ach, t1 = t0 + ach < nb ? t0 + ach : nb;
+  const double* th = theta + (int64_t)c * 2 * J;
+  double a = 0.0;
+  for (int t = t0; t < t1; t += 16) {
+    // sixteen blocks' values fetched together, then added in list order
+    double v[32];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int64_t b = blocks[t + u < t1 ? t + u : t];
+      v[2 * u] = th[2 * b];
+      v[2 * u + 1] = th[2 * b + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      if (t + u >= t1) break;
+      a = a + v[2 * u] * v[2 * u];
+      a = a + v[2 * u + 1] * v[2 * u + 1];
+    }
+  }
+  ca[x] = a;
+}
+
+// Reassignment cross term over chunk blockIdx.y of the kept-block list (ascending, nblocks[0] entries, `ach`
+// per chunk): pc[(ch * k + c) * I + i] = sum over the chunk's SNPs of (2 z_ij) theta_cj, a missing z taken
+// as theta of the current cluster; the cost of cluster c is sum_ch (ca[ch][c] - pc[ch][c][i]).
+template <int KC>
 __global__ void __launch_bounds__(256) k_sk_cost(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J,
-                                                 int k, const int* __restrict__ blocks, const int* __restrict__ nblocks,
+                                                 const int* __restrict__ blocks, const int* __restrict__ nblocks,
                                                  int ach, const double* __restrict__ ztab,
                                                  const double* __restrict__ theta, const int* __restrict__ labels,
                                                  double* __restrict__ pc) {
+  constexpr int k = KC;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= I) return;
   const int nb = *nblocks;
@@ -377,10 +453,7 @@ __global__ void __launch_bounds__(256) k_sk_cost(const uint32_t* __restrict__ gq
       const int64_t jj = 2 * b[u >> 1] + (u & 1);
 #pragma unroll
       for (int c = 0; c < SK_MAXC; ++c)
-        if (c < k) {
-          const double th = theta[(int64_t)c * 2 * J + jj];
-          cost[c] = cost[c] + (th * th - z2[u] * th);
-        }
+        if (c < k) cost[c] = cost[c] + z2[u] * theta[(int64_t)c * 2 * J + jj];
     }
   }
 #pragma unroll
@@ -388,9 +461,10 @@ __global__ void __launch_bounds__(256) k_sk_cost(const uint32_t* __restrict__ gq
     if (c < k) pc[((int64_t)blockIdx.y * k + c) * I + i] = cost[c];
 }
 
-// Chunk costs summed in chunk order, argmin (smaller c on a tie) -> labels_out; *changed counts movers.
-__global__ void k_sk_pick(const double* __restrict__ pc, const int* __restrict__ nblocks, int ach, int64_t I, int k,
-                          const int* __restrict__ labels, int* __restrict__ labels_out, int* __restrict__ changed) {
+// Chunk costs ca - pc summed in chunk order, argmin (smaller c on a tie) -> labels_out; *changed counts movers.
+__global__ void k_sk_pick(const double* __restrict__ pc, const double* __restrict__ ca, const int* __restrict__ nblocks,
+                          int ach, int64_t I, int k, const int* __restrict__ labels, int* __restrict__ labels_out,
+                          int* __restrict__ changed) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= I) return;
   const int nch = (*nblocks + ach - 1) / ach;
@@ -400,7 +474,7 @@ __global__ void k_sk_pick(const double* __restrict__ pc, const int* __restrict__
   for (int ch = 0; ch < nch; ++ch)
 #pragma unroll
     for (int c = 0; c < SK_MAXC; ++c)
-      if (c < k) tot[c] = tot[c] + pc[((int64_t)ch * k + c) * I + i];
+      if (c < k) tot[c] = tot[c] + (ca[ch * k + c] - pc[((int64_t)ch * k + c) * I + i]);
   int best = 0;
   double bv = tot[0];
 #pragma unroll
@@ -414,42 +488,52 @@ __global__ void k_sk_pick(const double* __restrict__ pc, const int* __restrict__
 }
 
 // Within-cluster sum of squares of the new labels over chunk blockIdx.y of the kept list (a missing entry
-// contributes 0): pw[ch * I + i].
+// contributes 0): pw[ch * I + i].  Dynamic LDS kept_lds_bytes(k).
 __global__ void __launch_bounds__(256) k_sk_wss(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J,
-                                                const int* __restrict__ blocks, const int* __restrict__ nblocks,
+                                                int k, const int* __restrict__ blocks, const int* __restrict__ nblocks,
                                                 int ach, const double* __restrict__ ztab,
                                                 const double* __restrict__ theta, const int* __restrict__ labels,
                                                 double* __restrict__ pw) {
+  extern __shared__ double sk_lds[];
+  double* zt = sk_lds;
+  double* th = sk_lds + 8 * SK_SUB;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= I) return;
+  const bool act = i < I;
   const int nb = *nblocks;
   const int t0 = (int)blockIdx.y * ach;
-  if (t0 >= nb) return;
+  if (t0 >= nb) return;  // uniform over the workgroup
   const int t1 = t0 + ach < nb ? t0 + ach : nb;
-  const double* thb = theta + (int64_t)labels[i] * 2 * J;
+  const int lab = act ? labels[i] : 0;
   double w = 0.0;
-  for (int t = t0; t < t1; t += 4) {
-    const int tn = t1 - t < 4 ? t1 - t : 4;
-    int64_t b[4];
-    unsigned nib[4];
+  for (int u0 = t0; u0 < t1; u0 += SK_SUB) {
+    const int nu = t1 - u0 < SK_SUB ? t1 - u0 : SK_SUB;
+    __syncthreads();
+    stage_kept(blocks, u0, nu, J, k, ztab, theta, zt, th);
+    __syncthreads();
+    if (!act) continue;
+    for (int t = 0; t < nu; t += 4) {
+      const int tn = nu - t < 4 ? nu - t : 4;
+      unsigned nib[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) b[u] = blocks[u < tn ? t + u : t];
+      for (int u = 0; u < 4; ++u) {
+        const int64_t b = blocks[u0 + t + (u < tn ? u : 0)];
+        nib[u] = (gq[(b >> 3) * Ip + i] >> (4 * (b & 7))) & 15u;
+      }
+      double e[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) nib[u] = (gq[(b[u] >> 3) * Ip + i] >> (4 * (b[u] & 7))) & 15u;
-    double e[8];
+      for (int u = 0; u < 8; ++u) {
+        const int r = 2 * t + (u < 2 * tn ? u : 0);
+        const double tb = th[lab * 2 * SK_SUB + r];
+        e[u] = zval(zt, r, (nib[u >> 1] >> (2 * (u & 1))) & 3u, tb) - tb;
+      }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int64_t jj = 2 * b[u >> 1] + (u & 1);
-      const double th = thb[jj];
-      e[u] = zval(ztab, jj, (nib[u >> 1] >> (2 * (u & 1))) & 3u, th) - th;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if ((u >> 1) >= tn) break;
-      w = w + e[u] * e[u];
+      for (int u = 0; u < 8; ++u) {
+        if ((u >> 1) >= tn) break;
+        w = w + e[u] * e[u];
+      }
     }
   }
-  pw[(int64_t)blockIdx.y * I + i] = w;
+  if (act) pw[(int64_t)blockIdx.y * I + i] = w;
 }
 
 __global__ void k_sk_wss_sum(const double* __restrict__ pw, const int* __restrict__ nblocks, int ach, int64_t I,
@@ -533,6 +617,21 @@ struct LaunchTimer {
 
 }  // namespace
 
+// Launch a kernel template instantiated for the cluster count kc (1..SK_MAXC) as KC_.
+#define SK_BY_K(kc, LAUNCH)                          \
+  do {                                               \
+    switch (kc) {                                    \
+      case 1: { constexpr int KC_ = 1; LAUNCH; } break; \
+      case 2: { constexpr int KC_ = 2; LAUNCH; } break; \
+      case 3: { constexpr int KC_ = 3; LAUNCH; } break; \
+      case 4: { constexpr int KC_ = 4; LAUNCH; } break; \
+      case 5: { constexpr int KC_ = 5; LAUNCH; } break; \
+      case 6: { constexpr int KC_ = 6; LAUNCH; } break; \
+      case 7: { constexpr int KC_ = 7; LAUNCH; } break; \
+      default: { constexpr int KC_ = 8; LAUNCH; } break; \
+    }                                                \
+  } while (0)
+
 #define SKC(x)                                            \
   do {                                                    \
     const hipError_t e_ = (x);                            \
@@ -569,7 +668,7 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
   const int64_t gr = local ? k : 0;               // row of the all-cluster score
   DevBuf db;
   double *mean, *sd, *ztab, *rows, *part, *d2, *theta, *score, *score_sorted, *key, *wss;
-  double *pc, *pw;
+  double *pc, *pw, *ca;
   int *labels, *labels2, *idx, *idx_sorted, *flags, *blocks, *nblocks, *changed, *ps, *pm;
   unsigned* sel;
   uint32_t* memb;
@@ -587,6 +686,7 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
   SKC(db.get(&wss, I));
   SKC(db.get(&pc, (size_t)nach * k * I));
   SKC(db.get(&pw, (size_t)nach * I));
+  SKC(db.get(&ca, (size_t)nach * k));
   SKC(db.get(&ps, (size_t)nic * k * 2 * J));
   SKC(db.get(&pm, (size_t)nic * k * 2 * J));
   SKC(db.get(&memb, (size_t)nw * SK_MAXC));
@@ -611,7 +711,7 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
 
   // prof: device ms and launch counts of the center update (k_sk_member + k_sk_counts + k_sk_centers), the
   // all-SNP distances (k_sk_dist) and the reassignment (k_sk_cost + k_sk_pick + k_sk_wss + k_sk_wss_sum),
-  // then the whole call's ms
+  // then the whole call's ms (the reassignment class includes k_sk_chunk_a)
   LaunchTimer t_cen, t_dist, t_asg, t_all;
   SKC(t_cen.init(prof != nullptr));
   SKC(t_dist.init(prof != nullptr));
@@ -666,7 +766,7 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
       hipLaunchKernelGGL(k_sk_row, dim3(nblk(J)), dim3(256), 0, st, g.gq, g.Ip, J, r, (const double*)ztab, row);
       if (!given && c + 1 < k) {
         t_dist.start(st);
-        hipLaunchKernelGGL(k_sk_dist, gdist, dim3(256), 0, st, g.gq, g.Ip, I, J, 1, (const double*)ztab,
+        hipLaunchKernelGGL((k_sk_dist<1>), gdist, dim3(256), SK_DIST_LDS, st, g.gq, g.Ip, I, J, (const double*)ztab,
                            (const double*)row, part);
         t_dist.stop(st);
         hipLaunchKernelGGL(k_sk_dist_finish, dim3(nblk(I)), dim3(256), 0, st, (const double*)part, nch, 1, I, 0,
@@ -675,8 +775,8 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
       SKC(hipGetLastError());
     }
     t_dist.start(st);
-    hipLaunchKernelGGL(k_sk_dist, gdist, dim3(256), 0, st, g.gq, g.Ip, I, J, k, (const double*)ztab,
-                       (const double*)rows, part);
+    SK_BY_K(k, hipLaunchKernelGGL((k_sk_dist<KC_>), gdist, dim3(256), SK_DIST_LDS, st, g.gq, g.Ip, I, J,
+                                  (const double*)ztab, (const double*)rows, part));
     t_dist.stop(st);
     hipLaunchKernelGGL(k_sk_dist_finish, dim3(nblk(I)), dim3(256), 0, st, (const double*)part, nch, k, I, 1, 0, d2,
                        labels);
@@ -691,8 +791,8 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
       SKC(hipMemcpyAsync(csize, cs.data(), sizeof(int64_t) * k, hipMemcpyHostToDevice, st));
       t_cen.start(st);
       hipLaunchKernelGGL(k_sk_member, dim3(nblk(nw)), dim3(256), 0, st, (const int*)labels, I, nw, memb);
-      hipLaunchKernelGGL(k_sk_counts, dim3(nblk(J), (unsigned)nic), dim3(256), 0, st, g.gp, g.Jp, J, nw, k,
-                         (const uint32_t*)memb, ps, pm);
+      SK_BY_K(k, hipLaunchKernelGGL((k_sk_counts<KC_>), dim3(nblk(J), (unsigned)nic), dim3(256), 0, st, g.gp, g.Jp,
+                                    J, nw, (const uint32_t*)memb, ps, pm));
       hipLaunchKernelGGL(k_sk_centers, dim3(nblk(J)), dim3(256), 0, st, (const int*)ps, (const int*)pm, nic, J, k,
                          (const int64_t*)csize, (const double*)mean, (const double*)sd, theta, score, local);
       t_cen.stop(st);
@@ -710,12 +810,14 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
         SKC(hipcub::DeviceSelect::Flagged((void*)tmp, tb, idx, flags, blocks, nblocks, (int)J, st));
       }
       t_asg.start(st);
-      hipLaunchKernelGGL(k_sk_cost, dim3(nblk(I), (unsigned)nach), dim3(256), 0, st, g.gq, g.Ip, I, J, k,
-                         (const int*)blocks, (const int*)nblocks, ach, (const double*)ztab, (const double*)theta,
-                         (const int*)labels, pc);
-      hipLaunchKernelGGL(k_sk_pick, dim3(nblk(I)), dim3(256), 0, st, (const double*)pc, (const int*)nblocks, ach, I, k,
-                         (const int*)labels, labels2, changed);
-      hipLaunchKernelGGL(k_sk_wss, dim3(nblk(I), (unsigned)nach), dim3(256), 0, st, g.gq, g.Ip, I, J,
+      hipLaunchKernelGGL(k_sk_chunk_a, dim3(nblk((int64_t)nach * k)), dim3(256), 0, st, (const int*)blocks,
+                         (const int*)nblocks, ach, nach, J, k, (const double*)theta, ca);
+      SK_BY_K(k, hipLaunchKernelGGL((k_sk_cost<KC_>), dim3(nblk(I), (unsigned)nach), dim3(256), 0, st, g.gq, g.Ip, I,
+                                    J, (const int*)blocks, (const int*)nblocks, ach, (const double*)ztab,
+                                    (const double*)theta, (const int*)labels, pc));
+      hipLaunchKernelGGL(k_sk_pick, dim3(nblk(I)), dim3(256), 0, st, (const double*)pc, (const double*)ca,
+                         (const int*)nblocks, ach, I, k, (const int*)labels, labels2, changed);
+      hipLaunchKernelGGL(k_sk_wss, dim3(nblk(I), (unsigned)nach), dim3(256), kept_lds_bytes(k), st, g.gq, g.Ip, I, J, k,
                          (const int*)blocks, (const int*)nblocks, ach, (const double*)ztab, (const double*)theta,
                          (const int*)labels2, pw);
       hipLaunchKernelGGL(k_sk_wss_sum, dim3(nblk(I)), dim3(256), 0, st, (const double*)pw, (const int*)nblocks, ach, I,
@@ -771,5 +873,6 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
 }
 
 #undef SKC
+#undef SK_BY_K
 
 }  // namespace hk

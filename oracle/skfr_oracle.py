@@ -17,8 +17,9 @@ operation order as the device engine, so labels and markers agree exactly:
   z taken as 0, summed in chunks of ``CHUNK`` blocks (SNPs ascending, then chunks ascending);
 * ``skfr``: center update (mean of the observed z per cluster), block scores
   sum_c n_c (theta_c,2b^2 + theta_c,2b+1^2) (global) or per cluster (local), stable top-s, theta = 0
-  off the kept blocks, reassignment by sum_kept (theta^2 - 2 z theta) with a missing z = the center of
-  the current cluster (summed in partials of ``assign_chunk`` kept blocks), stop when no label moves; tries keep the smallest within-cluster sum of squares;
+  off the kept blocks, reassignment by sum_kept theta^2 - sum_kept (2 z) theta with a missing z = the
+  center of the current cluster (both sums per partial of ``assign_chunk`` kept blocks, partials of
+  sum theta^2 - sum (2 z) theta added in order), stop when no label moves; tries keep the smallest within-cluster sum of squares;
   markers are the kept blocks' SNP pairs in decreasing order of the all-cluster score of the last center
   update (ties to the smaller block), as ``aims`` is documented (docs/src/index.md:287).
 
@@ -168,15 +169,18 @@ def assign(Z: np.ndarray, theta: np.ndarray, labels: np.ndarray, blocks: np.ndar
     k = theta.shape[0]
     cost = np.zeros((I, k))
     for t0 in range(0, len(blocks), ach):
+        # chunk constant sum theta^2 and cross term sum (2 z) theta, each in list order
+        a_c = np.zeros(k)
         acc = np.zeros((I, k))
         for b in blocks[t0:t0 + ach]:
             for a in (0, 1):
                 col = 2 * int(b) + a
+                th = theta[:, col]
+                a_c = a_c + th * th
                 z = Z[:, col]
                 z2 = 2.0 * np.where(np.isnan(z), theta[labels, col], z)
-                th = theta[:, col]
-                acc = acc + (th * th - z2[:, None] * th)
-        cost = cost + acc
+                acc = acc + z2[:, None] * th
+        cost = cost + (a_c[None, :] - acc)
     best = np.argmin(cost, axis=1)
     w = np.zeros(I)
     for t0 in range(0, len(blocks), ach):
