@@ -58,28 +58,50 @@ __global__ void k_gauss(float* __restrict__ R, int64_t n, uint64_t seed) {
 // dosage of BED code c: 0 -> 0, 2 (het) -> 1, 3 -> 2; code 1 = missing.
 __device__ __forceinline__ float dosage(unsigned c) { return c == 3u ? 2.0f : (c == 2u ? 1.0f : 0.0f); }
 
-// mu[s] for s < 2 nb: mean dosage over observed individuals (thread per SNP; fixed order).
-__global__ void k_snp_mean(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int nb, float* __restrict__ mu) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= 2 * nb) return;
-  const int j = s >> 1, hi = s & 1;
-  const uint32_t* col = gq + (int64_t)(j >> 3) * Ip;
-  const int sh = 4 * (j & 7) + 2 * hi;
-  double sum = 0.0;
-  int64_t n = 0;
-  for (int64_t i = 0; i < I; ++i) {
-    const unsigned c = (col[i] >> sh) & 3u;
-    if (c != 1u) { sum += dosage(c); ++n; }
+// mu[s] for s < 2 nb: mean dosage over observed individuals.  Integer dosage sums and observed counts per
+// SNP from popcounts of the block-major stream (lane = block, grid.y = chunk of MU_CW individual words;
+// with lo / hi the code bits of a SNP, dosage = hi + (hi & lo), missing = lo & ~hi), added with integer
+// atomics (exact in any order); then mu = sum / n.
+constexpr int MU_CW = 64;
+__global__ void k_snp_counts(const uint32_t* __restrict__ gp, int64_t Jp, int nb, int64_t nw,
+                             unsigned long long* __restrict__ cnt) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nb) return;
+  const int64_t w0 = (int64_t)blockIdx.y * MU_CW, w1 = w0 + MU_CW < nw ? w0 + MU_CW : nw;
+  constexpr uint32_t LOW = 0x11111111u;
+  unsigned sum[2] = {0u, 0u}, obs[2] = {0u, 0u};
+  for (int64_t iw = w0; iw < w1; ++iw) {
+    const uint32_t w = gp[iw * Jp + j];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const uint32_t lo = (w >> (2 * a)) & LOW, hi = (w >> (2 * a + 1)) & LOW;
+      sum[a] += __builtin_popcount(hi) + __builtin_popcount(hi & lo);
+      obs[a] += 8u - __builtin_popcount(lo & ~hi);
+    }
   }
-  mu[s] = n ? (float)(sum / (double)n) : 0.0f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    atomicAdd(&cnt[2 * (2 * j + a)], (unsigned long long)sum[a]);
+    atomicAdd(&cnt[2 * (2 * j + a) + 1], (unsigned long long)obs[a]);
+  }
 }
 
-// F[i * D + d]: workgroup = 64 individuals x 256 features; thread = 16 individuals x 4 features.
-// LDS holds 32 SNPs of R and of the centred dosages per stage.
+__global__ void k_snp_mu(const unsigned long long* __restrict__ cnt, int S, float* __restrict__ mu) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const unsigned long long sum = cnt[2 * s], n = cnt[2 * s + 1];
+  mu[s] = n ? (float)((double)sum / (double)n) : 0.0f;
+}
+
+// Partial projection over SNP range blockIdx.y (PROJ_SPLIT SNPs):
+// part[(split * I + i) * D + d] = sum_{s in range, observed} (g_is - mu_s) R_sd.
+// Workgroup = 64 individuals x 256 features; thread = 16 individuals x 4 features.  LDS holds 32 SNPs of R
+// and of the centred dosages per stage.
 constexpr int PROJ_D = 256;
+constexpr int PROJ_SPLIT = 2048;  // SNPs per partial (multiple of 32)
 __global__ __launch_bounds__(256) void k_project(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int nb,
                                                  const float* __restrict__ mu, const float* __restrict__ R,
-                                                 float scale, float* __restrict__ F) {
+                                                 float* __restrict__ part) {
   __shared__ float Rs[32][PROJ_D];
   __shared__ float As[64][33];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -90,8 +112,9 @@ __global__ __launch_bounds__(256) void k_project(const uint32_t* __restrict__ gq
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
   const int S = 2 * nb;
-  for (int s0 = 0; s0 < S; s0 += 32) {
-    const int ns = min(32, S - s0);
+  const int sb = blockIdx.y * PROJ_SPLIT, se = min(S, sb + PROJ_SPLIT);
+  for (int s0 = sb; s0 < se; s0 += 32) {
+    const int ns = min(32, se - s0);
     __syncthreads();
     for (int e = threadIdx.x; e < 32 * PROJ_D; e += 256) {
       const int r = e / PROJ_D, d = e - r * PROJ_D;
@@ -121,14 +144,25 @@ __global__ __launch_bounds__(256) void k_project(const uint32_t* __restrict__ gq
       }
     }
   }
+  float* out = part + (int64_t)blockIdx.y * I * PROJ_D;
 #pragma unroll
   for (int a = 0; a < 16; ++a) {
     const int64_t i = i0 + 16 * ty + a;
     if (i < I) {
-      float4 o = {acc[a][0] * scale, acc[a][1] * scale, acc[a][2] * scale, acc[a][3] * scale};
-      *reinterpret_cast<float4*>(&F[i * PROJ_D + 4 * tx]) = o;
+      float4 o = {acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
+      *reinterpret_cast<float4*>(&out[i * PROJ_D + 4 * tx]) = o;
     }
   }
+}
+
+// F = scale * (sum of the partials in range order).
+__global__ void k_proj_sum(const float* __restrict__ part, int nsplit, int64_t n, float scale,
+                           float* __restrict__ F) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float v = 0.0f;
+  for (int sp = 0; sp < nsplit; ++sp) v += part[(int64_t)sp * n + t];
+  F[t] = scale * v;
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -560,22 +594,35 @@ int hk_mlp_adam(hk_mlp* m, float lr, float beta1, float beta2, float eps, int st
 int hk::project_features(const PassGeom& g, int n_blocks, uint64_t seed, float* F_host, hipStream_t s,
                              char* err, size_t errlen) {
   const int S = 2 * n_blocks;
-  float *R = nullptr, *mu = nullptr, *F = nullptr;
+  const int64_t nw = (g.I + 7) / 8;
+  const int nsplit = (S + PROJ_SPLIT - 1) / PROJ_SPLIT;
+  const int64_t nF = g.I * PROJ_D;
+  float *R = nullptr, *mu = nullptr, *F = nullptr, *part = nullptr;
+  unsigned long long* cnt = nullptr;
   hipError_t e = hipMalloc(&R, sizeof(float) * (int64_t)S * PROJ_D);
   if (e == hipSuccess) e = hipMalloc(&mu, sizeof(float) * S);
-  if (e == hipSuccess) e = hipMalloc(&F, sizeof(float) * g.I * PROJ_D);
+  if (e == hipSuccess) e = hipMalloc(&F, sizeof(float) * nF);
+  if (e == hipSuccess) e = hipMalloc(&part, sizeof(float) * nF * nsplit);
+  if (e == hipSuccess) e = hipMalloc(&cnt, sizeof(unsigned long long) * 2 * S);
+  if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 2 * S, s);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_gauss, dim3(nblk((int64_t)S * PROJ_D)), dim3(256), 0, s, R, (int64_t)S * PROJ_D, seed);
-    hipLaunchKernelGGL(k_snp_mean, dim3(nblk(S)), dim3(256), 0, s, g.gq, g.Ip, g.I, n_blocks, mu);
-    hipLaunchKernelGGL(k_project, dim3((unsigned)((g.I + 63) / 64)), dim3(256), 0, s, g.gq, g.Ip, g.I, n_blocks, mu,
-                       R, 1.0f / sqrtf((float)S), F);
+    hipLaunchKernelGGL(k_snp_counts, dim3(nblk(n_blocks), (unsigned)((nw + MU_CW - 1) / MU_CW)), dim3(256), 0, s, g.gp,
+                       g.Jp, n_blocks, nw, cnt);
+    hipLaunchKernelGGL(k_snp_mu, dim3(nblk(S)), dim3(256), 0, s, (const unsigned long long*)cnt, S, mu);
+    hipLaunchKernelGGL(k_project, dim3((unsigned)((g.I + 63) / 64), (unsigned)nsplit), dim3(256), 0, s, g.gq, g.Ip,
+                       g.I, n_blocks, (const float*)mu, (const float*)R, part);
+    hipLaunchKernelGGL(k_proj_sum, dim3(nblk(nF)), dim3(256), 0, s, (const float*)part, nsplit, nF,
+                       1.0f / sqrtf((float)S), F);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(F_host, F, sizeof(float) * g.I * PROJ_D, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(F_host, F, sizeof(float) * nF, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (R) (void)hipFree(R);
   if (mu) (void)hipFree(mu);
   if (F) (void)hipFree(F);
+  if (part) (void)hipFree(part);
+  if (cnt) (void)hipFree(cnt);
   if (e != hipSuccess) {
     snprintf(err, errlen, "feature projection: %s", hipGetErrorString(e));
     return HK_EHIP;

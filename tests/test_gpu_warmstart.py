@@ -17,7 +17,7 @@ from haplo_amd._native import HipError
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("I,J,nb", [(97, 53, 40), (97, 53, 53), (300, 20, 1), (513, 64, 64)])
+@pytest.mark.parametrize("I,J,nb", [(97, 53, 40), (97, 53, 53), (300, 20, 1), (513, 64, 64), (300, 2600, 2500)])
 def test_features_match_numpy(I, J, nb):
     print(f"features I={I} J={J} nb={nb}", flush=True)
     s, q, p = case(I, J, 3, miss=0.1)
