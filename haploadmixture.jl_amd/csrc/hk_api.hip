@@ -1062,6 +1062,16 @@ extern "C" int hk_copy_bench(hk_ctx* c, int64_t bytes, int reps, double* gbps) {
   return HK_OK;
 }
 
+extern "C" int hk_mlp_predict(hk_mlp* m, hk_ctx* c, int n_blocks, uint64_t seed, float* P) {
+  if (!c || !m || !P || n_blocks < 1 || n_blocks > c->J)
+    return fail(c, HK_EINVAL, "invalid arguments (1 <= n_blocks <= J)");
+  HIPC(c, hipSetDevice(c->dev));
+  HIPC(c, hipStreamSynchronize(c->s));  // genotype streams ready before the network's stream reads them
+  const int rc = mlp_predict_from(m, c->g, n_blocks, seed, P);
+  if (rc != HK_OK) return fail(c, rc, "hk_mlp_predict: %s", hk_mlp_last_error(m));
+  return HK_OK;
+}
+
 extern "C" int hk_features(hk_ctx* c, int n_blocks, uint64_t seed, float* F) {
   if (!c || !F || n_blocks < 1 || n_blocks > c->J) return fail(c, HK_EINVAL, "invalid arguments (1 <= n_blocks <= J)");
   return project_features(c->g, n_blocks, seed, F, c->s, c->err, sizeof c->err);

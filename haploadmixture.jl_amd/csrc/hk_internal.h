@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+typedef struct hk_mlp hk_mlp;
+
 namespace hk {
 
 enum { MODE_LL = 0, MODE_GRAD = 1, MODE_CURV = 2 };
@@ -83,6 +85,12 @@ hipError_t launch_copy16(const void* src, void* dst, int64_t n16, hipStream_t s)
 // Random-projection features (hk_mlp.hip): F_host (I x 256) from the first n_blocks blocks of gq.
 int project_features(const PassGeom& g, int n_blocks, uint64_t seed, float* F_host, hipStream_t s, char* err,
                      size_t errlen);
+// The same into a device buffer, with reusable scratch.
+struct ProjBuffers;
+hipError_t project_features_into(const PassGeom& g, int n_blocks, uint64_t seed, float* F_dev, ProjBuffers& b,
+                                 hipStream_t s);
+// Network prediction from a context's features computed on the device (hk_mlp_predict).
+int mlp_predict_from(hk_mlp* m, const PassGeom& g, int n_blocks, uint64_t seed, float* P);
 
 // SKFR marker pre-selection (hk_skfr.hip) on the genotype streams of g; returns 0, -1 (bad argument), -2 (HIP
 // error) or -3 (out of device memory) with err filled.

@@ -86,6 +86,11 @@ __global__ void k_snp_counts(const uint32_t* __restrict__ gp, int64_t Jp, int nb
   }
 }
 
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0ull;
+}
+
 __global__ void k_snp_mu(const unsigned long long* __restrict__ cnt, int S, float* __restrict__ mu) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
@@ -164,6 +169,26 @@ __global__ void k_proj_sum(const float* __restrict__ part, int nsplit, int64_t n
   for (int sp = 0; sp < nsplit; ++sp) v += part[(int64_t)sp * n + t];
   F[t] = scale * v;
 }
+
+// Projection scratch (Gaussian matrix, SNP counts / means, range partials), grown on demand; R is
+// regenerated only when (seed, S) change.
+struct ProjBuffers {
+  float *R = nullptr, *mu = nullptr, *part = nullptr;
+  unsigned long long* cnt = nullptr;
+  int64_t capS = 0, capPart = 0;
+  int S_gen = -1;
+  uint64_t seed_gen = 0;
+  ~ProjBuffers() { release(); }
+  void release() {
+    for (void* p : {(void*)R, (void*)mu, (void*)part, (void*)cnt})
+      if (p) (void)hipFree(p);
+    R = mu = part = nullptr;
+    cnt = nullptr;
+    capS = capPart = 0;
+    S_gen = -1;
+  }
+};
+
 
 // ---------------------------------------------------------------------------------------------------
 // MLP kernels
@@ -300,6 +325,10 @@ struct hk_mlp {
   int64_t cap = 0;
   double* Mx = nullptr;
   int* perm = nullptr;
+  // device-side prediction: features of a context, projection scratch
+  float* Fp = nullptr;
+  int64_t capFp = 0;
+  hk::ProjBuffers* proj = nullptr;
   char err[256] = {0};
 };
 
@@ -380,6 +409,8 @@ int hk_mlp_destroy(hk_mlp* m) {
                   m->Mx, m->perm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (m->Fp) (void)hipFree(m->Fp);
+  delete m->proj;
   if (m->s) (void)hipStreamDestroy(m->s);
   delete m;
   return HK_OK;
@@ -591,38 +622,82 @@ int hk_mlp_adam(hk_mlp* m, float lr, float beta1, float beta2, float eps, int st
 
 }  // extern "C"
 
-int hk::project_features(const PassGeom& g, int n_blocks, uint64_t seed, float* F_host, hipStream_t s,
-                             char* err, size_t errlen) {
+// P (I x dout, host) = softmax(MLP(features of the context's individuals)), the features computed on the
+// device from the context's genotype streams (first n_blocks blocks) and fed to the network in place.
+int hk::mlp_predict_from(hk_mlp* m, const PassGeom& g, int n_blocks, uint64_t seed, float* P) {
+  if (!m || !P || m->din != PROJ_D || n_blocks < 1 || n_blocks > g.J) return HK_EINVAL;
+  const int64_t n = g.I;
+  if (m->capFp < n) {
+    if (m->Fp) (void)hipFree(m->Fp);
+    m->Fp = nullptr;
+    m->capFp = 0;
+    MH(hipMalloc(&m->Fp, sizeof(float) * n * PROJ_D));
+    m->capFp = n;
+  }
+  if (!m->proj) m->proj = new ProjBuffers();
+  MH(project_features_into(g, n_blocks, seed, m->Fp, *m->proj, m->s));
+  int rc = ensure_cap(m, n);
+  if (rc == HK_OK) rc = mlp_forward_dev(m, m->Fp, n);
+  if (rc != HK_OK) return rc;
+  MH(hipMemcpyAsync(P, m->P, sizeof(float) * n * m->dout, hipMemcpyDeviceToHost, m->s));
+  MH(hipStreamSynchronize(m->s));
+  return HK_OK;
+}
+
+// F_dev (I x 256, device) = the features of the context's individuals over its first n_blocks blocks.
+hipError_t hk::project_features_into(const PassGeom& g, int n_blocks, uint64_t seed, float* F_dev, ProjBuffers& b,
+                                     hipStream_t s) {
   const int S = 2 * n_blocks;
   const int64_t nw = (g.I + 7) / 8;
   const int nsplit = (S + PROJ_SPLIT - 1) / PROJ_SPLIT;
   const int64_t nF = g.I * PROJ_D;
-  float *R = nullptr, *mu = nullptr, *F = nullptr, *part = nullptr;
-  unsigned long long* cnt = nullptr;
-  hipError_t e = hipMalloc(&R, sizeof(float) * (int64_t)S * PROJ_D);
-  if (e == hipSuccess) e = hipMalloc(&mu, sizeof(float) * S);
-  if (e == hipSuccess) e = hipMalloc(&F, sizeof(float) * nF);
-  if (e == hipSuccess) e = hipMalloc(&part, sizeof(float) * nF * nsplit);
-  if (e == hipSuccess) e = hipMalloc(&cnt, sizeof(unsigned long long) * 2 * S);
-  if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 2 * S, s);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_gauss, dim3(nblk((int64_t)S * PROJ_D)), dim3(256), 0, s, R, (int64_t)S * PROJ_D, seed);
-    hipLaunchKernelGGL(k_snp_counts, dim3(nblk(n_blocks), (unsigned)((nw + MU_CW - 1) / MU_CW)), dim3(256), 0, s, g.gp,
-                       g.Jp, n_blocks, nw, cnt);
-    hipLaunchKernelGGL(k_snp_mu, dim3(nblk(S)), dim3(256), 0, s, (const unsigned long long*)cnt, S, mu);
-    hipLaunchKernelGGL(k_project, dim3((unsigned)((g.I + 63) / 64), (unsigned)nsplit), dim3(256), 0, s, g.gq, g.Ip,
-                       g.I, n_blocks, (const float*)mu, (const float*)R, part);
-    hipLaunchKernelGGL(k_proj_sum, dim3(nblk(nF)), dim3(256), 0, s, (const float*)part, nsplit, nF,
-                       1.0f / sqrtf((float)S), F);
-    e = hipGetLastError();
+  hipError_t e = hipSuccess;
+  if (b.capS < S) {
+    for (void* p : {(void*)b.R, (void*)b.mu, (void*)b.cnt})
+      if (p) (void)hipFree(p);
+    b.R = b.mu = nullptr;
+    b.cnt = nullptr;
+    b.capS = 0;
+    b.S_gen = -1;
+    e = hipMalloc(&b.R, sizeof(float) * (int64_t)S * PROJ_D);
+    if (e == hipSuccess) e = hipMalloc(&b.mu, sizeof(float) * S);
+    if (e == hipSuccess) e = hipMalloc(&b.cnt, sizeof(unsigned long long) * 2 * S);
+    if (e != hipSuccess) return e;
+    b.capS = S;
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(F_host, F, sizeof(float) * nF, hipMemcpyDeviceToHost, s);
+  if (b.capPart < nF * nsplit) {
+    if (b.part) (void)hipFree(b.part);
+    b.part = nullptr;
+    b.capPart = 0;
+    e = hipMalloc(&b.part, sizeof(float) * nF * nsplit);
+    if (e != hipSuccess) return e;
+    b.capPart = nF * nsplit;
+  }
+  if (b.S_gen != S || b.seed_gen != seed) {
+    hipLaunchKernelGGL(k_gauss, dim3(nblk((int64_t)S * PROJ_D)), dim3(256), 0, s, b.R, (int64_t)S * PROJ_D, seed);
+    b.S_gen = S;
+    b.seed_gen = seed;
+  }
+  hipLaunchKernelGGL(k_fill_u64, dim3(nblk(2 * (int64_t)S)), dim3(256), 0, s, b.cnt, 2 * (int64_t)S);
+  hipLaunchKernelGGL(k_snp_counts, dim3(nblk(n_blocks), (unsigned)((nw + MU_CW - 1) / MU_CW)), dim3(256), 0, s, g.gp,
+                     g.Jp, n_blocks, nw, b.cnt);
+  hipLaunchKernelGGL(k_snp_mu, dim3(nblk(S)), dim3(256), 0, s, (const unsigned long long*)b.cnt, S, b.mu);
+  hipLaunchKernelGGL(k_project, dim3((unsigned)((g.I + 63) / 64), (unsigned)nsplit), dim3(256), 0, s, g.gq, g.Ip, g.I,
+                     n_blocks, (const float*)b.mu, (const float*)b.R, b.part);
+  hipLaunchKernelGGL(k_proj_sum, dim3(nblk(nF)), dim3(256), 0, s, (const float*)b.part, nsplit, nF,
+                     1.0f / sqrtf((float)S), F_dev);
+  return hipGetLastError();
+}
+
+int hk::project_features(const PassGeom& g, int n_blocks, uint64_t seed, float* F_host, hipStream_t s,
+                         char* err, size_t errlen) {
+  ProjBuffers b;
+  float* F = nullptr;
+  hipError_t e = hipMalloc(&F, sizeof(float) * g.I * PROJ_D);
+  if (e == hipSuccess) e = project_features_into(g, n_blocks, seed, F, b, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(F_host, F, sizeof(float) * g.I * PROJ_D, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (R) (void)hipFree(R);
-  if (mu) (void)hipFree(mu);
   if (F) (void)hipFree(F);
-  if (part) (void)hipFree(part);
-  if (cnt) (void)hipFree(cnt);
   if (e != hipSuccess) {
     snprintf(err, errlen, "feature projection: %s", hipGetErrorString(e));
     return HK_EHIP;

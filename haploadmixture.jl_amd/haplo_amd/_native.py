@@ -122,6 +122,7 @@ def hip():
     L.hk_mlp_set_params.argtypes = [_vp, _vp]
     L.hk_mlp_load_data.argtypes = [_vp, _vp, _vp, _i64]
     L.hk_mlp_forward.argtypes = [_vp, _vp, _i64, _vp]
+    L.hk_mlp_predict.argtypes = [_vp, _vp, _int, ctypes.c_uint64, _vp]
     L.hk_mlp_grad.argtypes = [_vp, _i64, _i64, _int, ctypes.POINTER(ctypes.c_float), _vp]
     L.hk_mlp_adam.argtypes = [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _int]
     _hip = L

@@ -185,9 +185,18 @@ def train(ts: TrainingSet, K: int, epochs=10, batch=500, lr=1e-3, seed=3, log=No
     return mlp, hist
 
 
+def predict_probs(engine: HipAdmixData, mlp: MLP, n_blocks: int | None = None, seed: int = FEATURE_SEED) -> np.ndarray:
+    """softmax(MLP(features)) (I x K) with the features computed on the device and fed to the network in
+    place (hk_mlp_predict); equal to ``mlp.forward(features(engine))``."""
+    nb = min(engine.J, PANEL_BLOCKS) if n_blocks is None else int(n_blocks)
+    P = np.zeros((engine.I, mlp.K), dtype=np.float32)
+    engine._check(engine._L.hk_mlp_predict(mlp._h, engine._h, nb, seed, ptr(P)))
+    return P
+
+
 def predict_q(engine: HipAdmixData, mlp: MLP, floor=1e-5) -> np.ndarray:
     """Q0 (K x I) for the engine's individuals: softmax(MLP(features)), floored and renormalised."""
-    P = mlp.forward(features(engine)).astype(np.float64)
+    P = predict_probs(engine, mlp).astype(np.float64)
     P = np.maximum(P, floor)
     P /= P.sum(axis=1, keepdims=True)
     return np.asfortranarray(P.T)

@@ -180,6 +180,10 @@ int hk_mlp_set_params(hk_mlp* m, const float* w);
 int hk_mlp_load_data(hk_mlp* m, const float* X, const float* T, int64_t n);
 /* P (n x dout) = softmax(MLP(X)) for host X. */
 int hk_mlp_forward(hk_mlp* m, const float* X, int64_t n, float* P);
+/* P (I x dout, host) = softmax(MLP(features of ctx)): the hk_features projection of ctx's first n_blocks
+ * blocks (seed), computed on the device and fed to the network without a host round trip; scratch is kept
+ * with the network for the next call (din must be 256).  Errors are reported on ctx. */
+int hk_mlp_predict(hk_mlp* m, hk_ctx* ctx, int n_blocks, uint64_t seed, float* P);
 /* Forward + backward on resident rows [row0, row0+n): gradients stay on the device (hk_mlp_get_params);
  * match != 0 relabels the target columns by the cross-entropy-minimising permutation. */
 int hk_mlp_grad(hk_mlp* m, int64_t row0, int64_t n, int match, float* loss, int* perm);

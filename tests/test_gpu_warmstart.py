@@ -28,6 +28,21 @@ def test_features_match_numpy(I, J, nb):
     assert np.max(np.abs(F - ref)) <= 2e-5 * np.max(np.abs(ref))
 
 
+def test_device_prediction_matches_host_path():
+    """hk_mlp_predict (features on the device, fed to the network in place) equals features() + forward(),
+    on two calls in a row with different sizes (the scratch is reused)."""
+    m, _ = _random_mlp(256, 64, 3, seed=5)
+    for I, J, nb in ((400, 300, 250), (150, 80, 80)):
+        print(f"predict I={I} J={J} nb={nb}", flush=True)
+        s, q, p = case(I, J, 3, miss=0.05)
+        with HipAdmixData(s.bed, I, J, 3, precision="mixed") as e:
+            ref = m.forward(ws.features(e, n_blocks=nb))
+            got = ws.predict_probs(e, m, n_blocks=nb)
+        assert got.shape == (I, 3)
+        assert np.max(np.abs(got - ref)) <= 1e-6
+    m.close()
+
+
 def test_features_reject_bad_block_count():
     s, q, p = case(40, 10, 2)
     with HipAdmixData(s.bed, 40, 10, 2) as e:
