@@ -98,65 +98,91 @@ __global__ void k_snp_mu(const unsigned long long* __restrict__ cnt, int S, floa
   mu[s] = n ? (float)((double)sum / (double)n) : 0.0f;
 }
 
-// Partial projection over SNP range blockIdx.y (PROJ_SPLIT SNPs):
-// part[(split * I + i) * D + d] = sum_{s in range, observed} (g_is - mu_s) R_sd.
-// Workgroup = 64 individuals x 256 features; thread = 16 individuals x 4 features.  LDS holds 32 SNPs of R
-// and of the centred dosages per stage.
+// Partial projection over SNP range blockIdx.y (PROJ_SPLIT SNPs) on the matrix cores:
+// part[(split * I + i) * D + d] = sum_{s in range} (x_is R_sd - o_is mu_s R_sd), x = allele-2 dosage (0 if
+// missing), o = 1 if observed: F = X R - O (mu o R).  X and O are exact in BF16; R and W = -(mu o R) enter
+// as BF16 pairs (hi = bf16(v), lo = bf16(v - hi), ~16 mantissa bits), so each 32-SNP slice is four
+// v_mfma_f32_16x16x32_bf16 per 16 x 16 tile (X R_hi, X R_lo, O W_hi, O W_lo), accumulated in FP32.
+// Workgroup = 64 individuals (one 16-row tile per wave) x 256 features (16 tiles per wave); the slice's
+// four B operands (64 KB) are staged in LDS as [op][k group][feature][8 SNPs].
 constexpr int PROJ_D = 256;
 constexpr int PROJ_SPLIT = 2048;  // SNPs per partial (multiple of 32)
-__global__ __launch_bounds__(256) void k_project(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int nb,
-                                                 const float* __restrict__ mu, const float* __restrict__ R,
-                                                 float* __restrict__ part) {
-  __shared__ float Rs[32][PROJ_D];
-  __shared__ float As[64][33];
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int64_t i0 = (int64_t)blockIdx.x * 64;
-  float acc[16][4];
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+// Bg[((slice * 4 + op) * 4 + g) * D + d] (8 BF16: SNPs 32 slice + 8 g + e), op: R_hi, R_lo, W_hi, W_lo;
+// SNPs >= S are zero.  Thread per (slice, g, d).
+__global__ void k_proj_prep(const float* __restrict__ R, const float* __restrict__ mu, int S, int nslice,
+                            u16x8* __restrict__ Bg) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nslice * 4 * PROJ_D) return;
+  const int d = (int)(t % PROJ_D);
+  const int64_t sg = t / PROJ_D;  // slice * 4 + g
+  const int64_t slice = sg >> 2;
+  const int g = (int)(sg & 3);
+  u16x8 v[4];
 #pragma unroll
-  for (int a = 0; a < 16; ++a)
+  for (int e = 0; e < 8; ++e) {
+    const int64_t sn = slice * 32 + 8 * g + e;
+    const float r = sn < S ? R[sn * PROJ_D + d] : 0.0f;
+    const float w = sn < S ? -(mu[sn] * r) : 0.0f;
+    const uint16_t rh = f32_to_bf16(r), wh = f32_to_bf16(w);
+    v[0][e] = rh;
+    v[1][e] = f32_to_bf16(r - bf16_to_f32(rh));
+    v[2][e] = wh;
+    v[3][e] = f32_to_bf16(w - bf16_to_f32(wh));
+  }
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
-  const int S = 2 * nb;
-  const int sb = blockIdx.y * PROJ_SPLIT, se = min(S, sb + PROJ_SPLIT);
-  for (int s0 = sb; s0 < se; s0 += 32) {
-    const int ns = min(32, se - s0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < 32 * PROJ_D; e += 256) {
-      const int r = e / PROJ_D, d = e - r * PROJ_D;
-      Rs[r][d] = r < ns ? R[(int64_t)(s0 + r) * PROJ_D + d] : 0.0f;
+  for (int op = 0; op < 4; ++op) Bg[((slice * 4 + op) * 4 + g) * PROJ_D + d] = v[op];
+}
+
+__global__ __launch_bounds__(256) void k_project(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J8,
+                                                 int S, const u16x8* __restrict__ Bg, float* __restrict__ part) {
+  __shared__ u16x8 Bs[4 * 4 * PROJ_D];  // 64 KB
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int64_t ia = (int64_t)blockIdx.x * 64 + 16 * wv + m;  // A row of this lane
+  const int nslice = (S + 31) / 32;
+  const int sl0 = blockIdx.y * (PROJ_SPLIT / 32), sl1 = min(nslice, sl0 + PROJ_SPLIT / 32);
+  typedef float v4f_t __attribute__((ext_vector_type(4)));
+  v4f_t acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc[t] = (v4f_t){0.f, 0.f, 0.f, 0.f};
+  for (int sl = sl0; sl < sl1; ++sl) {
+    // genotype codes of this lane's 8 SNPs (requested before the barrier)
+    const int64_t wi = 2 * (int64_t)sl + (g >> 1);
+    const uint32_t w = (ia < I && wi < J8) ? gq[wi * Ip + ia] : 0x55555555u;
+    const uint32_t h16 = w >> (16 * (g & 1));
+    __syncthreads();  // the previous slice has been consumed
+    const u16x8* src = Bg + (int64_t)sl * 4 * 4 * PROJ_D;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) Bs[tid + 256 * x] = src[tid + 256 * x];
+    u16x8 ax, ao;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const unsigned c = (h16 >> (2 * e)) & 3u;  // 0b00 -> 0, 0b10 -> 1, 0b11 -> 2, 0b01 missing
+      ax[e] = c == 3u ? (unsigned short)0x4000 : (c == 2u ? (unsigned short)0x3F80 : (unsigned short)0);
+      ao[e] = c == 1u ? (unsigned short)0 : (unsigned short)0x3F80;
     }
-    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-      const int ii = e >> 5, r = e & 31;
-      const int64_t i = i0 + ii;
-      float v = 0.0f;
-      if (r < ns && i < I) {
-        const int s = s0 + r, j = s >> 1;
-        const unsigned c = (gq[(int64_t)(j >> 3) * Ip + i] >> (4 * (j & 7) + 2 * (s & 1))) & 3u;
-        v = c == 1u ? 0.0f : dosage(c) - mu[s];
-      }
-      As[ii][r] = v;
-    }
+    const bf16x8 aX = __builtin_bit_cast(bf16x8, ax), aO = __builtin_bit_cast(bf16x8, ao);
     __syncthreads();
-    for (int r = 0; r < ns; ++r) {
-      const float4 rv = *reinterpret_cast<const float4*>(&Rs[r][4 * tx]);
 #pragma unroll
-      for (int a = 0; a < 16; ++a) {
-        const float x = As[16 * ty + a][r];
-        acc[a][0] = __builtin_fmaf(x, rv.x, acc[a][0]);
-        acc[a][1] = __builtin_fmaf(x, rv.y, acc[a][1]);
-        acc[a][2] = __builtin_fmaf(x, rv.z, acc[a][2]);
-        acc[a][3] = __builtin_fmaf(x, rv.w, acc[a][3]);
+    for (int op = 0; op < 4; ++op) {
+      const bf16x8 a = op < 2 ? aX : aO;
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const bf16x8 b = __builtin_bit_cast(bf16x8, Bs[(op * 4 + g) * PROJ_D + 16 * t + m]);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
       }
     }
   }
   float* out = part + (int64_t)blockIdx.y * I * PROJ_D;
 #pragma unroll
-  for (int a = 0; a < 16; ++a) {
-    const int64_t i = i0 + 16 * ty + a;
-    if (i < I) {
-      float4 o = {acc[a][0], acc[a][1], acc[a][2], acc[a][3]};
-      *reinterpret_cast<float4*>(&out[i * PROJ_D + 4 * tx]) = o;
-    }
+  for (int r = 0; r < 4; ++r) {
+    const int64_t i = (int64_t)blockIdx.x * 64 + 16 * wv + 4 * g + r;  // D row of acc[t][r]
+    if (i < I)
+#pragma unroll
+      for (int t = 0; t < 16; ++t) out[i * PROJ_D + 16 * t + m] = acc[t][r];
   }
 }
 
@@ -175,15 +201,17 @@ __global__ void k_proj_sum(const float* __restrict__ part, int nsplit, int64_t n
 struct ProjBuffers {
   float *R = nullptr, *mu = nullptr, *part = nullptr;
   unsigned long long* cnt = nullptr;
+  u16x8* Bg = nullptr;  // BF16 operand slices
   int64_t capS = 0, capPart = 0;
   int S_gen = -1;
   uint64_t seed_gen = 0;
   ~ProjBuffers() { release(); }
   void release() {
-    for (void* p : {(void*)R, (void*)mu, (void*)part, (void*)cnt})
+    for (void* p : {(void*)R, (void*)mu, (void*)part, (void*)cnt, (void*)Bg})
       if (p) (void)hipFree(p);
     R = mu = part = nullptr;
     cnt = nullptr;
+    Bg = nullptr;
     capS = capPart = 0;
     S_gen = -1;
   }
@@ -652,16 +680,19 @@ hipError_t hk::project_features_into(const PassGeom& g, int n_blocks, uint64_t s
   const int nsplit = (S + PROJ_SPLIT - 1) / PROJ_SPLIT;
   const int64_t nF = g.I * PROJ_D;
   hipError_t e = hipSuccess;
+  const int nslice = (S + 31) / 32;
   if (b.capS < S) {
-    for (void* p : {(void*)b.R, (void*)b.mu, (void*)b.cnt})
+    for (void* p : {(void*)b.R, (void*)b.mu, (void*)b.cnt, (void*)b.Bg})
       if (p) (void)hipFree(p);
     b.R = b.mu = nullptr;
     b.cnt = nullptr;
+    b.Bg = nullptr;
     b.capS = 0;
     b.S_gen = -1;
     e = hipMalloc(&b.R, sizeof(float) * (int64_t)S * PROJ_D);
     if (e == hipSuccess) e = hipMalloc(&b.mu, sizeof(float) * S);
     if (e == hipSuccess) e = hipMalloc(&b.cnt, sizeof(unsigned long long) * 2 * S);
+    if (e == hipSuccess) e = hipMalloc(&b.Bg, sizeof(u16x8) * (int64_t)nslice * 4 * 4 * PROJ_D);
     if (e != hipSuccess) return e;
     b.capS = S;
   }
@@ -682,8 +713,10 @@ hipError_t hk::project_features_into(const PassGeom& g, int n_blocks, uint64_t s
   hipLaunchKernelGGL(k_snp_counts, dim3(nblk(n_blocks), (unsigned)((nw + MU_CW - 1) / MU_CW)), dim3(256), 0, s, g.gp,
                      g.Jp, n_blocks, nw, b.cnt);
   hipLaunchKernelGGL(k_snp_mu, dim3(nblk(S)), dim3(256), 0, s, (const unsigned long long*)b.cnt, S, b.mu);
+  hipLaunchKernelGGL(k_proj_prep, dim3(nblk((int64_t)nslice * 4 * PROJ_D)), dim3(256), 0, s, (const float*)b.R,
+                     (const float*)b.mu, S, nslice, b.Bg);
   hipLaunchKernelGGL(k_project, dim3((unsigned)((g.I + 63) / 64), (unsigned)nsplit), dim3(256), 0, s, g.gq, g.Ip, g.I,
-                     n_blocks, (const float*)b.mu, (const float*)b.R, b.part);
+                     g.J8, S, (const u16x8*)b.Bg, b.part);
   hipLaunchKernelGGL(k_proj_sum, dim3(nblk(nF)), dim3(256), 0, s, (const float*)b.part, nsplit, nF,
                      1.0f / sqrtf((float)S), F_dev);
   return hipGetLastError();
