@@ -752,8 +752,10 @@ static int read_ll(hk_ctx* c, int n) {
   return HK_OK;
 }
 
-// init_em! pass (algorithms_outer.jl:4-16): EM update into x, log-likelihood -> d_ll[3], projections.
-static int enqueue_em_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm) {
+// init_em! pass (algorithms_outer.jl:4-16): EM update into x, log-likelihood -> d_ll[3] (want_ll), projections.
+// init_em! uses the log-likelihood of an EM iteration only for its progress line and, for the last one, as
+// d.ll_new; the others are skipped unless verbose.
+static int enqueue_em_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm, bool want_ll = true) {
   const int K = c->K;
   double* x = c->xb[0];
   double* xn = c->xb[1];
@@ -769,7 +771,7 @@ static int enqueue_em_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm) {
   HIPC(c, launch_em_p(K, c->J, c->res_p, K, pof(c, x), pof(c, xn), o->eps, c->s));
   if (tm) tm->end();
   HIPC(c, launch_copy_f64(N, xn, x, c->s));  // d.p .= d.p_next; d.q .= d.q_next
-  CHK(run_qpass(c, MODE_LL, qof(c, x), pof(c, x), tm, T_LL, c->d_ll + 3));
+  if (want_ll) CHK(run_qpass(c, MODE_LL, qof(c, x), pof(c, x), tm, T_LL, c->d_ll + 3));
   HIPC(c, launch_project_p(K, c->J, pof(c, x), o->eps, c->s));
   HIPC(c, launch_project_q(K, c->I, qof(c, x), o->eps, c->s));
   return HK_OK;
@@ -811,7 +813,9 @@ static int fit_em(FitRun& f, double ll_start) {
   const hk_fit_opts* o = f.o;
   double ll = NAN;
   for (int it = 0; it < o->em_iters; ++it) {
-    CHK(enqueue_em_iteration(c, o, f.tm.on ? &f.tm : nullptr));
+    const bool want_ll = o->verbose || it + 1 == o->em_iters;
+    CHK(enqueue_em_iteration(c, o, f.tm.on ? &f.tm : nullptr, want_ll));
+    if (!want_ll) continue;
     CHK(read_ll(c, 4));
     ll = c->h_ll[3];
     if (o->verbose) fprintf(stderr, "EM iter %d, ll: %.17g\n", it + 1, ll);
@@ -927,9 +931,10 @@ extern "C" int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const
   std::vector<double> ll_em(n, NAN);
   for (int it = 0; it < max_em; ++it) {
     for (int b = 0; b < n; ++b)
-      if (it < opts[b].em_iters) CHK(enqueue_em_iteration(ctxs[b], &opts[b], nullptr));
+      if (it < opts[b].em_iters)
+        CHK(enqueue_em_iteration(ctxs[b], &opts[b], nullptr, opts[b].verbose || it + 1 == opts[b].em_iters));
     for (int b = 0; b < n; ++b)
-      if (it < opts[b].em_iters) {
+      if (it + 1 == opts[b].em_iters) {
         CHK(read_ll(ctxs[b], 4));
         ll_em[b] = ctxs[b]->h_ll[3];
       }
