@@ -64,38 +64,45 @@ __global__ void k_repack_q(const uint8_t* __restrict__ raw, int64_t bps, int64_t
   gq[idx] = w;
 }
 
-__global__ void k_repack_p(const uint8_t* __restrict__ raw, int64_t bps, int64_t I, int64_t J, int64_t Jp,
-                           int64_t I8, uint32_t* __restrict__ gp) {
+// Block-major stream from the individual-major one: an 8 x 8 nibble transpose per thread.  Thread (iw, jw)
+// reads gq[jw][8 iw .. 8 iw + 7] (8 individuals x 8 blocks) and writes gp[iw][8 jw .. 8 jw + 7]; consecutive
+// threads take consecutive jw, so the 32-byte writes are contiguous.  Padding nibbles carry over.
+__global__ void k_repack_p(const uint32_t* __restrict__ gq, int64_t Ip, int64_t J8, int64_t Jp, int64_t I8,
+                           uint32_t* __restrict__ gp) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= I8 * Jp) return;
-  const int64_t iw = idx / Jp, j = idx - iw * Jp;
-  uint32_t w = 0;
+  if (idx >= I8 * J8) return;
+  const int64_t iw = idx / J8, jw = idx - iw * J8;
+  uint32_t in[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) in[t] = gq[jw * Ip + 8 * iw + t];
+#pragma unroll
   for (int n = 0; n < 8; ++n) {
-    const int64_t i = iw * 8 + n;
-    uint32_t nib = HK_NIB_DM;
-    if (i < I && j < J) {
-      const unsigned sh = (unsigned)(i & 3) * 2u;
-      const unsigned c1 = (raw[(2 * j) * bps + (i >> 2)] >> sh) & 3u;
-      const unsigned c2 = (raw[(2 * j + 1) * bps + (i >> 2)] >> sh) & 3u;
-      nib = c1 | (c2 << 2);
-    }
-    w |= nib << (4 * n);
+    uint32_t w = 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) w |= ((in[t] >> (4 * n)) & 15u) << (4 * t);
+    gp[iw * Jp + 8 * jw + n] = w;
   }
-  gp[idx] = w;
 }
 
-// count_double_missing! (src/utils.jl:1-18) over the first J blocks.
+// count_double_missing! (src/utils.jl:1-18) over the first J blocks: per individual, the double-missing
+// nibbles (0b0101) of a chunk of DM_CW words, counted with bit operations (x = w ^ 0x55555555 has a zero
+// nibble exactly there), added with integer atomics (exact in any order).  dm must start at zero.
+constexpr int DM_CW = 64;
 __global__ void k_count_dm(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J, int64_t J8,
-                           int64_t* __restrict__ dm) {
+                           unsigned long long* __restrict__ dm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= I) return;
-  int64_t n = 0;
-  for (int64_t jw = 0; jw < J8; ++jw) {
-    const uint32_t w = gq[jw * Ip + i];
-    for (int t = 0; t < 8; ++t)
-      if (jw * 8 + t < J && ((w >> (4 * t)) & 15u) == HK_NIB_DM) ++n;
+  const int64_t w0 = (int64_t)blockIdx.y * DM_CW, w1 = w0 + DM_CW < J8 ? w0 + DM_CW : J8;
+  unsigned n = 0;
+  for (int64_t jw = w0; jw < w1; ++jw) {
+    uint32_t x = gq[jw * Ip + i] ^ 0x55555555u;
+    x |= x >> 1;
+    x |= x >> 2;
+    uint32_t valid = 0x11111111u;  // nibbles of blocks < J
+    if (jw * 8 + 8 > J) valid = J > jw * 8 ? (0x11111111u >> (4 * (int)(jw * 8 + 8 - J))) : 0u;
+    n += (unsigned)__builtin_popcount(~x & valid);
   }
-  dm[i] = n;
+  if (n) atomicAdd(&dm[i], (unsigned long long)n);
 }
 
 // Decoded genotype readback (bit-exact decode check): g[s * I + i] = BED code of SNP s.
@@ -1140,16 +1147,24 @@ void stage_sizes(int K, int64_t Ip, int64_t Jp, int64_t* nq, int64_t* np) {
 int nacc(int K, int mode) { return mode == MODE_LL ? 1 : (mode == MODE_GRAD ? K : K + K * (K + 1) / 2); }
 
 hipError_t launch_repack(const uint8_t* raw, int64_t bps, const PassGeom& g, hipStream_t s) {
-  int64_t nq = g.J8 * g.Ip, np = g.I8 * g.Jp;
+  int64_t nq = g.J8 * g.Ip, nt = g.I8 * g.J8;
   hipLaunchKernelGGL(k_repack_q, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, raw, bps, g.I, g.J, g.Ip,
                      g.J8, (uint32_t*)g.gq);
-  hipLaunchKernelGGL(k_repack_p, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, raw, bps, g.I, g.J, g.Jp,
-                     g.I8, (uint32_t*)g.gp);
+  hipLaunchKernelGGL(k_repack_p, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, g.gq, g.Ip, g.J8, g.Jp, g.I8,
+                     (uint32_t*)g.gp);
   return hipGetLastError();
 }
 
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0ull;
+}
+
 hipError_t launch_count_dm(const PassGeom& g, int64_t* dm, hipStream_t s) {
-  hipLaunchKernelGGL(k_count_dm, dim3((unsigned)((g.I + 255) / 256)), dim3(256), 0, s, g.gq, g.Ip, g.I, g.J, g.J8, dm);
+  unsigned long long* d = (unsigned long long*)dm;
+  hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((g.I + 255) / 256)), dim3(256), 0, s, d, g.I);
+  hipLaunchKernelGGL(k_count_dm, dim3((unsigned)((g.I + 255) / 256), (unsigned)((g.J8 + DM_CW - 1) / DM_CW)), dim3(256), 0,
+                     s, g.gq, g.Ip, g.I, g.J, g.J8, d);
   return hipGetLastError();
 }
 
