@@ -939,16 +939,21 @@ __global__ void k_xtz_q(int K, int64_t I, const double* __restrict__ q, double* 
 }
 
 // Curvature pass, P side: Xtz_p[:, jh] = XtX_p[:, :, jh] p[:, jh] into slots 0..K-1 of res[jh][NT].
-__global__ void k_finish_p(int K, int64_t J, const double* __restrict__ p, double* __restrict__ res) {
+template <int K>
+__global__ void k_finish_p(int64_t J, const double* __restrict__ p, double* __restrict__ res) {
   const int64_t jh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (jh >= 4 * J) return;
-  const int NT = K + K * (K + 1) / 2;
+  constexpr int NT = K + K * (K + 1) / 2;
   double* r = res + jh * NT;
-  double a[HK_MAXK * (HK_MAXK + 1) / 2], pv[HK_MAXK];
+  double a[K * (K + 1) / 2], pv[K];
+#pragma unroll
   for (int t = 0; t < K * (K + 1) / 2; ++t) a[t] = r[K + t];
+#pragma unroll
   for (int k = 0; k < K; ++k) pv[k] = p[k + (int64_t)K * jh];
+#pragma unroll
   for (int k = 0; k < K; ++k) {
     double s = 0.0;
+#pragma unroll
     for (int k2 = 0; k2 < K; ++k2) {
       const int hi = k > k2 ? k : k2, lo = k > k2 ? k2 : k;
       s += a[hi * (hi + 1) / 2 + lo] * pv[k2];
@@ -1191,7 +1196,17 @@ hipError_t launch_reduce_q_curv(const double* part, int nchunk, int K, const Pas
 }
 
 hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* res, hipStream_t s) {
-  hipLaunchKernelGGL(k_finish_p, dim3((unsigned)((4 * g.J + 255) / 256)), dim3(256), 0, s, K, g.J, p, res);
+  const dim3 grid((unsigned)((4 * g.J + 255) / 256));
+  switch (K) {
+    case 1: hipLaunchKernelGGL(k_finish_p<1>, grid, dim3(256), 0, s, g.J, p, res); break;
+    case 2: hipLaunchKernelGGL(k_finish_p<2>, grid, dim3(256), 0, s, g.J, p, res); break;
+    case 3: hipLaunchKernelGGL(k_finish_p<3>, grid, dim3(256), 0, s, g.J, p, res); break;
+    case 4: hipLaunchKernelGGL(k_finish_p<4>, grid, dim3(256), 0, s, g.J, p, res); break;
+    case 5: hipLaunchKernelGGL(k_finish_p<5>, grid, dim3(256), 0, s, g.J, p, res); break;
+    case 6: hipLaunchKernelGGL(k_finish_p<6>, grid, dim3(256), 0, s, g.J, p, res); break;
+    case 7: hipLaunchKernelGGL(k_finish_p<7>, grid, dim3(256), 0, s, g.J, p, res); break;
+    default: hipLaunchKernelGGL(k_finish_p<8>, grid, dim3(256), 0, s, g.J, p, res); break;
+  }
   return hipGetLastError();
 }
 

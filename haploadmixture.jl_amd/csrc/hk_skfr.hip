@@ -403,11 +403,11 @@ __global__ void k_sk_chunk_a(const int* __restrict__ blocks, const int* __restri
       v[2 * u + 1] = th[2 * b + 1];
     }
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      if (t + u >= t1) break;
-      a = a + v[2 * u] * v[2 * u];
-      a = a + v[2 * u + 1] * v[2 * u + 1];
-    }
+    for (int u = 0; u < 16; ++u)
+      if (t + u < t1) {
+        a = a + v[2 * u] * v[2 * u];
+        a = a + v[2 * u + 1] * v[2 * u + 1];
+      }
   }
   ca[x] = a;
 }

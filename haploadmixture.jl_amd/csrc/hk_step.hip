@@ -856,6 +856,7 @@ __global__ __launch_bounds__(256) void k_zal_dots(int64_t N, const int* __restri
       acc[3 * Q2 + ZAL_QMAX + a] = __builtin_fma(va[a], rr, acc[3 * Q2 + ZAL_QMAX + a]);
     }
   }
+#pragma unroll
   for (int d = 0; d < ZAL_NDOT; ++d) {
     const double s = block_sum<256>(acc[d], sh);
     if (threadIdx.x == 0) part[blockIdx.x * ZAL_NDOT + d] = s;
