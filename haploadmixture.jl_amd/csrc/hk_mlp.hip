@@ -653,7 +653,10 @@ int hk_mlp_adam(hk_mlp* m, float lr, float beta1, float beta2, float eps, int st
 // P (I x dout, host) = softmax(MLP(features of the context's individuals)), the features computed on the
 // device from the context's genotype streams (first n_blocks blocks) and fed to the network in place.
 int hk::mlp_predict_from(hk_mlp* m, const PassGeom& g, int n_blocks, uint64_t seed, float* P) {
-  if (!m || !P || m->din != PROJ_D || n_blocks < 1 || n_blocks > g.J) return HK_EINVAL;
+  if (!m || !P) return HK_EINVAL;
+  if (m->din != PROJ_D) return mlp_fail(m, HK_EINVAL, "network input width %d, features have %d", m->din, PROJ_D);
+  if (n_blocks < 1 || n_blocks > g.J)
+    return mlp_fail(m, HK_EINVAL, "n_blocks %d outside 1..J=%lld", n_blocks, (long long)g.J);
   const int64_t n = g.I;
   if (m->capFp < n) {
     if (m->Fp) (void)hipFree(m->Fp);

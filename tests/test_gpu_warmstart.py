@@ -43,6 +43,19 @@ def test_device_prediction_matches_host_path():
     m.close()
 
 
+def test_device_prediction_rejects_bad_inputs():
+    s, q, p = case(40, 10, 2)
+    m, _ = _random_mlp(37, 16, 2, seed=3)  # input width is not the feature width
+    m256, _ = _random_mlp(256, 16, 2, seed=3)
+    with HipAdmixData(s.bed, 40, 10, 2) as e:
+        with pytest.raises(HipError, match="input width"):
+            ws.predict_probs(e, m, n_blocks=10)
+        with pytest.raises(HipError):
+            ws.predict_probs(e, m256, n_blocks=11)
+    m.close()
+    m256.close()
+
+
 def test_features_reject_bad_block_count():
     s, q, p = case(40, 10, 2)
     with HipAdmixData(s.bed, 40, 10, 2) as e:
