@@ -45,6 +45,9 @@ CASES = [
     (500, 2100, 6, 300, "local", 1, 0.05, 9, None),
     (400, 3000, 4, 700, "global", 1, 0.02, 13, None),   # three reassignment partials
     (2500, 300, 3, 30, "global", 2, 0.03, 14, None),     # three center-count partials
+    (9, 5, 3, 2, "global", 1, 0.1, 15, None),            # fewer individuals / blocks than a genotype word
+    (16, 8, 2, 8, "local", 2, 0.3, 16, [3, 11]),         # heavy missingness, given starts, then a k-means++ try
+    (7, 3, 7, 3, "global", 1, 0.0, 17, None),            # k = I
 ]
 
 
