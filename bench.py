@@ -229,8 +229,15 @@ def skfr_bench(a, w):
     cen_ms = pr["centers_ms"] / max(pr["centers_launches"], 1)
     cen_bytes = 0.5 * I * J + 8.0 * (2 * K * J + J)
     dist_ms = pr["dist_ms"] / max(pr["dist_launches"], 1)
-    roof = {"kernel": "center update (k_sk_member + k_sk_counts + k_sk_centers)", "bound": "hbm", "achieved": round(cen_bytes / (cen_ms * 1e-3) / 1e9, 1),
-            "peak": 8000.0, "unit": "GB/s", "frac": None, "traffic": None,
+    # measured HBM bytes per center update (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled) from profiles/
+    cen_name = "center update (k_sk_member + k_sk_counts + k_sk_centers)"
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath)).get(a.workload, {}).get(cen_name)
+        traffic = t["hbm_bytes_per_launch"] if t else None
+    roof = {"kernel": cen_name, "bound": "hbm", "achieved": round(cen_bytes / (cen_ms * 1e-3) / 1e9, 1),
+            "peak": 8000.0, "unit": "GB/s", "frac": None, "traffic": traffic,
             "avg_launch_ms": round(cen_ms, 4), "algorithmic_bytes_per_launch": cen_bytes}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
     # bounded CPU sample: the NumPy restatement and the GPU on the same sub-problem
