@@ -89,7 +89,7 @@ def cpu_baseline(bed, I, J, K, q0, p0, gpu_iters, em_iters, j_sample):
 
 
 def select_bench(a, w, prec):
-    """28 fits (K = 2..8 x 4 restarts) of one dataset: lockstep batch (shared genotype, per-K autotuned
+    """28 fits (K = 2..8 x 4 restarts) of one dataset: concurrent batch (shared genotype, per-K autotuned
     chunk shapes, one captured graph per fit per iteration) against the same 28 fits run one at a time."""
     from haplo_amd.select import select_k
 
@@ -99,10 +99,10 @@ def select_bench(a, w, prec):
     Ks = list(range(2, 9))
     log("warm-up (kernel loading, graph capture)")
     select_k(data.bed, I, J, Ks=Ks, restarts=1, precision=prec, rtol=a.rtol, n_iter=3, autotune=False)
-    log("lockstep batch")
-    b = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, lockstep=True)
+    log("concurrent batch")
+    b = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, batched=True)
     log(f"batch wall {b.wall_s:.2f}s best K={b.best_k}")
-    sq = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, lockstep=False)
+    sq = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, batched=False)
     log(f"sequential wall {sq.wall_s:.2f}s best K={sq.best_k}")
     lat_b = np.array([t["finished_at_s"] for t in b.table])
     lat_s = np.array([t["seconds"] for t in sq.table])

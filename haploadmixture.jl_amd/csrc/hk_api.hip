@@ -791,6 +791,20 @@ struct FitRun {
   bool active = true;
   std::chrono::steady_clock::time_point t0;
 };
+
+// One completion event per fit of a batch (no timing).
+struct BatchEvents {
+  std::vector<hipEvent_t> e;
+  hipError_t status = hipSuccess;
+  explicit BatchEvents(int n) : e(n, nullptr) {
+    for (int b = 0; b < n && status == hipSuccess; ++b)
+      status = hipEventCreateWithFlags(&e[b], hipEventDisableTiming);
+  }
+  ~BatchEvents() {
+    for (auto x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
 }  // namespace
 
 static int fit_prepare(FitRun& f, double ll_start) {
@@ -946,21 +960,39 @@ extern "C" int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const
     CHK(fit_em(runs[b], opts[b].em_iters > 0 ? ll_em[b] : (ll_start ? ll_start[b] : NAN)));
     runs[b].o = &opts[b];
   }
+  // QN iterations, event-driven: a fit's next iteration is enqueued as soon as its previous one (and the
+  // copy of its log-likelihoods) completed, so no fit waits for the slowest one of a round.  A fit's
+  // results do not depend on the order (every context owns its buffers and stream).
+  BatchEvents ev(n);
+  HIPC(nullptr, ev.status);
+  auto launch = [&](int b) -> int {
+    hk_ctx* c = ctxs[b];
+    CHK(launch_iteration(c, &opts[b], opts[b].use_graph != 0, runs[b].ll_new));
+    HIPC(c, hipMemcpyAsync(c->h_ll, c->d_ll, sizeof(double) * 3, hipMemcpyDeviceToHost, c->s));
+    HIPC(c, hipEventRecord(ev.e[b], c->s));
+    return HK_OK;
+  };
   int active = 0;
-  for (int b = 0; b < n; ++b) active += runs[b].active;
+  for (int b = 0; b < n; ++b)
+    if (runs[b].active) {
+      CHK(launch(b));
+      ++active;
+    }
   while (active > 0) {
-    for (int b = 0; b < n; ++b)
-      if (runs[b].active) CHK(launch_iteration(ctxs[b], &opts[b], opts[b].use_graph != 0, runs[b].ll_new));
-    for (int b = 0; b < n; ++b)
+    for (int b = 0; b < n; ++b) {
+      if (!runs[b].active) continue;
+      const hipError_t e = hipEventQuery(ev.e[b]);
+      if (e == hipErrorNotReady) continue;
+      HIPC(ctxs[b], e);
+      CHK(fit_after_iteration(runs[b]));
       if (runs[b].active) {
-        CHK(read_ll(ctxs[b], 3));
-        CHK(fit_after_iteration(runs[b]));
-        if (!runs[b].active) {
-          --active;
-          if (seconds_at_finish)
-            seconds_at_finish[b] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        }
+        CHK(launch(b));
+      } else {
+        --active;
+        if (seconds_at_finish)
+          seconds_at_finish[b] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       }
+    }
   }
   for (int b = 0; b < n; ++b) CHK(fit_finish(runs[b]));
   return HK_OK;

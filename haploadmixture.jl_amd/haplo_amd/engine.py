@@ -234,7 +234,8 @@ class HipAdmixData:
 
 
 def fit_batch(engines, opts_list, ll_start=None):
-    """hk_fit_batch: fits of several contexts in lockstep; returns (results, seconds_at_finish)."""
+    """hk_fit_batch: fits of several contexts run concurrently (each enqueues its next
+    iteration as soon as its previous one completed); returns (results, seconds_at_finish)."""
     n = len(engines)
     L = _native.hip()
     handles = (ctypes.c_void_p * n)(*[e._h for e in engines])

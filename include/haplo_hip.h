@@ -138,8 +138,9 @@ int hk_fit(hk_ctx* ctx, const hk_fit_opts* opts, double ll_start, double* ll_tra
 int hk_fit_ex(hk_ctx* ctx, const hk_fit_opts* opts, double ll_start, double* ll_trace, double* t_trace,
               hk_fit_result* res);
 
-/* Fits of several contexts in lockstep (each on its own stream; with use_graph every round launches one
- * captured graph per active fit).  opts, ll_start (may be NULL) and res have n entries; seconds_at_finish
+/* Fits of several contexts run concurrently, each on its own stream: a fit's next iteration (one captured
+ * graph with use_graph) is enqueued as soon as its previous one completed, without waiting for the other
+ * fits.  Each fit's results are those of hk_fit_ex on its context.  opts, ll_start (may be NULL) and res have n entries; seconds_at_finish
  * (may be NULL) receives the wall time at which each fit stopped. */
 int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const double* ll_start, hk_fit_result* res,
                  double* seconds_at_finish);

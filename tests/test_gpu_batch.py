@@ -1,4 +1,4 @@
-"""GPU: graph-captured iterations, shared genotype contexts, lockstep batches, autotuner, AIC selection."""
+"""GPU: graph-captured iterations, shared genotype contexts, concurrent batches, autotuner, AIC selection."""
 import numpy as np
 import pytest
 
@@ -41,7 +41,7 @@ def test_shared_context_matches_own_context():
     assert np.array_equal(r1["trace"], r2["trace"])
 
 
-def test_lockstep_batch_matches_sequential_fits():
+def test_batch_matches_sequential_fits():
     """hk_fit_batch over uneven K gives the same fits as one-at-a-time hk_fit."""
     I, J = 400, 500
     s = simulate(I, J, 4, fst=0.1, miss=0.02, seed=9)

@@ -210,6 +210,38 @@ def test_deterministic_repeat():
     assert l1 == l2
 
 
+def test_error_paths_leave_context_usable(oracle):
+    """Rejected calls (NaN / Inf parameters, algorithms_inner.jl:132-133; bad fit options; bad pass
+    kind) return their status without touching the device state: the context's parameters and
+    results afterwards are those of the last accepted call."""
+    from haplo_amd import HipError
+    print("error paths", flush=True)
+    s, q, p = case(50, 33, 4)
+    with _eng(s, q, p) as e:
+        ll0 = e.loglikelihood()
+        for bad_q, bad_p in ((np.nan, None), (None, np.inf)):
+            qb, pb = q.copy(), p.copy()
+            if bad_q is not None:
+                qb[2, 17] = bad_q
+            if bad_p is not None:
+                pb[1, 40] = bad_p
+            with pytest.raises(HipError) as ex:
+                e.set_params(qb, pb)
+            assert ex.value.code == -4 and "non-finite" in str(ex.value)
+        q1, p1 = e.get_params()
+        assert np.array_equal(q1, q) and np.array_equal(p1, p)
+        with pytest.raises(HipError) as ex:
+            e.fit(Qn=0)
+        assert ex.value.code == -1
+        with pytest.raises(HipError) as ex:
+            e.time_pass(7)
+        assert ex.value.code == -1
+        with pytest.raises(ValueError):
+            e.set_params(q[:, :-1], p)
+        assert e.loglikelihood() == ll0
+    assert abs(ll0 - oracle.loglik(s.bed, s.I, s.J, q, p)) <= 1e-12 * abs(ll0)
+
+
 @pytest.mark.slow
 def test_c2_scale_fp64_parity(oracle):
     """Single-evaluation parity at the FP64 GPU workload size: I=5000, J=50,000, K=4 (1e-12)."""
