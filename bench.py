@@ -100,9 +100,11 @@ def select_bench(a, w, prec):
     log("warm-up (kernel loading, graph capture)")
     select_k(data.bed, I, J, Ks=Ks, restarts=1, precision=prec, rtol=a.rtol, n_iter=3, autotune=False)
     log("concurrent batch")
-    b = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, batched=True)
+    b = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, batched=True,
+                 use_graph=not a.eager)
     log(f"batch wall {b.wall_s:.2f}s best K={b.best_k}")
-    sq = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, batched=False)
+    sq = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, batched=False,
+                  use_graph=not a.eager)
     log(f"sequential wall {sq.wall_s:.2f}s best K={sq.best_k}")
     lat_b = np.array([t["finished_at_s"] for t in b.table])
     lat_s = np.array([t["seconds"] for t in sq.table])
@@ -113,7 +115,7 @@ def select_bench(a, w, prec):
         "vs_baseline": None, "dtype": "f32-products/f64-accum" if prec == "mixed" else "f64",
         "data": "synthetic (seeded generator, true K=5)",
         "config": {"workload": a.workload, "I": I, "J": J, "Ks": Ks, "restarts": 4, "fits": len(b.table),
-                   "precision": prec, "rtol": a.rtol},
+                   "precision": prec, "rtol": a.rtol, "graphs": not a.eager},
         "sequential_wall_s": round(sq.wall_s, 3),
         "speedup_batch_over_sequential": round(sq.wall_s / b.wall_s, 3),
         "per_fit_latency_batch_s": {"p50": round(float(np.percentile(lat_b, 50)), 3),
@@ -291,6 +293,9 @@ def main():
                     help="skip the fit with the reference's T2 = Float32 semantics (FP32 accumulation)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-blocks", type=int, default=1000)
+    ap.add_argument("--eager", action="store_true",
+                    help="model selection: enqueue every iteration's kernels directly instead of replaying one "
+                         "captured graph per fit (for kernel traces)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the timed fits, run a plain copy kernel of this size (counter calibration)")
     a = ap.parse_args()

@@ -998,8 +998,43 @@ extern "C" int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const
   return HK_OK;
 }
 
+namespace {
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+}  // namespace
+
+// Device time of a whole pass as a fit runs it -- parameter conversion, staging tables, the pass kernel,
+// the reduction over its chunks and the finishing kernels: the chunk count trades the pass kernel's
+// parallelism against the partial sums that the reduction reads.  kind 0 = log-likelihood, 1 = update_q
+// curvature pass, 2 = update_p curvature pass.
+static int time_full_pass(hk_ctx* c, int kind, int reps, double* ms) {
+  EventPair ev;
+  HIPC(c, hipEventCreate(&ev.a));
+  HIPC(c, hipEventCreate(&ev.b));
+  double* x = c->xb[0];
+  double total = 0.0;
+  for (int r = 0; r < reps; ++r) {
+    HIPC(c, hipEventRecord(ev.a, c->s));
+    if (kind == 0) CHK(run_qpass(c, MODE_LL, qof(c, x), pof(c, x), nullptr, 0, c->d_ll + 5));
+    else if (kind == 1) CHK(run_qpass(c, MODE_CURV, qof(c, x), pof(c, x)));
+    else CHK(run_ppass(c, MODE_CURV, qof(c, x), pof(c, x)));
+    HIPC(c, hipEventRecord(ev.b, c->s));
+    HIPC(c, hipEventSynchronize(ev.b));
+    float t = 0.f;
+    HIPC(c, hipEventElapsedTime(&t, ev.a, ev.b));
+    total += t;
+  }
+  *ms = total / reps;
+  return HK_OK;
+}
+
 // Autotuner: time the chunk (tile) shapes of the individual-parallel passes (log-likelihood + update_q)
-// and of the block-parallel update_p pass on the context's data, keep the fastest.
+// and of the block-parallel update_p pass on the context's data, each with its reduction, keep the fastest.
 extern "C" int hk_autotune(hk_ctx* c, int reps, int* q_chunks, int* p_chunks) {
   if (!c || reps < 1) return fail(c, HK_EINVAL, "invalid arguments");
   drop_graph(c);
@@ -1018,15 +1053,15 @@ extern "C" int hk_autotune(hk_ctx* c, int reps, int* q_chunks, int* p_chunks) {
     if (qv > c->g.J8) continue;
     CHK(hk_set_tuning(c, qv, preq0));
     double tl = 0, tu = 0;
-    CHK(hk_time_pass(c, 0, reps, &tl));
-    CHK(hk_time_pass(c, 1, reps, &tu));
+    CHK(time_full_pass(c, 0, reps, &tl));
+    CHK(time_full_pass(c, 1, reps, &tu));
     if (tl + tu < tq_best) { tq_best = tl + tu; qbest = c->g.qchunks; }
   }
   for (int pv : pc) {
     if (pv > c->g.I8) continue;
     CHK(hk_set_tuning(c, qbest, pv));
     double tp = 0;
-    CHK(hk_time_pass(c, 2, reps, &tp));
+    CHK(time_full_pass(c, 2, reps, &tp));
     if (tp < tp_best) { tp_best = tp; pbest = c->g.pchunks; }
   }
   (void)qreq0;

@@ -870,17 +870,34 @@ __global__ __launch_bounds__(256) void k_zal_dots(int64_t N, const int* __restri
 // Both are multisecant Broyden updates from H0 = -I of the inverse Jacobian of G(x) = F(x) - x: ZAL the
 // first ("good") kind, LBQN the second ("bad") kind, H = -I + (U + Y)(Y'Y)^-1 Y', which satisfies the
 // secants H Y = U; x_qn = x - H G(x).
-__global__ void k_zal_solve(int nb, const int* __restrict__ d_iter, int Qn, const double* __restrict__ part,
-                            double* __restrict__ coef, int mode) {
+// The ZAL_NB block partials of each dot product are summed by ZAL_NG thread groups (group g takes the
+// partials g, g + ZAL_NG, ...; all loads issued together), then over the groups in order.
+#define ZAL_NG 16
+__global__ __launch_bounds__(64 * ZAL_NG) void k_zal_solve(const int* __restrict__ d_iter, int Qn,
+                                                          const double* __restrict__ part,
+                                                          double* __restrict__ coef, int mode) {
+  static_assert(ZAL_NDOT <= 64 && ZAL_NB % ZAL_NG == 0, "one dot product per lane of a group");
+  __shared__ double red[ZAL_NG][64];
   __shared__ double dots[ZAL_NDOT];
   int col, ncols;
   zal_cols(d_iter, Qn, col, ncols);
   (void)col;
-  const int d = threadIdx.x;
+  const int d = threadIdx.x & 63, grp = threadIdx.x >> 6;
   if (d < ZAL_NDOT) {
+    double v[ZAL_NB / ZAL_NG];
+#pragma unroll
+    for (int t = 0; t < ZAL_NB / ZAL_NG; ++t) v[t] = part[(grp + ZAL_NG * t) * ZAL_NDOT + d];
     double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += part[b * ZAL_NDOT + d];
-    dots[d] = s;
+#pragma unroll
+    for (int t = 0; t < ZAL_NB / ZAL_NG; ++t) s += v[t];
+    red[grp][d] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < ZAL_NDOT) {
+    double s = 0.0;
+#pragma unroll
+    for (int g = 0; g < ZAL_NG; ++g) s += red[g][threadIdx.x];
+    dots[threadIdx.x] = s;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -1084,7 +1101,7 @@ hipError_t launch_zal(int64_t N, int Qn, const int* d_iter, double* U, double* V
                       const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s, int mode) {
   if (Qn < 1 || Qn > ZAL_QMAX || mode < 0 || mode > 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_zal_dots, dim3(ZAL_NB), dim3(256), 0, s, N, d_iter, Qn, U, V, x0, x1, x2, dots_part);
-  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(64), 0, s, ZAL_NB, d_iter, Qn, dots_part, coef, mode);
+  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(64 * ZAL_NG), 0, s, d_iter, Qn, dots_part, coef, mode);
   hipLaunchKernelGGL(k_zal_apply, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, d_iter, Qn, V, x1, coef, x_out);
   return hipGetLastError();
 }

@@ -146,7 +146,8 @@ int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const double* ll
                  double* seconds_at_finish);
 
 /* Autotuner: time candidate chunk shapes of the passes on the context's data (reps launches each) and keep
- * the fastest; the chosen q/p chunk counts are returned. */
+ * the fastest (each pass timed with the reduction over its chunks); the chosen q/p chunk counts are
+ * returned. */
 int hk_autotune(hk_ctx* ctx, int reps, int* q_chunks, int* p_chunks);
 
 /* Pass geometry tuning: number of j chunks of the individual-parallel passes and i chunks of the
