@@ -474,252 +474,312 @@ __device__ __forceinline__ void small_spd_solve(const double (&S)[K][K], double 
   }
 }
 
-template <int K, bool PEN>
+// Each 4-lane group (slot) solves the quads s, s + S, s + 2S, ... one after another (S = slots in the
+// grid, which is capped at what the device holds at once), so lanes whose QP converged in a few
+// active-set iterations go on to their next quad instead of idling until the slowest quad of their
+// wavefront is done.  Every quad's result depends only on its own data, not on the slot or the order.
+template <int K, bool PEN, bool MULTI>
+__device__ __forceinline__ void p_step4_body(int64_t J, const double* __restrict__ res, const double* __restrict__ p,
+                                             double* __restrict__ p_next, double rho, double eps,
+                                             double* __restrict__ dp, int* __restrict__ redo) {
+  constexpr int NT = K + K * (K + 1) / 2;
+  constexpr int maxit = 4 * (4 * K) + 20;  // as wave_qp with N = 4K
+  constexpr int NKK = K * (K + 1) / 2;
+  const int l = threadIdx.x & 3;
+  const int64_t nslot = ((int64_t)gridDim.x * blockDim.x) >> 2;
+  int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+  bool valid = j < J;
+  double A[NKK];  // packed lower triangle, A[pk(k, k2)]
+  double g[K], lo[K], hi[K], pv[K], x[K];
+  int st[K];      // 0 free, -1 at lower, +1 at upper
+  double lam[K];  // equality-mode multipliers of the last Newton step
+  double a = 0.0, tol = 0.0, piv_min = 0.0;
+  int it = 0;
+  bool done = false, dense = false;
+  // quad-uniform: all four lanes of a quad take the same branches
+  auto load = [&]() {
+    const double* r = res + (4 * j + l) * NT;
+    #pragma unroll
+    for (int t = 0; t < NKK; ++t) A[t] = r[K + t];
+    double gs = 0.0, bm = 0.0, dg = 0.0;
+    #pragma unroll
+    for (int k = 0; k < K; ++k) {
+      pv[k] = p[k + (int64_t)K * (4 * j + l)];
+      g[k] = -r[k];
+      lo[k] = 0.0 - pv[k];
+      hi[k] = 1.0 - pv[k];
+      x[k] = 0.0;
+      lam[k] = 0.0;
+      st[k] = lo[k] >= 0.0 ? -1 : (hi[k] <= 0.0 ? 1 : 0);
+      gs = fmax(gs, fabs(g[k]));
+      bm = fmax(bm, fmax(fabs(lo[k]), fabs(hi[k])));
+      dg = fmax(dg, A[pk(k, k)]);
+    }
+    a = PEN ? rho : quad_max(dg);
+    double hm = a;  // the entries a of the off-diagonal blocks
+    #pragma unroll
+    for (int k = 0; k < K; ++k)
+      #pragma unroll
+      for (int k2 = 0; k2 <= k; ++k2) hm = fmax(hm, fabs(A[pk(k, k2)] + (k == k2 ? a : 0.0)));
+    hm = quad_max(hm);
+    gs = quad_max(gs);
+    bm = quad_max(bm);
+    tol = 1e-11 * (gs + hm * bm) + 1e-300;
+    // pivots below this share of the block's largest diagonal send the QP to the dense solver
+    piv_min = 1e-6 * dg + 1e-300;
+    it = 0;
+    done = false;
+    dense = false;
+  };
+  // one active-set iteration (sets done when the QP is solved or handed to the dense solver)
+  auto iterate = [&]() {
+      bool fr[K];
+      #pragma unroll
+      for (int k = 0; k < K; ++k) fr[k] = st[k] == 0;
+      // masked block Cholesky: A on F_l x F_l, identity elsewhere
+      double L[K][K], di[K];
+      bool pd = true;
+      #pragma unroll
+      for (int c = 0; c < K; ++c) {
+        double s = fr[c] ? A[pk(c, c)] : 1.0;
+        #pragma unroll
+        for (int t = 0; t < c; ++t) s -= L[c][t] * L[c][t];
+        const bool ok = !fr[c] || s > piv_min;
+        pd = pd && ok;
+        s = ok ? s : 1.0;
+        const double sq = sqrt(s);
+        di[c] = 1.0 / sq;
+        L[c][c] = sq;
+        #pragma unroll
+        for (int rr = c + 1; rr < K; ++rr) {
+          double v = (fr[rr] && fr[c]) ? A[pk(rr, c)] : 0.0;
+          #pragma unroll
+          for (int t = 0; t < c; ++t) v -= L[rr][t] * L[c][t];
+          L[rr][c] = v * di[c];
+        }
+      }
+      if (quad_max(pd ? 0.0 : 1.0) > 0.0) {
+        dense = true;
+        done = true;
+        return;
+      }
+      // C = (A_l[F, F])^-1, zero outside F (packed lower triangle): column c solves L L' C[:, c] = e_c
+      double C[NKK];
+      #pragma unroll
+      for (int c = 0; c < K; ++c) {
+        double u[K];
+        #pragma unroll
+        for (int rr = 0; rr < K; ++rr) {
+          double v = rr == c ? 1.0 : 0.0;
+          #pragma unroll
+          for (int t = 0; t < rr; ++t) v -= L[rr][t] * u[t];
+          u[rr] = v * di[rr];
+        }
+        #pragma unroll
+        for (int rr = K - 1; rr >= 0; --rr) {
+          double v = u[rr];
+          #pragma unroll
+          for (int t = rr + 1; t < K; ++t) v -= L[t][rr] * u[t];
+          u[rr] = v * di[rr];
+        }
+        #pragma unroll
+        for (int rr = c; rr < K; ++rr) C[pk(rr, c)] = (fr[rr] && fr[c]) ? u[rr] : 0.0;
+      }
+      // gradient at x: r = A x + a sum_l x + g
+      double xs[K], gr[K], y[K], ys[K], Cs[K][K];  // Cs: lower triangle used
+      #pragma unroll
+      for (int k = 0; k < K; ++k) xs[k] = quad_sum<double>(x[k]);
+      #pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double s = g[k];
+        #pragma unroll
+        for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(A[pk(k, k2)], x[k2], s);
+        gr[k] = __builtin_fma(a, xs[k], s);
+      }
+      #pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        #pragma unroll
+        for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(C[pk(k, k2)], fr[k2] ? gr[k2] : 0.0, s);
+        y[k] = s;
+      }
+      #pragma unroll
+      for (int k = 0; k < K; ++k) {
+        ys[k] = quad_sum<double>(y[k]);
+        #pragma unroll
+        for (int k2 = 0; k2 <= k; ++k2) Cs[k][k2] = quad_sum<double>(C[pk(k, k2)]);
+      }
+      double d[K];
+      if constexpr (PEN) {
+        double t[K];
+        #pragma unroll
+        for (int k = 0; k < K; ++k) {
+          t[k] = ys[k];
+          Cs[k][k] += 1.0 / rho;
+        }
+        small_spd_solve<K>(Cs, t);
+        #pragma unroll
+        for (int k = 0; k < K; ++k) {
+          double s = y[k];
+          #pragma unroll
+          for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(-C[pk(k, k2)], t[k2], s);
+          d[k] = fr[k] ? -s : 0.0;
+        }
+      } else {
+        // groups without a free member are already satisfied: identity rows, zero multiplier
+        #pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const bool has = Cs[k][k] > 0.0;
+          lam[k] = has ? -ys[k] : 0.0;
+          if (!has) {
+            #pragma unroll
+            for (int k2 = 0; k2 < K; ++k2) {
+              if (k2 <= k) Cs[k][k2] = k2 == k ? 1.0 : 0.0;
+              else Cs[k2][k] = 0.0;
+            }
+          }
+        }
+        small_spd_solve<K>(Cs, lam);
+        #pragma unroll
+        for (int k = 0; k < K; ++k) {
+          double s = y[k];
+          #pragma unroll
+          for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(C[pk(k, k2)], lam[k2], s);
+          // the lone free member of a group cannot move: its step is exactly 0
+          const double nfree = quad_sum<double>(fr[k] ? 1.0 : 0.0);
+          d[k] = (fr[k] && nfree > 1.5) ? -s : 0.0;
+        }
+      }
+      // ratio test (smallest step to a bound; ties to the smaller variable index l K + k)
+      double tmin = 2.0;
+      int imin = l * K;
+      #pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double tc = 2.0;
+        if (fr[k]) {
+          if (d[k] < 0.0 && x[k] + d[k] < lo[k]) tc = (lo[k] - x[k]) / d[k];
+          else if (d[k] > 0.0 && x[k] + d[k] > hi[k]) tc = (hi[k] - x[k]) / d[k];
+        }
+        if (tc < tmin) {
+          tmin = tc;
+          imin = l * K + k;
+        }
+      }
+      quad_arg<true>(tmin, imin);
+      const bool blocked = tmin < 1.0;
+      const double alpha = blocked ? fmax(tmin, 0.0) : 1.0;
+      #pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (fr[k]) {
+          const double v = x[k] + alpha * d[k];
+          x[k] = v < lo[k] ? lo[k] : (v > hi[k] ? hi[k] : v);
+        }
+      if (blocked) {
+        #pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (imin == l * K + k) {
+            const int side = d[k] < 0.0 ? -1 : 1;
+            x[k] = side < 0 ? lo[k] : hi[k];
+            st[k] = side;
+          }
+        return;
+      }
+      // at the minimiser of the working set: bound multipliers, release the most violated
+      #pragma unroll
+      for (int k = 0; k < K; ++k) xs[k] = quad_sum<double>(x[k]);
+      double vmax = -1.0;
+      int iv = l * K;
+      #pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double s = g[k];
+        #pragma unroll
+        for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(A[pk(k, k2)], x[k2], s);
+        s = __builtin_fma(a, xs[k], s);
+        if constexpr (!PEN) s += lam[k];
+        const double v = st[k] != 0 ? (st[k] < 0 ? -s : s) : -1.0;
+        if (v > vmax) {
+          vmax = v;
+          iv = l * K + k;
+        }
+      }
+      quad_arg<false>(vmax, iv);
+      if (!(vmax > tol)) {
+        done = true;
+        return;
+      }
+      #pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (iv == l * K + k) st[k] = 0;
+  };
+  // result of a finished quad: the dense list, or p_next = proj(p + dp)
+  auto finish = [&]() {
+    if (dense) {
+      if (l == 0) redo[1 + atomicAdd(redo, 1)] = (int)j;
+      return;
+    }
+    if (dp) {
+      #pragma unroll
+      for (int k = 0; k < K; ++k) dp[k + (int64_t)K * (4 * j + l)] = x[k];
+    }
+    // p_next = p + dp, then project_p! on each (k, j) 4-vector (the quad's four lanes)
+    #pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double nv = pv[k] + x[k];
+      double v[4] = {quad_bcast<0>(nv), quad_bcast<1>(nv), quad_bcast<2>(nv), quad_bcast<3>(nv)};
+      proj_simplex<4>(v, eps);
+      p_next[k + (int64_t)K * (4 * j + l)] = l == 0 ? v[0] : (l == 1 ? v[1] : (l == 2 ? v[2] : v[3]));
+    }
+  };
+  if (valid) load();
+  if constexpr (!MULTI) {  // one quad per slot
+    while (__ballot(valid && !done) != 0ull) {
+      if (valid && !done) {
+        iterate();
+        if (!done && ++it >= maxit) done = true;
+      }
+    }
+    if (valid) finish();
+    return;
+  }
+  // A finished quad with a next one hands over at once; the others finish together once every quad of the
+  // wavefront is done (one convergent pass instead of one per completion time).
+  while (__ballot(valid) != 0ull) {
+    if (__ballot(valid && !done) == 0ull) {
+      if (valid) {
+        finish();
+        j += nslot;
+        valid = j < J;
+        if (valid) load();
+      }
+      continue;
+    }
+    if (valid && !done) {
+      iterate();
+      if (!done && ++it >= maxit) done = true;
+    }
+    if (valid && done && j + nslot < J) {
+      finish();
+      j += nslot;
+      load();
+    }
+  }
+}
+
+// MULTI: the grid is capped at the resident workgroups and slots solve several quads; otherwise one quad
+// per slot.  Two waves per SIMD (at most 256 registers) for the penalty form up to K = 6 and the equality
+// form up to K = 5; the larger ones need more registers and run one wave per SIMD.
+template <int K, bool PEN, bool MULTI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p_step4_w2(
+    int64_t J, const double* __restrict__ res, const double* __restrict__ p, double* __restrict__ p_next, double rho,
+    double eps, double* __restrict__ dp, int* __restrict__ redo) {
+  p_step4_body<K, PEN, MULTI>(J, res, p, p_next, rho, eps, dp, redo);
+}
+template <int K, bool PEN, bool MULTI>
 __global__ __launch_bounds__(256) void k_p_step4(int64_t J, const double* __restrict__ res,
                                                  const double* __restrict__ p, double* __restrict__ p_next,
                                                  double rho, double eps, double* __restrict__ dp,
                                                  int* __restrict__ redo) {
-  constexpr int NT = K + K * (K + 1) / 2;
-  constexpr int maxit = 4 * (4 * K) + 20;  // as wave_qp with N = 4K
-  const int l = threadIdx.x & 3;
-  const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
-  const bool valid = j < J;
-  const int64_t jj = valid ? j : J - 1;  // quads past the end mirror the last block and write nothing
-  const double* r = res + (4 * jj + l) * NT;
-  constexpr int NKK = K * (K + 1) / 2;
-  double A[NKK];  // packed lower triangle, A[pk(k, k2)]
-#pragma unroll
-  for (int t = 0; t < NKK; ++t) A[t] = r[K + t];
-  double g[K], lo[K], hi[K], pv[K], x[K];
-  int st[K];  // 0 free, -1 at lower, +1 at upper
-  double gs = 0.0, bm = 0.0, dg = 0.0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    pv[k] = p[k + (int64_t)K * (4 * jj + l)];
-    g[k] = -r[k];
-    lo[k] = 0.0 - pv[k];
-    hi[k] = 1.0 - pv[k];
-    x[k] = 0.0;
-    st[k] = lo[k] >= 0.0 ? -1 : (hi[k] <= 0.0 ? 1 : 0);
-    gs = fmax(gs, fabs(g[k]));
-    bm = fmax(bm, fmax(fabs(lo[k]), fabs(hi[k])));
-    dg = fmax(dg, A[pk(k, k)]);
-  }
-  const double a = PEN ? rho : quad_max(dg);
-  double hm = a;  // the entries a of the off-diagonal blocks
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-#pragma unroll
-    for (int k2 = 0; k2 <= k; ++k2) hm = fmax(hm, fabs(A[pk(k, k2)] + (k == k2 ? a : 0.0)));
-  hm = quad_max(hm);
-  gs = quad_max(gs);
-  bm = quad_max(bm);
-  const double tol = 1e-11 * (gs + hm * bm) + 1e-300;
-  // pivots below this share of the block's largest diagonal send the QP to the dense solver
-  const double piv_min = 1e-6 * dg + 1e-300;
-
-  bool done = !valid, dense = false;
-  double lam[K];  // equality-mode multipliers of the last Newton step
-#pragma unroll
-  for (int k = 0; k < K; ++k) lam[k] = 0.0;
-  for (int it = 0; it < maxit; ++it) {
-    if (__ballot(!done) == 0ull) break;
-    if (done) continue;
-    bool fr[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) fr[k] = st[k] == 0;
-    // masked block Cholesky: A on F_l x F_l, identity elsewhere
-    double L[K][K], di[K];
-    bool pd = true;
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-      double s = fr[c] ? A[pk(c, c)] : 1.0;
-#pragma unroll
-      for (int t = 0; t < c; ++t) s -= L[c][t] * L[c][t];
-      const bool ok = !fr[c] || s > piv_min;
-      pd = pd && ok;
-      s = ok ? s : 1.0;
-      const double sq = sqrt(s);
-      di[c] = 1.0 / sq;
-      L[c][c] = sq;
-#pragma unroll
-      for (int rr = c + 1; rr < K; ++rr) {
-        double v = (fr[rr] && fr[c]) ? A[pk(rr, c)] : 0.0;
-#pragma unroll
-        for (int t = 0; t < c; ++t) v -= L[rr][t] * L[c][t];
-        L[rr][c] = v * di[c];
-      }
-    }
-    if (quad_max(pd ? 0.0 : 1.0) > 0.0) {
-      dense = true;
-      done = true;
-      continue;
-    }
-    // C = (A_l[F, F])^-1, zero outside F (packed lower triangle): column c solves L L' C[:, c] = e_c
-    double C[NKK];
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-      double u[K];
-#pragma unroll
-      for (int rr = 0; rr < K; ++rr) {
-        double v = rr == c ? 1.0 : 0.0;
-#pragma unroll
-        for (int t = 0; t < rr; ++t) v -= L[rr][t] * u[t];
-        u[rr] = v * di[rr];
-      }
-#pragma unroll
-      for (int rr = K - 1; rr >= 0; --rr) {
-        double v = u[rr];
-#pragma unroll
-        for (int t = rr + 1; t < K; ++t) v -= L[t][rr] * u[t];
-        u[rr] = v * di[rr];
-      }
-#pragma unroll
-      for (int rr = c; rr < K; ++rr) C[pk(rr, c)] = (fr[rr] && fr[c]) ? u[rr] : 0.0;
-    }
-    // gradient at x: r = A x + a sum_l x + g
-    double xs[K], gr[K], y[K], ys[K], Cs[K][K];  // Cs: lower triangle used
-#pragma unroll
-    for (int k = 0; k < K; ++k) xs[k] = quad_sum<double>(x[k]);
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double s = g[k];
-#pragma unroll
-      for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(A[pk(k, k2)], x[k2], s);
-      gr[k] = __builtin_fma(a, xs[k], s);
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double s = 0.0;
-#pragma unroll
-      for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(C[pk(k, k2)], fr[k2] ? gr[k2] : 0.0, s);
-      y[k] = s;
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      ys[k] = quad_sum<double>(y[k]);
-#pragma unroll
-      for (int k2 = 0; k2 <= k; ++k2) Cs[k][k2] = quad_sum<double>(C[pk(k, k2)]);
-    }
-    double d[K];
-    if constexpr (PEN) {
-      double t[K];
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        t[k] = ys[k];
-        Cs[k][k] += 1.0 / rho;
-      }
-      small_spd_solve<K>(Cs, t);
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        double s = y[k];
-#pragma unroll
-        for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(-C[pk(k, k2)], t[k2], s);
-        d[k] = fr[k] ? -s : 0.0;
-      }
-    } else {
-      // groups without a free member are already satisfied: identity rows, zero multiplier
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const bool has = Cs[k][k] > 0.0;
-        lam[k] = has ? -ys[k] : 0.0;
-        if (!has) {
-#pragma unroll
-          for (int k2 = 0; k2 < K; ++k2) {
-            if (k2 <= k) Cs[k][k2] = k2 == k ? 1.0 : 0.0;
-            else Cs[k2][k] = 0.0;
-          }
-        }
-      }
-      small_spd_solve<K>(Cs, lam);
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        double s = y[k];
-#pragma unroll
-        for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(C[pk(k, k2)], lam[k2], s);
-        // the lone free member of a group cannot move: its step is exactly 0
-        const double nfree = quad_sum<double>(fr[k] ? 1.0 : 0.0);
-        d[k] = (fr[k] && nfree > 1.5) ? -s : 0.0;
-      }
-    }
-    // ratio test (smallest step to a bound; ties to the smaller variable index l K + k)
-    double tmin = 2.0;
-    int imin = l * K;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double tc = 2.0;
-      if (fr[k]) {
-        if (d[k] < 0.0 && x[k] + d[k] < lo[k]) tc = (lo[k] - x[k]) / d[k];
-        else if (d[k] > 0.0 && x[k] + d[k] > hi[k]) tc = (hi[k] - x[k]) / d[k];
-      }
-      if (tc < tmin) {
-        tmin = tc;
-        imin = l * K + k;
-      }
-    }
-    quad_arg<true>(tmin, imin);
-    const bool blocked = tmin < 1.0;
-    const double alpha = blocked ? fmax(tmin, 0.0) : 1.0;
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      if (fr[k]) {
-        const double v = x[k] + alpha * d[k];
-        x[k] = v < lo[k] ? lo[k] : (v > hi[k] ? hi[k] : v);
-      }
-    if (blocked) {
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-        if (imin == l * K + k) {
-          const int side = d[k] < 0.0 ? -1 : 1;
-          x[k] = side < 0 ? lo[k] : hi[k];
-          st[k] = side;
-        }
-      continue;
-    }
-    // at the minimiser of the working set: bound multipliers, release the most violated
-#pragma unroll
-    for (int k = 0; k < K; ++k) xs[k] = quad_sum<double>(x[k]);
-    double vmax = -1.0;
-    int iv = l * K;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double s = g[k];
-#pragma unroll
-      for (int k2 = 0; k2 < K; ++k2) s = __builtin_fma(A[pk(k, k2)], x[k2], s);
-      s = __builtin_fma(a, xs[k], s);
-      if constexpr (!PEN) s += lam[k];
-      const double v = st[k] != 0 ? (st[k] < 0 ? -s : s) : -1.0;
-      if (v > vmax) {
-        vmax = v;
-        iv = l * K + k;
-      }
-    }
-    quad_arg<false>(vmax, iv);
-    if (!(vmax > tol)) {
-      done = true;
-      continue;
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      if (iv == l * K + k) st[k] = 0;
-  }
-  if (!valid) return;
-  if (dense) {
-    if (l == 0) redo[1 + atomicAdd(redo, 1)] = (int)j;
-    return;
-  }
-  if (dp) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) dp[k + (int64_t)K * (4 * j + l)] = x[k];
-  }
-  // p_next = p + dp, then project_p! on each (k, j) 4-vector (the quad's four lanes)
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const double nv = pv[k] + x[k];
-    double v[4] = {quad_bcast<0>(nv), quad_bcast<1>(nv), quad_bcast<2>(nv), quad_bcast<3>(nv)};
-    proj_simplex<4>(v, eps);
-    p_next[k + (int64_t)K * (4 * j + l)] = l == 0 ? v[0] : (l == 1 ? v[1] : (l == 2 ? v[2] : v[3]));
-  }
+  p_step4_body<K, PEN, MULTI>(J, res, p, p_next, rho, eps, dp, redo);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1009,23 +1069,50 @@ hipError_t launch_q_step(int K, int64_t I, const double* res_q, const double* q,
   });
 }
 
-// Four lanes per block QP (k_p_step4); blocks whose free curvature is not positive definite go through
-// redo to the dense one-wave solver, whose grid strides over the list (empty in the usual case).
+// Workgroups of k_p_step4 resident on the whole device at once (occupancy x compute units), computed once
+// per kernel instance: the grid never exceeds it, each slot then solves several quads in turn.
+template <typename F>
+static unsigned resident_blocks(F kernel, int block) {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 256;
+  return (unsigned)(per_cu * cus);
+}
+
+// Four lanes per block QP (k_p_step4, several quads per slot); blocks whose free curvature is not positive
+// definite go through redo ([count, list], J + 1 ints) to the dense one-wave solver, whose grid strides
+// over the list (empty in the usual case).
 hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
                          int* fails, int* redo, hipStream_t s, double* dp) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, redo, 0);
   const unsigned nq = (unsigned)((4 * J + 255) / 256), nd = (unsigned)(J < 1024 ? J : 1024);
   return with_k(K, [&](auto kc) {
     constexpr int KT = decltype(kc)::value;
+    auto run = [&](auto single, auto multi, auto list_kernel, unsigned cap) {
+      if (nq <= cap)
+        hipLaunchKernelGGL(single, dim3(nq), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp, redo);
+      else
+        hipLaunchKernelGGL(multi, dim3(cap), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp, redo);
+      hipLaunchKernelGGL(list_kernel, dim3(nd), dim3(64), 0, s, res_p, p, p_next, o.rho, o.eps, fails, dp, redo);
+    };
     if (o.penalty) {
-      hipLaunchKernelGGL((k_p_step4<KT, true>), dim3(nq), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp, redo);
-      hipLaunchKernelGGL((k_p_step_list<KT, true>), dim3(nd), dim3(64), 0, s, res_p, p, p_next, o.rho, o.eps, fails,
-                         dp, redo);
+      if constexpr (KT <= 6) {
+        static const unsigned cap = resident_blocks(k_p_step4_w2<KT, true, true>, 256);
+        run(k_p_step4_w2<KT, true, false>, k_p_step4_w2<KT, true, true>, k_p_step_list<KT, true>, cap);
+      } else {
+        static const unsigned cap = resident_blocks(k_p_step4<KT, true, true>, 256);
+        run(k_p_step4<KT, true, false>, k_p_step4<KT, true, true>, k_p_step_list<KT, true>, cap);
+      }
     } else {
-      hipLaunchKernelGGL((k_p_step4<KT, false>), dim3(nq), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp,
-                         redo);
-      hipLaunchKernelGGL((k_p_step_list<KT, false>), dim3(nd), dim3(64), 0, s, res_p, p, p_next, o.rho, o.eps, fails,
-                         dp, redo);
+      if constexpr (KT <= 5) {
+        static const unsigned cap = resident_blocks(k_p_step4_w2<KT, false, true>, 256);
+        run(k_p_step4_w2<KT, false, false>, k_p_step4_w2<KT, false, true>, k_p_step_list<KT, false>, cap);
+      } else {
+        static const unsigned cap = resident_blocks(k_p_step4<KT, false, true>, 256);
+        run(k_p_step4<KT, false, false>, k_p_step4<KT, false, true>, k_p_step_list<KT, false>, cap);
+      }
     }
   });
 }

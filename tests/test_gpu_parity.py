@@ -129,6 +129,50 @@ def test_p_step_four_lane_path_matches_oracle(K, oracle):
             assert rel(e.p_step(q_pass=q, penalty=pen), op) <= 1e-8, pen
 
 
+def _well_conditioned_blocks(XtX, K, J, ratio=1e-6):
+    """Blocks whose four haplotype curvatures A_l all have min / max eigenvalue above ratio.  With few
+    individuals, rare haplotypes make some A_l singular or nearly so; the P-side QP of such a block can be
+    flat or ill-conditioned along directions that move mass between haplotypes, where two correct solvers
+    differ by far more than round-off (parity unpinned for those blocks)."""
+    A = np.moveaxis(XtX.reshape(K, K, J, 4), (0, 1), (2, 3))  # J x 4 x K x K
+    ev = np.linalg.eigvalsh(A)
+    return np.all(ev[..., 0] > ratio * ev[..., -1], axis=1)
+
+
+@pytest.mark.parametrize("K", [3, 6])
+def test_p_step_work_queue_beyond_resident_slots(K, oracle):
+    """More block QPs than four-lane slots resident on the device (J = 100,000 > 65,536 slots at K = 3,
+    32,768 at K = 6), so most slots solve several quads in turn.  The result of every block is
+    bitwise the one a context holding only the first 20,000 blocks (one quad per slot) gives,
+    and two calls agree bitwise; against the oracle, the blocks with well-conditioned curvature agree to
+    1e-8 and every block is a valid simplex point.  The dense fallback list (singular haplotype
+    curvature, 64 individuals) is exercised in the same calls."""
+    I, J, Js = 64, 100000, 20000
+    print(f"p step work queue I={I} J={J} K={K}", flush=True)
+    s, q, p = case(I, J, K, miss=0.02, fst=0.1)
+    q = oracle.project_q(q)
+    p = oracle.project_p(p)
+    with _eng(s, q, p) as e:
+        a = e.p_step(q_pass=q, penalty=True)
+        b = e.p_step(q_pass=q, penalty=True)
+        tuning = e.get_tuning()
+    assert np.array_equal(a, b)
+    bps = (I + 3) // 4
+    with HipAdmixData(s.bed[: 2 * Js * bps], I, Js, K, precision="fp64") as e:
+        e.set_tuning(*tuning)  # same i-chunks: the same curvature sums, bit for bit
+        e.set_params(q, p[:, : 4 * Js])
+        assert np.array_equal(e.p_step(q_pass=q, penalty=True), a[:, : 4 * Js])
+    XtX, Xtz = oracle.update_p(s.bed, I, J, q, p)
+    op, fails = oracle.p_step(XtX, Xtz, p, penalty=True)
+    assert fails == 0
+    good = _well_conditioned_blocks(XtX, K, J)
+    assert good.mean() > 0.5
+    cols = np.repeat(good, 4)
+    assert rel(a[:, cols], op[:, cols]) <= 1e-8
+    blk = a.reshape(K, J, 4)
+    assert np.all(blk >= 1e-5 - 1e-15) and np.max(np.abs(blk.sum(axis=2) - 1.0)) <= 1e-12
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_fit_matches_oracle_fit(prec, oracle):
     """init_em! + admixture_qn! on the device vs the CPU restatement of the same loop."""
