@@ -41,7 +41,9 @@ struct hk_ctx {
   double* zal_part = nullptr;
   double* zal_coef = nullptr;
   int* fails = nullptr;
-  int* redo = nullptr;  // J + 1 ints: blocks handed to the dense P-step solver
+  int* redo = nullptr;       // J + 1 ints: early list of the blocks for the dense P-step solver
+  int* redo_late = nullptr;  // J + 1 ints: blocks the four-lane P-step kernel hands over itself
+  unsigned char* pflag = nullptr;  // J: block on the early list
   double* full_a = nullptr;  // ABI expansion buffers
   double* full_b = nullptr;
   int64_t full_len = 0;
@@ -82,6 +84,9 @@ static int fail(hk_ctx* c, int code, const char* fmt, ...) {
     int rc_ = (x);            \
     if (rc_ != HK_OK) return rc_; \
   } while (0)
+
+// The P row step's device lists; the early list is filled by the P-side curvature pass (run_ppass).
+static PStepLists p_lists(hk_ctx* c) { return PStepLists{c->redo, c->pflag, c->redo_late}; }
 
 static int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
@@ -229,7 +234,7 @@ static int run_ppass(hk_ctx* c, int mode, const double* q, const double* p, Time
     if (tm) tm->end();
     HIPC(c, launch_reduce_p(c->part_p, c->g.pchunks, NT, c->g, c->res_p, c->s));
   }
-  if (mode == MODE_CURV) HIPC(c, launch_finish_p(c->K, c->g, p, c->res_p, c->s));
+  if (mode == MODE_CURV) HIPC(c, launch_finish_p(c->K, c->g, p, c->res_p, c->pflag, c->redo, c->s));
   return HK_OK;
 }
 
@@ -257,7 +262,7 @@ int hk_ctx_destroy(hk_ctx* c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   void* owned[] = {c->qq, c->pp, c->xb[0], c->xb[1], c->xb[2], c->xb[3], c->x32, c->res_q, c->res_p,
                    c->part_q, c->part_p, c->ll_dev, c->ll_rows, c->U, c->V, c->zal_part, c->zal_coef, c->fails,
-                   c->redo, c->full_a, c->full_b, c->d_ll, c->d_iter};
+                   c->redo, c->redo_late, c->pflag, c->full_a, c->full_b, c->d_ll, c->d_iter};
   for (void* p : owned)
     if (p) (void)hipFree(p);
   if (c->owns_geno) {
@@ -299,6 +304,8 @@ static int init_fit_buffers(hk_ctx* c) {
   HIPC(c, hipMalloc(&c->ll_rows, sizeof(double) * I));
   HIPC(c, hipMalloc(&c->fails, sizeof(int)));
   HIPC(c, hipMalloc(&c->redo, sizeof(int) * (c->J + 1)));
+  HIPC(c, hipMalloc(&c->redo_late, sizeof(int) * (c->J + 1)));
+  HIPC(c, hipMalloc(&c->pflag, c->J));
   HIPC(c, hipMalloc(&c->d_ll, sizeof(double) * 8));
   HIPC(c, hipMalloc(&c->d_iter, sizeof(int)));
   HIPC(c, hipHostMalloc((void**)&c->h_ll, sizeof(double) * 8, hipHostMallocDefault));
@@ -558,7 +565,7 @@ int hk_p_step(hk_ctx* c, const double* q_pass, int penalty, double rho, double* 
   CHK(run_ppass(c, MODE_CURV, qp, pof(c, x)));
   StepOpts so{penalty, rho, 1e-5};
   HIPC(c, launch_set_int(c->fails, 0, c->s));
-  HIPC(c, launch_p_step(c->K, c->J, c->res_p, pof(c, x), pof(c, xn), so, c->fails, c->redo, c->s));
+  HIPC(c, launch_p_step(c->K, c->J, c->res_p, pof(c, x), pof(c, xn), so, c->fails, p_lists(c), c->s));
   int nf = 0;
   HIPC(c, hipMemcpyAsync(&nf, c->fails, sizeof(int), hipMemcpyDeviceToHost, c->s));
   HIPC(c, hipMemcpyAsync(p_next, pof(c, xn), sizeof(double) * c->K * 4 * c->J, hipMemcpyDeviceToHost, c->s));
@@ -664,7 +671,7 @@ static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm, double 
   auto pstep = [&](const double* q, const double* p, double* p_out) -> int {
     CHK(run_ppass(c, MODE_CURV, q, p, tm, T_PPASS));
     if (tm) tm->begin(T_PSTEP);
-    HIPC(c, launch_p_step(K, J, c->res_p, p, p_out, pso, c->fails, c->redo, c->s, dp));
+    HIPC(c, launch_p_step(K, J, c->res_p, p, p_out, pso, c->fails, p_lists(c), c->s, dp));
     if (tm) tm->end();
     if (gi) CHK(halve_step(c, o, tm, false, p, dp, p_out, q, p_out, &ll_tmp));
     return HK_OK;

@@ -44,7 +44,10 @@ hipError_t launch_pside(int K, int prec, int mode, const PassGeom& g, const void
 hipError_t launch_reduce_q(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s);
 hipError_t launch_reduce_q_curv(const double* part, int nchunk, int K, const PassGeom& g, const double* q, double* res,
                                hipStream_t s);
-hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* res, hipStream_t s);
+// Gradient slots of the P-side curvature pass; with flags (J bytes) also the early list of the blocks the
+// P row step must solve densely (dense: J + 1 ints, [count, blocks]), reset here.
+hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* res, unsigned char* flags, int* dense,
+                           hipStream_t s);
 hipError_t launch_reduce_p(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s);
 hipError_t launch_reduce_ll(const double* part, int nchunk, const PassGeom& g, double* rows, double* total,
                             hipStream_t s);
@@ -61,9 +64,16 @@ struct StepOpts {
 // dq / dp (optional): the raw QP steps before projection, for the step-halving line search.
 hipError_t launch_q_step(int K, int64_t I, const double* res_q, const double* q, double* q_next, const StepOpts& o,
                          int* fails, hipStream_t s, double* dq = nullptr);
-// redo: J + 1 ints of scratch (the list of blocks left to the dense solver).
+// Device lists of the P row step.  dense / flags: the blocks launch_finish_p marked, solved by the dense
+// one-wave solver in the same launch as the four-lane QPs; late: J + 1 ints for blocks the four-lane
+// kernel hands over itself.
+struct PStepLists {
+  const int* dense;
+  const unsigned char* flags;
+  int* late;
+};
 hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
-                         int* fails, int* redo, hipStream_t s, double* dp = nullptr);
+                         int* fails, const PStepLists& lists, hipStream_t s, double* dp = nullptr);
 hipError_t launch_project_q(int K, int64_t I, double* q, double eps, hipStream_t s);
 hipError_t launch_project_p(int K, int64_t J, double* p, double eps, hipStream_t s);
 hipError_t launch_em_q(int K, int64_t I, int64_t J, const double* res_q, int NT, const int64_t* dm, const double* q,

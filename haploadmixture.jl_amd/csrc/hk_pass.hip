@@ -939,8 +939,15 @@ __global__ void k_xtz_q(int K, int64_t I, const double* __restrict__ q, double* 
 }
 
 // Curvature pass, P side: Xtz_p[:, jh] = XtX_p[:, :, jh] p[:, jh] into slots 0..K-1 of res[jh][NT].
+// With flags: also marks the blocks whose P row QP the four-lane solver (k_p_step4) would hand to the
+// dense one: a free haplotype block of the curvature with a Cholesky pivot below 1e-6 of its largest
+// diagonal, the test k_p_step4 applies at its first active-set iteration, whose free set p fixes (and
+// free subsets only raise the pivots).  They are listed in dense[1 .. dense[0]] so that the dense solver
+// can run alongside the four-lane kernel instead of after it.  Lane = (block, haplotype): the four
+// lanes of a block are adjacent (4J is a multiple of 4), so the quad's verdict is a DPP maximum.
 template <int K>
-__global__ void k_finish_p(int64_t J, const double* __restrict__ p, double* __restrict__ res) {
+__global__ void k_finish_p(int64_t J, const double* __restrict__ p, double* __restrict__ res,
+                           unsigned char* __restrict__ flags, int* __restrict__ dense) {
   const int64_t jh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (jh >= 4 * J) return;
   constexpr int NT = K + K * (K + 1) / 2;
@@ -959,6 +966,45 @@ __global__ void k_finish_p(int64_t J, const double* __restrict__ p, double* __re
       s += a[hi * (hi + 1) / 2 + lo] * pv[k2];
     }
     r[k] = s;
+  }
+  if (flags == nullptr) return;
+  bool fr[K];
+  double dg = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double lo = 0.0 - pv[k], hi = 1.0 - pv[k];
+    fr[k] = !(lo >= 0.0) && !(hi <= 0.0);
+    dg = fmax(dg, a[k * (k + 1) / 2 + k]);
+  }
+  const double piv_min = 1e-6 * dg + 1e-300;
+  double L[K][K], di[K];
+  bool pd = true;
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    double s = fr[c] ? a[c * (c + 1) / 2 + c] : 1.0;
+#pragma unroll
+    for (int t = 0; t < c; ++t) s -= L[c][t] * L[c][t];
+    const bool ok = !fr[c] || s > piv_min;
+    pd = pd && ok;
+    s = ok ? s : 1.0;
+    const double sq = sqrt(s);
+    di[c] = 1.0 / sq;
+    L[c][c] = sq;
+#pragma unroll
+    for (int rr = c + 1; rr < K; ++rr) {
+      double v = (fr[rr] && fr[c]) ? a[rr * (rr + 1) / 2 + c] : 0.0;
+#pragma unroll
+      for (int t = 0; t < c; ++t) v -= L[rr][t] * L[c][t];
+      L[rr][c] = v * di[c];
+    }
+  }
+  double bad = pd ? 0.0 : 1.0;
+  bad = fmax(bad, dpp_f<0xB1>(bad));
+  bad = fmax(bad, dpp_f<0x4E>(bad));
+  if ((jh & 3) == 0) {
+    const int64_t j = jh >> 2;
+    flags[j] = bad > 0.0 ? 1 : 0;
+    if (bad > 0.0) dense[1 + atomicAdd(dense, 1)] = (int)j;
   }
 }
 
@@ -1195,17 +1241,22 @@ hipError_t launch_reduce_q_curv(const double* part, int nchunk, int K, const Pas
   return hipGetLastError();
 }
 
-hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* res, hipStream_t s) {
+hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* res, unsigned char* flags, int* dense,
+                           hipStream_t s) {
+  if (flags) {
+    const hipError_t e = launch_set_int(dense, 0, s);
+    if (e != hipSuccess) return e;
+  }
   const dim3 grid((unsigned)((4 * g.J + 255) / 256));
   switch (K) {
-    case 1: hipLaunchKernelGGL(k_finish_p<1>, grid, dim3(256), 0, s, g.J, p, res); break;
-    case 2: hipLaunchKernelGGL(k_finish_p<2>, grid, dim3(256), 0, s, g.J, p, res); break;
-    case 3: hipLaunchKernelGGL(k_finish_p<3>, grid, dim3(256), 0, s, g.J, p, res); break;
-    case 4: hipLaunchKernelGGL(k_finish_p<4>, grid, dim3(256), 0, s, g.J, p, res); break;
-    case 5: hipLaunchKernelGGL(k_finish_p<5>, grid, dim3(256), 0, s, g.J, p, res); break;
-    case 6: hipLaunchKernelGGL(k_finish_p<6>, grid, dim3(256), 0, s, g.J, p, res); break;
-    case 7: hipLaunchKernelGGL(k_finish_p<7>, grid, dim3(256), 0, s, g.J, p, res); break;
-    default: hipLaunchKernelGGL(k_finish_p<8>, grid, dim3(256), 0, s, g.J, p, res); break;
+    case 1: hipLaunchKernelGGL(k_finish_p<1>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
+    case 2: hipLaunchKernelGGL(k_finish_p<2>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
+    case 3: hipLaunchKernelGGL(k_finish_p<3>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
+    case 4: hipLaunchKernelGGL(k_finish_p<4>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
+    case 5: hipLaunchKernelGGL(k_finish_p<5>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
+    case 6: hipLaunchKernelGGL(k_finish_p<6>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
+    case 7: hipLaunchKernelGGL(k_finish_p<7>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
+    default: hipLaunchKernelGGL(k_finish_p<8>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
   }
   return hipGetLastError();
 }

@@ -98,6 +98,14 @@ __device__ __forceinline__ void proj_simplex(double (&v)[N], double eps) {
 // Returns the iteration count (>= 0) or a negative code; x holds the solution on lanes < N.
 // lds must hold N*N doubles.
 // ------------------------------------------------------------------------------------------------
+// Orders this wavefront's LDS accesses around the point (its lanes share a private buffer): the LDS
+// executes one wavefront's operations in order, the fences keep the compiler from moving them.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int N, int M>
 __device__ int wave_qp(const double (&Hrow)[N], double g, double lo, double hi, int grp, double& x_out,
                        double* lds) {
@@ -153,14 +161,15 @@ __device__ int wave_qp(const double (&Hrow)[N], double g, double lo, double hi, 
       if (ok) break;
     }
     if (!ok) { code = -2; break; }
-    // Transpose L through LDS so lane r also holds column r (LT[k] = L[k][r]).
-    __syncthreads();
+    // Transpose L through LDS so lane r also holds column r (LT[k] = L[k][r]).  The buffer belongs to
+    // this wavefront alone (a workgroup may run several QPs, one per wavefront).
+    wave_lds_sync();
     if (act) {
 #pragma unroll
       for (int c = 0; c < N; ++c)
         if (c <= lane) lds[lane * N + c] = R[c];
     }
-    __syncthreads();
+    wave_lds_sync();
     double LT[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) LT[k] = (act && k >= lane) ? lds[k * N + lane] : 0.0;
@@ -343,7 +352,7 @@ __device__ __forceinline__ void p_step_dense(int64_t j, const double* __restrict
   constexpr bool penalty = PEN;
   constexpr int NT = K + K * (K + 1) / 2;
   constexpr int N = 4 * K;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const bool act = lane < N;
   const int l = act ? lane / K : 0, k = act ? lane % K : 0;
   const double* r = res + (4 * j + l) * NT;
@@ -481,13 +490,26 @@ __device__ __forceinline__ void small_spd_solve(const double (&S)[K][K], double 
 template <int K, bool PEN, bool MULTI>
 __device__ __forceinline__ void p_step4_body(int64_t J, const double* __restrict__ res, const double* __restrict__ p,
                                              double* __restrict__ p_next, double rho, double eps,
-                                             double* __restrict__ dp, int* __restrict__ redo) {
+                                             double* __restrict__ dp, const unsigned char* __restrict__ flags,
+                                             const int* __restrict__ dense_list, int ndb, int* __restrict__ fails,
+                                             int* __restrict__ late) {
   constexpr int NT = K + K * (K + 1) / 2;
   constexpr int maxit = 4 * (4 * K) + 20;  // as wave_qp with N = 4K
   constexpr int NKK = K * (K + 1) / 2;
+  // The first ndb workgroups solve the early dense list, one QP per wavefront at a time, while the others
+  // run the four-lane QPs: both start together in one launch.
+  __shared__ double dlds[4][4 * K * 4 * K];
+  if ((int)blockIdx.x < ndb) {
+    const int w = threadIdx.x >> 6;
+    const int n = dense_list[0];
+    for (int idx = blockIdx.x * 4 + w; idx < n; idx += ndb * 4)
+      p_step_dense<K, PEN>(dense_list[1 + idx], res, p, p_next, rho, eps, fails, dp, dlds[w]);
+    return;
+  }
+  const int64_t blk = (int64_t)blockIdx.x - ndb;
   const int l = threadIdx.x & 3;
-  const int64_t nslot = ((int64_t)gridDim.x * blockDim.x) >> 2;
-  int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+  const int64_t nslot = ((int64_t)(gridDim.x - ndb) * blockDim.x) >> 2;
+  int64_t j = (blk * blockDim.x + threadIdx.x) >> 2;
   bool valid = j < J;
   double A[NKK];  // packed lower triangle, A[pk(k, k2)]
   double g[K], lo[K], hi[K], pv[K], x[K];
@@ -495,9 +517,14 @@ __device__ __forceinline__ void p_step4_body(int64_t J, const double* __restrict
   double lam[K];  // equality-mode multipliers of the last Newton step
   double a = 0.0, tol = 0.0, piv_min = 0.0;
   int it = 0;
-  bool done = false, dense = false;
+  bool done = false, dense = false, skip = false;
   // quad-uniform: all four lanes of a quad take the same branches
-  auto load = [&]() {
+  auto load = [&]() __attribute__((always_inline)) {
+    skip = flags[j] != 0;  // on the early dense list: solved by the dense kernel
+    dense = false;
+    it = 0;
+    done = skip;
+    if (skip) return;
     const double* r = res + (4 * j + l) * NT;
     #pragma unroll
     for (int t = 0; t < NKK; ++t) A[t] = r[K + t];
@@ -527,12 +554,9 @@ __device__ __forceinline__ void p_step4_body(int64_t J, const double* __restrict
     tol = 1e-11 * (gs + hm * bm) + 1e-300;
     // pivots below this share of the block's largest diagonal send the QP to the dense solver
     piv_min = 1e-6 * dg + 1e-300;
-    it = 0;
-    done = false;
-    dense = false;
   };
   // one active-set iteration (sets done when the QP is solved or handed to the dense solver)
-  auto iterate = [&]() {
+  auto iterate = [&]() __attribute__((always_inline)) {
       bool fr[K];
       #pragma unroll
       for (int k = 0; k < K; ++k) fr[k] = st[k] == 0;
@@ -712,9 +736,10 @@ __device__ __forceinline__ void p_step4_body(int64_t J, const double* __restrict
         if (iv == l * K + k) st[k] = 0;
   };
   // result of a finished quad: the dense list, or p_next = proj(p + dp)
-  auto finish = [&]() {
+  auto finish = [&]() __attribute__((always_inline)) {
+    if (skip) return;
     if (dense) {
-      if (l == 0) redo[1 + atomicAdd(redo, 1)] = (int)j;
+      if (l == 0) late[1 + atomicAdd(late, 1)] = (int)j;
       return;
     }
     if (dp) {
@@ -771,15 +796,18 @@ __device__ __forceinline__ void p_step4_body(int64_t J, const double* __restrict
 template <int K, bool PEN, bool MULTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p_step4_w2(
     int64_t J, const double* __restrict__ res, const double* __restrict__ p, double* __restrict__ p_next, double rho,
-    double eps, double* __restrict__ dp, int* __restrict__ redo) {
-  p_step4_body<K, PEN, MULTI>(J, res, p, p_next, rho, eps, dp, redo);
+    double eps, double* __restrict__ dp, const unsigned char* __restrict__ flags, const int* __restrict__ dense_list,
+    int ndb, int* __restrict__ fails, int* __restrict__ late) {
+  p_step4_body<K, PEN, MULTI>(J, res, p, p_next, rho, eps, dp, flags, dense_list, ndb, fails, late);
 }
 template <int K, bool PEN, bool MULTI>
 __global__ __launch_bounds__(256) void k_p_step4(int64_t J, const double* __restrict__ res,
                                                  const double* __restrict__ p, double* __restrict__ p_next,
                                                  double rho, double eps, double* __restrict__ dp,
-                                                 int* __restrict__ redo) {
-  p_step4_body<K, PEN, MULTI>(J, res, p, p_next, rho, eps, dp, redo);
+                                                 const unsigned char* __restrict__ flags,
+                                                 const int* __restrict__ dense_list, int ndb, int* __restrict__ fails,
+                                                 int* __restrict__ late) {
+  p_step4_body<K, PEN, MULTI>(J, res, p, p_next, rho, eps, dp, flags, dense_list, ndb, fails, late);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1081,24 +1109,37 @@ static unsigned resident_blocks(F kernel, int block) {
   return (unsigned)(per_cu * cus);
 }
 
-// Four lanes per block QP (k_p_step4, several quads per slot); blocks whose free curvature is not positive
-// definite go through redo ([count, list], J + 1 ints) to the dense one-wave solver, whose grid strides
-// over the list (empty in the usual case).
+// One launch for the P row QPs: the first NDB workgroups run the dense one-wave solver over the early list
+// (blocks launch_finish_p marked), the rest the four-lane QPs of all other blocks (several quads per slot
+// when J exceeds the slots the device holds), so the dense solver's long active-set loops overlap the
+// four-lane work.  Blocks the four-lane kernel finds not positive definite itself (rare: the early test is
+// the same) go to the late list, solved after it.
 hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
-                         int* fails, int* redo, hipStream_t s, double* dp) {
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, redo, 0);
+                         int* fails, const PStepLists& lists, hipStream_t s, double* dp) {
+  hipError_t e = launch_set_int(lists.late, 0, s);
+  if (e != hipSuccess) return e;
+  constexpr int NDB = 64;  // 256 wavefronts for the early list (it holds a few blocks in the usual case)
   const unsigned nq = (unsigned)((4 * J + 255) / 256), nd = (unsigned)(J < 1024 ? J : 1024);
   return with_k(K, [&](auto kc) {
     constexpr int KT = decltype(kc)::value;
     auto run = [&](auto single, auto multi, auto list_kernel, unsigned cap) {
-      if (nq <= cap)
-        hipLaunchKernelGGL(single, dim3(nq), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp, redo);
+      // the dense workgroups exit at once when their list is empty (the usual case), so the four-lane grid
+      // keeps the whole device's worth of slots
+      const unsigned room = cap;
+      if (nq <= room)
+        hipLaunchKernelGGL(single, dim3(NDB + nq), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp,
+                           lists.flags, lists.dense, NDB, fails, lists.late);
       else
-        hipLaunchKernelGGL(multi, dim3(cap), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp, redo);
-      hipLaunchKernelGGL(list_kernel, dim3(nd), dim3(64), 0, s, res_p, p, p_next, o.rho, o.eps, fails, dp, redo);
+        hipLaunchKernelGGL(multi, dim3(NDB + room), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp,
+                           lists.flags, lists.dense, NDB, fails, lists.late);
+      hipLaunchKernelGGL(list_kernel, dim3(nd), dim3(64), 0, s, res_p, p, p_next, o.rho, o.eps, fails, dp,
+                         lists.late);
     };
     if (o.penalty) {
-      if constexpr (KT <= 6) {
+      if constexpr (KT == 6) {  // the one-quad-per-slot form would spill at 256 registers: the striding one for both
+        static const unsigned cap = resident_blocks(k_p_step4_w2<KT, true, true>, 256);
+        run(k_p_step4_w2<KT, true, true>, k_p_step4_w2<KT, true, true>, k_p_step_list<KT, true>, cap);
+      } else if constexpr (KT <= 5) {
         static const unsigned cap = resident_blocks(k_p_step4_w2<KT, true, true>, 256);
         run(k_p_step4_w2<KT, true, false>, k_p_step4_w2<KT, true, true>, k_p_step_list<KT, true>, cap);
       } else {
