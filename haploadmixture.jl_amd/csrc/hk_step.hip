@@ -394,7 +394,7 @@ __device__ __forceinline__ void p_step_dense(int64_t j, const double* __restrict
   }
 }
 
-// Dense solves of the blocks listed in redo[1 .. redo[0]] (grid-stride over the list).
+// Dense solves of the blocks listed in redo[1 .. redo[0]] (grid-stride over the list): the late list.
 template <int K, bool PEN>
 __global__ __launch_bounds__(64) void k_p_step_list(const double* __restrict__ res, const double* __restrict__ p,
                                                     double* __restrict__ p_next, double rho, double eps,
@@ -419,7 +419,9 @@ __global__ __launch_bounds__(64) void k_p_step_list(const double* __restrict__ r
 //     penalty:   (B_F + rho E E')^-1 r = y - C t,     (I / rho + sum_l C_l) t = sum_l y_l   (Woodbury)
 //     equality:  d = -(y + C lam),                    (sum_l C_l) lam = -sum_l y_l          (Schur)
 // (sums over the quad by DPP).  A quad with a free block that is not positive definite (a haplotype
-// compatible with too few individuals) is appended to `redo` and solved by the one-wave dense kernel.
+// compatible with too few individuals) is solved by the one-wave dense solver instead: the P-side
+// finishing kernel lists those it can tell from the first active-set iteration (flags, early list, solved
+// by the first workgroups of the same launch); any other lands on the late list, solved after it.
 // ------------------------------------------------------------------------------------------------
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
