@@ -146,8 +146,7 @@ static int pass_params(hk_ctx* c, const double* q, const double* p, const void**
     return HK_OK;
   }
   const int64_t nq = (int64_t)c->K * c->I, np = (int64_t)c->K * 4 * c->g.Jp;
-  HIPC(c, launch_to_f32(q, c->x32, nq, c->s));
-  HIPC(c, launch_to_f32(p, c->x32 + nq, np, c->s));
+  HIPC(c, launch_pair_to_f32(q, nq, p, np, c->x32, c->s));
   *qq = c->x32;
   *pp = c->x32 + nq;
   return HK_OK;

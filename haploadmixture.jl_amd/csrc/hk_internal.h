@@ -80,7 +80,7 @@ hipError_t launch_em_q(int K, int64_t I, int64_t J, const double* res_q, int NT,
                        double* q_next, double eps, hipStream_t s);
 hipError_t launch_em_p(int K, int64_t J, const double* res_p, int NT, const double* p, double* p_next, double eps,
                        hipStream_t s);
-hipError_t launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
+hipError_t launch_pair_to_f32(const double* q, int64_t nq, const double* p, int64_t np, float* dst, hipStream_t s);
 // Quasi-Newton extrapolation of admixture_qn! (mode 0: ZAL, 1: LBQN); dots_part holds 512 x 64 doubles.
 hipError_t launch_zal(int64_t N, int Qn, const int* d_iter, double* U, double* V, const double* x0, const double* x1,
                       const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s, int mode);

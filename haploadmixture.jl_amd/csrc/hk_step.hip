@@ -873,9 +873,12 @@ __global__ void k_em_p(int K, int64_t J, const double* __restrict__ res, int NT,
   for (int h = 0; h < 4; ++h) p_next[k + (int64_t)K * (4 * j + h)] = v[h];
 }
 
-__global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
+// FP32 mirror of a (q, p) pair in one launch: dst = [q (nq) | p (np)].
+__global__ void k_pair_to_f32(const double* __restrict__ q, int64_t nq, const double* __restrict__ p, int64_t np,
+                              float* __restrict__ dst) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = (float)src[i];
+  if (i < nq) dst[i] = (float)q[i];
+  else if (i < nq + np) dst[i] = (float)p[i - nq];
 }
 
 __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
@@ -1189,8 +1192,8 @@ hipError_t launch_em_p(int K, int64_t J, const double* res_p, int NT, const doub
   return hipGetLastError();
 }
 
-hipError_t launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+hipError_t launch_pair_to_f32(const double* q, int64_t nq, const double* p, int64_t np, float* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_to_f32, dim3((unsigned)((nq + np + 255) / 256)), dim3(256), 0, s, q, nq, p, np, dst);
   return hipGetLastError();
 }
 
