@@ -156,6 +156,9 @@ def warmstart_bench(a, w, prec):
         sd = ws.simulate_world(I, J, K, fst, miss=w["miss"], data_seed=seed, truth=True)
         with HipAdmixData(sd.bed, I, J, K, precision=prec) as e:
             q0, p0 = init_params(I, J, K, seed=11)
+            # set-up, untimed like the context's: the network's projection scratch for this panel size
+            # (~250 MB at I=10k) and the seeded projection matrix are made on its first prediction
+            ws.predict_q(e, mlp)
             plain_runs, warm_runs = [], []
             for _ in range(a.steps):
                 e.set_params(q0, p0)
