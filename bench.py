@@ -304,7 +304,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {a.gpus} is not supported by this build "
-              "(the sharded design over J is described in DESIGN.md)", file=sys.stderr)
+              "(a split over SNP blocks is the multi-GPU design; this build runs one GPU)", file=sys.stderr)
         sys.exit(2)
     w = dict(WORKLOADS[a.workload])
     prec = a.prec or w["prec"]

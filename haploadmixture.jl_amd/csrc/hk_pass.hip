@@ -944,7 +944,7 @@ __global__ void k_xtz_q(int K, int64_t I, const double* __restrict__ q, double* 
 // diagonal, the test k_p_step4 applies at its first active-set iteration, whose free set p fixes (and
 // free subsets only raise the pivots).  They are listed in dense[1 .. dense[0]] so that the dense solver
 // can run alongside the four-lane kernel instead of after it.  Lane = (block, haplotype): the four
-// lanes of a block are adjacent (4J is a multiple of 4), so the quad's verdict is a DPP maximum.
+// lanes of a block are adjacent (4J is a multiple of 4), so the quad's result is a DPP maximum.
 template <int K>
 __global__ void k_finish_p(int64_t J, const double* __restrict__ p, double* __restrict__ res,
                            unsigned char* __restrict__ flags, int* __restrict__ dense) {
