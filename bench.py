@@ -303,8 +303,8 @@ def main():
                     help="after the timed fits, run a plain copy kernel of this size (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on exactly one GPU; --gpus {a.gpus} is not supported by this build "
-              "(a split over SNP blocks is the multi-GPU design; this build runs one GPU)", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU; --gpus {a.gpus} is not supported (the engine is single-GPU; "
+              "a split over SNP blocks with one all-reduce per update_q pass is its multi-GPU design)", file=sys.stderr)
         sys.exit(2)
     w = dict(WORKLOADS[a.workload])
     prec = a.prec or w["prec"]

@@ -338,7 +338,7 @@ int hk_ctx_create(hk_ctx** out, int64_t I, int64_t J, int K, const uint8_t* bed,
                 "bytes_per_snp >= ceil(I/4), n_snp >= 2J)",
                 (long long)I, (long long)J, K, (long long)bps, (long long)n_snp, precision, HK_MAX_K);
   if (I > (int64_t)1 << 31 || J > (int64_t)1 << 29)
-    return fail(nullptr, HK_EINVAL, "I or J too large for this build");
+    return fail(nullptr, HK_EINVAL, "I or J too large (I <= 2^31, J <= 2^29)");
   hk_ctx* c = new hk_ctx();
   c->I = I;
   c->J = J;
