@@ -19,7 +19,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib")
 ROOT = os.path.dirname(HERE)
 
-HIP_SOURCES = ["hk_pass.hip", "hk_step.hip", "hk_api.hip", "hk_mlp.hip", "hk_skfr.hip"]
+# every HIP translation unit of csrc/ goes into the one library
+HIP_SOURCES = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 VALU forms are disabled for every HIP translation unit.

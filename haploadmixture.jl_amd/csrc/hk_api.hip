@@ -16,50 +16,6 @@
 
 using namespace hk;
 
-struct hk_ctx {
-  int64_t I = 0, J = 0;
-  int K = 0, prec = 0, dev = 0;
-  PassGeom g{};
-  hipStream_t s = nullptr;
-  uint32_t* gq = nullptr;
-  uint32_t* gp = nullptr;
-  void* qq = nullptr;
-  void* pp = nullptr;
-  int64_t* dm = nullptr;
-  double* xb[4] = {nullptr, nullptr, nullptr, nullptr};  // parameter buffers [q | p | pad]
-  int64_t xlen = 0;                                      // K * (I + 4 Jp)
-  float* x32 = nullptr;                                  // FP32 mirror of one parameter buffer
-  double* res_q = nullptr;   // [I][NT]
-  double* res_p = nullptr;   // [4Jp][NT]
-  double* part_q = nullptr;  // [qchunks][NT][Ip]
-  double* part_p = nullptr;  // [pchunks][4Jp][NT]
-  double* ll_dev = nullptr;  // scalar
-  double* ll_rows = nullptr; // [I]
-  double* U = nullptr;
-  double* V = nullptr;
-  int Qn_alloc = 0;
-  double* zal_part = nullptr;
-  double* zal_coef = nullptr;
-  int* fails = nullptr;
-  int* redo = nullptr;       // J + 1 ints: early list of the blocks for the dense P-step solver
-  int* redo_late = nullptr;  // J + 1 ints: blocks the four-lane P-step kernel hands over itself
-  unsigned char* pflag = nullptr;  // J: block on the early list
-  double* full_a = nullptr;  // ABI expansion buffers
-  double* full_b = nullptr;
-  int64_t full_len = 0;
-  int qchunks_req = 0, pchunks_req = 0;
-  // fit state for the sync-free iteration: ll[0] ll_basic, ll[1] ll_qn, ll[2] ll_new, ll[3] scratch
-  double* d_ll = nullptr;
-  int* d_iter = nullptr;
-  double* h_ll = nullptr;  // pinned mirror of d_ll
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  unsigned long long graph_key = 0;
-  bool owns_geno = true;
-  int64_t part_q_len = 0, part_p_len = 0;
-  char err[512] = {0};
-};
-
 static char g_create_err[512];
 
 static int fail(hk_ctx* c, int code, const char* fmt, ...) {
@@ -261,7 +217,8 @@ int hk_ctx_destroy(hk_ctx* c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   void* owned[] = {c->qq, c->pp, c->xb[0], c->xb[1], c->xb[2], c->xb[3], c->x32, c->res_q, c->res_p,
                    c->part_q, c->part_p, c->ll_dev, c->ll_rows, c->U, c->V, c->zal_part, c->zal_coef, c->fails,
-                   c->redo, c->redo_late, c->pflag, c->full_a, c->full_b, c->d_ll, c->d_iter};
+                   c->redo, c->redo_late, c->pflag, c->full_a, c->full_b, c->d_ll, c->d_iter,
+                   c->named[0], c->named[1], c->named[2], c->named[3]};
   for (void* p : owned)
     if (p) (void)hipFree(p);
   if (c->owns_geno) {
@@ -570,6 +527,154 @@ int hk_p_step(hk_ctx* c, const double* q_pass, int penalty, double rho, double* 
   HIPC(c, hipMemcpyAsync(p_next, pof(c, xn), sizeof(double) * c->K * 4 * c->J, hipMemcpyDeviceToHost, c->s));
   HIPC(c, hipStreamSynchronize(c->s));
   if (nf) return fail(c, HK_EQP, "%d row QPs failed", nf);
+  return HK_OK;
+}
+
+// ---- runner level: the device arrays of CuAdmixData by name --------------------------------------
+
+// Device address and length (doubles) of a named array; allocates XtX / Xtz arrays on first use.
+static int named_buf(hk_ctx* c, int id, double** ptr, int64_t* len) {
+  const int64_t K = c->K, I = c->I, J4 = 4 * c->J;
+  switch (id) {
+    case HK_BUF_Q: *ptr = qof(c, c->xb[0]); *len = K * I; return HK_OK;
+    case HK_BUF_P: *ptr = pof(c, c->xb[0]); *len = K * J4; return HK_OK;
+    case HK_BUF_Q_NEXT: *ptr = qof(c, c->xb[1]); *len = K * I; return HK_OK;
+    case HK_BUF_P_NEXT: *ptr = pof(c, c->xb[1]); *len = K * J4; return HK_OK;
+    default: break;
+  }
+  if (id < HK_BUF_XTX_Q || id > HK_BUF_XTZ_P) return fail(c, HK_EINVAL, "unknown device array %d", id);
+  const int64_t lens[4] = {K * K * I, K * I, K * K * J4, K * J4};
+  const int t = id - HK_BUF_XTX_Q;
+  if (!c->named[t]) {
+    HIPC(c, hipMalloc(&c->named[t], sizeof(double) * lens[t]));
+    HIPC(c, launch_fill(c->named[t], lens[t], 0.0, c->s));
+  }
+  *ptr = c->named[t];
+  *len = lens[t];
+  return HK_OK;
+}
+
+// Same shape class: q-shaped (K x I), p-shaped (K x 4J), or the same curvature / gradient array kind.
+static int buf_shape(int id) {
+  switch (id) {
+    case HK_BUF_Q: case HK_BUF_Q_NEXT: return 0;
+    case HK_BUF_P: case HK_BUF_P_NEXT: return 1;
+    default: return 2 + id;
+  }
+}
+
+int hk_buf_upload(hk_ctx* c, int id, const double* host) {
+  if (!c || !host) return fail(c, HK_EINVAL, "null pointer");
+  double* d;
+  int64_t n;
+  CHK(named_buf(c, id, &d, &n));
+  if (id <= HK_BUF_P_NEXT)
+    for (int64_t t = 0; t < n; ++t)
+      if (!isfinite(host[t])) return fail(c, HK_ENAN, "array %d has a non-finite value at %lld", id, (long long)t);
+  HIPC(c, hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+  HIPC(c, hipStreamSynchronize(c->s));
+  return HK_OK;
+}
+
+int hk_buf_download(hk_ctx* c, int id, double* host) {
+  if (!c || !host) return fail(c, HK_EINVAL, "null pointer");
+  double* d;
+  int64_t n;
+  CHK(named_buf(c, id, &d, &n));
+  HIPC(c, hipMemcpyAsync(host, d, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+  HIPC(c, hipStreamSynchronize(c->s));
+  return HK_OK;
+}
+
+int hk_buf_copy(hk_ctx* c, int dst, int src) {
+  if (!c) return HK_EINVAL;
+  if (buf_shape(dst) != buf_shape(src)) return fail(c, HK_EINVAL, "arrays %d and %d differ in shape", dst, src);
+  double *a, *b;
+  int64_t na, nb;
+  CHK(named_buf(c, dst, &a, &na));
+  CHK(named_buf(c, src, &b, &nb));
+  if (a != b) HIPC(c, launch_copy_f64(na, b, a, c->s));
+  HIPC(c, hipStreamSynchronize(c->s));
+  return HK_OK;
+}
+
+int hk_loglik_buf(hk_ctx* c, int qbuf, int pbuf, double* ll) {
+  if (!c || !ll) return fail(c, HK_EINVAL, "null pointer");
+  if (buf_shape(qbuf) != 0 || buf_shape(pbuf) != 1) return fail(c, HK_EINVAL, "loglikelihood needs a q and a p array");
+  double *q, *p;
+  int64_t n;
+  CHK(named_buf(c, qbuf, &q, &n));
+  CHK(named_buf(c, pbuf, &p, &n));
+  CHK(run_qpass(c, MODE_LL, q, p));
+  HIPC(c, hipMemcpyAsync(ll, c->ll_dev, sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPC(c, hipStreamSynchronize(c->s));
+  return HK_OK;
+}
+
+int hk_em_buf(hk_ctx* c, int fix_q) {
+  if (!c) return HK_EINVAL;
+  double* q = qof(c, c->xb[0]);
+  double* p = pof(c, c->xb[0]);
+  if (!fix_q) {
+    CHK(run_qpass(c, MODE_GRAD, q, p));
+    HIPC(c, launch_em_expand(c->res_q, c->K, c->K, c->I, q, qof(c, c->xb[1]), c->s));
+  }
+  CHK(run_ppass(c, MODE_GRAD, q, p));
+  HIPC(c, launch_em_expand(c->res_p, c->K, c->K, 4 * c->J, p, pof(c, c->xb[1]), c->s));
+  HIPC(c, hipStreamSynchronize(c->s));
+  return HK_OK;
+}
+
+int hk_update_q_buf(hk_ctx* c) {
+  if (!c) return HK_EINVAL;
+  double *XtX, *Xtz;
+  int64_t n;
+  CHK(named_buf(c, HK_BUF_XTX_Q, &XtX, &n));
+  CHK(named_buf(c, HK_BUF_XTZ_Q, &Xtz, &n));
+  CHK(run_qpass(c, MODE_CURV, qof(c, c->xb[0]), pof(c, c->xb[0])));
+  HIPC(c, launch_expand(c->res_q, c->K, ntq(c->K), c->I, XtX, Xtz, c->s));
+  HIPC(c, hipStreamSynchronize(c->s));
+  return HK_OK;
+}
+
+int hk_update_p_buf(hk_ctx* c) {
+  if (!c) return HK_EINVAL;
+  double *XtX, *Xtz;
+  int64_t n;
+  CHK(named_buf(c, HK_BUF_XTX_P, &XtX, &n));
+  CHK(named_buf(c, HK_BUF_XTZ_P, &Xtz, &n));
+  CHK(run_ppass(c, MODE_CURV, qof(c, c->xb[0]), pof(c, c->xb[0])));
+  HIPC(c, launch_expand(c->res_p, c->K, ntq(c->K), 4 * c->J, XtX, Xtz, c->s));
+  HIPC(c, hipStreamSynchronize(c->s));
+  return HK_OK;
+}
+
+// count_double_missing!(doublemissing, g) as src/structs.jl:169 calls it: over the SNP pairs of the whole
+// file, not only the fit's J blocks.  The columns past the first 2J are streamed through a bounded
+// staging buffer and counted on the device; counts of the first J blocks come from the packed stream.
+int hk_count_double_missing_file(hk_ctx* c, const uint8_t* bed, int64_t bps, int64_t n_snp) {
+  if (!c || !bed) return fail(c, HK_EINVAL, "null pointer");
+  if (!c->owns_geno) return fail(c, HK_EINVAL, "call on the context that owns the genotype upload");
+  if (bps < (c->I + 3) / 4 || n_snp < 2 * c->J)
+    return fail(c, HK_EINVAL, "bytes_per_snp %lld < ceil(I/4) or n_snp %lld < 2J", (long long)bps, (long long)n_snp);
+  HIPC(c, launch_count_dm(c->g, c->dm, c->s));
+  const int64_t nblk = n_snp / 2;  // complete SNP pairs (an odd last SNP has no partner)
+  const int64_t chunk = (int64_t)1 << 14;  // blocks per staging round: 2 * 16384 * bps bytes
+  uint8_t* raw = nullptr;
+  if (nblk > c->J) {
+    const int64_t cb = nblk - c->J < chunk ? nblk - c->J : chunk;
+    HIPC(c, hipMalloc(&raw, (size_t)(2 * cb * bps)));
+  }
+  hipError_t e = hipSuccess;
+  for (int64_t b0 = c->J; b0 < nblk && e == hipSuccess; b0 += chunk) {
+    const int64_t nb = nblk - b0 < chunk ? nblk - b0 : chunk;
+    e = hipMemcpyAsync(raw, bed + 2 * b0 * bps, (size_t)(2 * nb * bps), hipMemcpyHostToDevice, c->s);
+    if (e == hipSuccess) e = launch_count_dm_raw(raw, bps, c->I, nb, c->dm, c->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s);  // the staging buffer is reused next round
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+  if (raw) (void)hipFree(raw);
+  HIPC(c, e);
   return HK_OK;
 }
 

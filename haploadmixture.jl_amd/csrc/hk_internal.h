@@ -32,6 +32,8 @@ void stage_sizes(int K, int64_t Ip, int64_t Jp, int64_t* nq, int64_t* np);
 
 hipError_t launch_repack(const uint8_t* raw, int64_t bps, const PassGeom& g, hipStream_t s);
 hipError_t launch_count_dm(const PassGeom& g, int64_t* dm, hipStream_t s);
+// Adds the double-missing pairs of nb blocks held as 2 nb raw BED columns (bps bytes each) to dm.
+hipError_t launch_count_dm_raw(const uint8_t* raw, int64_t bps, int64_t I, int64_t nb, int64_t* dm, hipStream_t s);
 hipError_t launch_decode(const PassGeom& g, uint8_t* codes, hipStream_t s);
 // MODE_CURV passes read staging tables built from the same (q, p) by launch_curv_tables (side 0: the
 // Q-side pass, side 1: the P-side pass), which must be enqueued first.
@@ -109,3 +111,51 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
              double* prof, hipStream_t s, char* err, size_t errlen);
 
 }  // namespace hk
+
+// Context of one fit (the C ABI's hk_ctx): genotype streams, parameters, pass buffers and the stream.
+struct hk_ctx {
+  int64_t I = 0, J = 0;
+  int K = 0, prec = 0, dev = 0;
+  hk::PassGeom g{};
+  hipStream_t s = nullptr;
+  uint32_t* gq = nullptr;
+  uint32_t* gp = nullptr;
+  void* qq = nullptr;
+  void* pp = nullptr;
+  int64_t* dm = nullptr;
+  double* xb[4] = {nullptr, nullptr, nullptr, nullptr};  // parameter buffers [q | p | pad]
+  int64_t xlen = 0;                                      // K * (I + 4 Jp)
+  float* x32 = nullptr;                                  // FP32 mirror of one parameter buffer
+  double* res_q = nullptr;   // [I][NT]
+  double* res_p = nullptr;   // [4Jp][NT]
+  double* part_q = nullptr;  // [qchunks][NT][Ip]
+  double* part_p = nullptr;  // [pchunks][4Jp][NT]
+  double* ll_dev = nullptr;  // scalar
+  double* ll_rows = nullptr; // [I]
+  double* U = nullptr;
+  double* V = nullptr;
+  int Qn_alloc = 0;
+  double* zal_part = nullptr;
+  double* zal_coef = nullptr;
+  int* fails = nullptr;
+  int* redo = nullptr;       // J + 1 ints: early list of the blocks for the dense P-step solver
+  int* redo_late = nullptr;  // J + 1 ints: blocks the four-lane P-step kernel hands over itself
+  unsigned char* pflag = nullptr;  // J: block on the early list
+  double* full_a = nullptr;  // ABI expansion buffers
+  double* full_b = nullptr;
+  int64_t full_len = 0;
+  // Runner-level device arrays of CuAdmixData (cuda/structs.jl:1-15) beyond x / x_next: XtX_q, Xtz_q,
+  // XtX_p, Xtz_p in the reference's full layouts, allocated on first use.
+  double* named[4] = {nullptr, nullptr, nullptr, nullptr};
+  int qchunks_req = 0, pchunks_req = 0;
+  // fit state for the sync-free iteration: ll[0] ll_basic, ll[1] ll_qn, ll[2] ll_new, ll[3] scratch
+  double* d_ll = nullptr;
+  int* d_iter = nullptr;
+  double* h_ll = nullptr;  // pinned mirror of d_ll
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  unsigned long long graph_key = 0;
+  bool owns_geno = true;
+  int64_t part_q_len = 0, part_p_len = 0;
+  char err[512] = {0};
+};

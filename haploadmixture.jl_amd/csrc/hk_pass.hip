@@ -105,6 +105,24 @@ __global__ void k_count_dm(const uint32_t* __restrict__ gq, int64_t Ip, int64_t 
   if (n) atomicAdd(&dm[i], (unsigned long long)n);
 }
 
+// Double-missing counts of SNP pairs straight from BED columns (the part of a file past the fit's blocks):
+// raw holds nb blocks = 2 nb SNP columns of bps bytes; thread (i, chunk of DM_CW blocks) adds its count to
+// dm[i] with an integer atomic (exact in any order).
+__global__ void k_count_dm_raw(const uint8_t* __restrict__ raw, int64_t bps, int64_t I, int64_t nb,
+                               unsigned long long* __restrict__ dm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I) return;
+  const int64_t b0 = (int64_t)blockIdx.y * DM_CW, b1 = b0 + DM_CW < nb ? b0 + DM_CW : nb;
+  const unsigned sh = (unsigned)(i & 3) * 2u;
+  unsigned n = 0;
+  for (int64_t b = b0; b < b1; ++b) {
+    const unsigned c1 = (raw[(2 * b) * bps + (i >> 2)] >> sh) & 3u;
+    const unsigned c2 = (raw[(2 * b + 1) * bps + (i >> 2)] >> sh) & 3u;
+    n += (c1 == 1u && c2 == 1u) ? 1u : 0u;
+  }
+  if (n) atomicAdd(&dm[i], (unsigned long long)n);
+}
+
 // Decoded genotype readback (bit-exact decode check): g[s * I + i] = BED code of SNP s.
 __global__ void k_decode(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J,
                          uint8_t* __restrict__ codes) {
@@ -1216,6 +1234,13 @@ hipError_t launch_count_dm(const PassGeom& g, int64_t* dm, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((g.I + 255) / 256)), dim3(256), 0, s, d, g.I);
   hipLaunchKernelGGL(k_count_dm, dim3((unsigned)((g.I + 255) / 256), (unsigned)((g.J8 + DM_CW - 1) / DM_CW)), dim3(256), 0,
                      s, g.gq, g.Ip, g.I, g.J, g.J8, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_dm_raw(const uint8_t* raw, int64_t bps, int64_t I, int64_t nb, int64_t* dm, hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_dm_raw, dim3((unsigned)((I + 255) / 256), (unsigned)((nb + DM_CW - 1) / DM_CW)), dim3(256),
+                     0, s, raw, bps, I, nb, (unsigned long long*)dm);
   return hipGetLastError();
 }
 

@@ -99,6 +99,14 @@ def hip():
     L.hk_update_p.argtypes = [_vp, _vp, _vp]
     L.hk_double_missing.argtypes = [_vp, _vp]
     L.hk_decode.argtypes = [_vp, _vp]
+    L.hk_count_double_missing_file.argtypes = [_vp, _vp, _i64, _i64]
+    L.hk_buf_upload.argtypes = [_vp, _int, _vp]
+    L.hk_buf_download.argtypes = [_vp, _int, _vp]
+    L.hk_buf_copy.argtypes = [_vp, _int, _int]
+    L.hk_loglik_buf.argtypes = [_vp, _int, _int, ctypes.POINTER(_dbl)]
+    L.hk_em_buf.argtypes = [_vp, _int]
+    L.hk_update_q_buf.argtypes = [_vp]
+    L.hk_update_p_buf.argtypes = [_vp]
     L.hk_q_step.argtypes = [_vp, _int, _dbl, _vp]
     L.hk_p_step.argtypes = [_vp, _vp, _int, _dbl, _vp]
     L.hk_fit_opts_default.argtypes = [ctypes.POINTER(FitOpts)]

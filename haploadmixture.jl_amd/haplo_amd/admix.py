@@ -45,9 +45,13 @@ class AdmixData2:
 def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iters=5, T=np.float64,
                     T2=np.float64, Q=3, use_gpu=True, verbose=False, fix_q=False, fix_p=False,
                     init_q=None, init_p=None, engine_opts=None, warm_start=False, guarantee_increase=False,
-                    mode="ZAL"):
+                    mode="ZAL", doublemissing="file"):
     """src/driver.jl:133-201.  guarantee_increase and mode (:ZAL / :LBQN) are admixture_qn!'s keywords
-    (algorithms_outer.jl:39-41)."""
+    (algorithms_outer.jl:39-41).  doublemissing: "file" (the reference: AdmixData2 counts double-missing
+    pairs over every SNP pair of the file, structs.jl:169, and the EM normaliser 2(J - dm) uses them even
+    when the fit reads only the first 2J SNPs) or "fit" (over the fit's J blocks only)."""
+    if doublemissing not in ("file", "fit"):
+        raise ValueError("doublemissing must be 'file' or 'fit'")
     if mode not in ("ZAL", "LBQN"):
         raise ValueError("mode must be 'ZAL' or 'LBQN'")
     if not use_gpu:
@@ -64,6 +68,8 @@ def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iter
     eng = HipAdmixData(np.asarray(payload[: 2 * J * bps]), I, J, K, bytes_per_snp=bps, n_snp=2 * J,
                        precision=precision)
     try:
+        if doublemissing == "file" and n_snps > 2 * J:
+            eng.count_double_missing_file(np.asarray(payload), n_snps, bytes_per_snp=bps)
         d.doublemissing = eng.double_missing()
         eng.set_params(d.q, d.p)
         base = dict(Qn=Q, fix_q=int(fix_q), fix_p=int(fix_p), verbose=int(verbose),
@@ -142,8 +148,9 @@ def _filter_SKFR(filename, K, sparsity, *, rng=1, prefix=None, tries=10, max_inn
 def run_admixture(filename, I, J, K, *, rng=1, sparsity=None, prefix=None, skfr_tries=1, skfr_max_inner_iter=50,
                   skfr_mode="global", admix_n_iter=1000, admix_rtol=1e-7, admix_n_em_iters=5, T=np.float64,
                   T2=np.float64, Q=3, use_gpu=True, verbose=False, progress_bar=False, fix_q=False, fix_p=False,
-                  init_q=None, init_p=None, warm_start=False, guarantee_increase=False):
-    """src/driver.jl:38-79.  Returns (d, clusters, aims); clusters/aims are None without SKFR."""
+                  init_q=None, init_p=None, warm_start=False, guarantee_increase=False, doublemissing="file"):
+    """src/driver.jl:38-79.  Returns (d, clusters, aims); clusters/aims are None without SKFR.
+    doublemissing: see _admixture_base ("file" = the reference's whole-file counts)."""
     if not str(filename).endswith(".bed"):
         raise ValueError("filename should end with .bed")
     print(f"Using {filename} as input.")
@@ -161,5 +168,5 @@ def run_admixture(filename, I, J, K, *, rng=1, sparsity=None, prefix=None, skfr_
     d = _admixture_base(admix_input, I, J, K, n_iter=admix_n_iter, rtol=admix_rtol, rng=rng,
                         em_iters=admix_n_em_iters, T=T, T2=T2, Q=Q, use_gpu=use_gpu, verbose=verbose,
                         fix_q=fix_q, fix_p=fix_p, init_q=init_q, init_p=init_p, warm_start=warm_start,
-                        guarantee_increase=guarantee_increase)
+                        guarantee_increase=guarantee_increase, doublemissing=doublemissing)
     return d, clusters, aims

@@ -138,6 +138,56 @@ class HipAdmixData:
         self._check(self._L.hk_decode(self._h, ptr(codes)))
         return codes
 
+    def count_double_missing_file(self, bed: np.ndarray, n_snp: int, bytes_per_snp: int | None = None):
+        """count_double_missing!(dm, g) over every SNP pair of the whole file (structs.jl:169 -> utils.jl:1-4),
+        as the reference's EM normaliser uses it; returns the new counts."""
+        bps = (self.I + 3) // 4 if bytes_per_snp is None else int(bytes_per_snp)
+        bed = np.ascontiguousarray(bed, dtype=np.uint8)
+        if bed.size < int(n_snp) * bps:
+            raise ValueError("payload shorter than n_snp * bytes_per_snp")
+        self._check(self._L.hk_count_double_missing_file(self._h, ptr(bed), bps, int(n_snp)))
+        return self.double_missing()
+
+    # -- runner level: the device arrays of CuAdmixData by name (cuda/structs.jl:1-15) --------------
+    BUFFERS = {"q": 0, "p": 1, "q_next": 2, "p_next": 3, "XtX_q": 4, "Xtz_q": 5, "XtX_p": 6, "Xtz_p": 7}
+
+    def buf_shape(self, name: str) -> tuple:
+        K, I, J4 = self.K, self.I, 4 * self.J
+        return {"q": (K, I), "q_next": (K, I), "p": (K, J4), "p_next": (K, J4), "XtX_q": (K, K, I),
+                "Xtz_q": (K, I), "XtX_p": (K, K, J4), "Xtz_p": (K, J4)}[name]
+
+    def buf_upload(self, name: str, a):
+        """copyto_sync!([d_cu.<name>], [a])."""
+        self._check(self._L.hk_buf_upload(self._h, self.BUFFERS[name], ptr(_F(a, self.buf_shape(name)))))
+
+    def buf_download(self, name: str) -> np.ndarray:
+        """copyto_sync!([a], [d_cu.<name>])."""
+        a = np.zeros(self.buf_shape(name), order="F")
+        self._check(self._L.hk_buf_download(self._h, self.BUFFERS[name], ptr(a)))
+        return a
+
+    def buf_copy(self, dst: str, src: str):
+        """d_cu.<dst> .= d_cu.<src>."""
+        self._check(self._L.hk_buf_copy(self._h, self.BUFFERS[dst], self.BUFFERS[src]))
+
+    def loglikelihood_buf(self, q: str = "q", p: str = "p") -> float:
+        """loglikelihood(d_cu, g_cu, d_cu.<q>, d_cu.<p>)."""
+        ll = ctypes.c_double()
+        self._check(self._L.hk_loglik_buf(self._h, self.BUFFERS[q], self.BUFFERS[p], ctypes.byref(ll)))
+        return ll.value
+
+    def em_buf(self, fix_q: bool = False):
+        """em!(d_cu, g_cu) / em_p!(d_cu, g_cu): unnormalised sums into d_cu.q_next, d_cu.p_next."""
+        self._check(self._L.hk_em_buf(self._h, int(bool(fix_q))))
+
+    def update_q_buf(self):
+        """update_q_cuda!(d_cu, g_cu): d_cu.XtX_q, d_cu.Xtz_q from d_cu.q, d_cu.p."""
+        self._check(self._L.hk_update_q_buf(self._h))
+
+    def update_p_buf(self):
+        """update_p_cuda!(d_cu, g_cu): d_cu.XtX_p, d_cu.Xtz_p from d_cu.q, d_cu.p."""
+        self._check(self._L.hk_update_p_buf(self._h))
+
     # -- device-resident row steps ----------------------------------------------------------------
     def q_step(self, penalty=False, rho=1e7):
         qn = np.zeros((self.K, self.I), order="F")

@@ -79,12 +79,55 @@ int hk_update_q(hk_ctx* ctx, double* XtX, double* Xtz);
 /* update_p_cuda!(d_cu, g_cu) (src/cuda/runners.jl:76-92 -> loops.jl:367-403): XtX_p, Xtz_p. */
 int hk_update_p(hk_ctx* ctx, double* XtX, double* Xtz);
 
-/* count_double_missing! (src/utils.jl:1-18) over the first J blocks: dm (length I). */
+/* count_double_missing! (src/utils.jl:1-18): dm (length I), the counts the EM normaliser 2(J - dm)
+ * (src/algorithms_inner.jl:66) uses.  After hk_ctx_create they cover the fit's J blocks; after
+ * hk_count_double_missing_file, every SNP pair of the file. */
 int hk_double_missing(hk_ctx* ctx, int64_t* dm);
+
+/* count_double_missing!(doublemissing, g) as AdmixData2 calls it (src/structs.jl:169 -> src/utils.jl:1-4):
+ * over the SNP pairs of the WHOLE file, ceil(n_snp / 2) in the reference, not only the J blocks of the fit,
+ * so the EM normaliser matches the reference when the file holds more than 2J SNPs.  bed, bytes_per_snp and
+ * n_snp describe the whole payload as for hk_ctx_create.  An odd last SNP has no partner and is not
+ * counted (the reference's loop reads one column past the matrix there).  Replaces the context's counts;
+ * contexts sharing its genotype upload see them too.  Must be called on the context that owns the upload. */
+int hk_count_double_missing_file(hk_ctx* ctx, const uint8_t* bed, int64_t bytes_per_snp, int64_t n_snp);
 
 /* Genotype codes as the device stream holds them: codes[s*I + i] = 2-bit BED code of individual i at
  * SNP s (s < 2J); bit-exact decode check of src/loops.jl:55-63. */
 int hk_decode(hk_ctx* ctx, uint8_t* codes);
+
+/* ---- Runner level: the device arrays of CuAdmixData (src/cuda/structs.jl:1-15) by name ----
+ * So that src/algorithms_inner.jl and src/algorithms_outer.jl run unchanged on a HipAdmixData whose fields
+ * proxy these arrays: copyto_sync! into / out of d_cu.q, d_cu.p, d_cu.q_next, d_cu.p_next, d_cu.XtX_q, ...
+ * (algorithms_inner.jl:36,42,80,124,126,182,250,252,326; algorithms_outer.jl:50,122,149), d_cu.p .= d_cu.p_next
+ * (algorithms_outer.jl:9-10) and loglikelihood(d_cu, g_cu, d_cu.q, d_cu.p) (algorithms_outer.jl:11,51,123,150;
+ * algorithms_inner.jl:183,327).  Arrays hold FP64 in the reference's layouts; HK_BUF_Q / HK_BUF_P are the
+ * parameters hk_set_params sets.  The curvature arrays are allocated on first use (zero-filled). */
+enum hk_buffer {
+  HK_BUF_Q = 0,      /* d_cu.q       K x I      */
+  HK_BUF_P = 1,      /* d_cu.p       K x 4J     */
+  HK_BUF_Q_NEXT = 2, /* d_cu.q_next  K x I      */
+  HK_BUF_P_NEXT = 3, /* d_cu.p_next  K x 4J     */
+  HK_BUF_XTX_Q = 4,  /* d_cu.XtX_q   K x K x I  */
+  HK_BUF_XTZ_Q = 5,  /* d_cu.Xtz_q   K x I      */
+  HK_BUF_XTX_P = 6,  /* d_cu.XtX_p   K x K x 4J */
+  HK_BUF_XTZ_P = 7   /* d_cu.Xtz_p   K x 4J     */
+};
+/* copyto_sync!([d_cu.<buf>], [host]) and copyto_sync!([host], [d_cu.<buf>]); HK_ENAN for non-finite
+ * parameters uploaded into q / p / q_next / p_next. */
+int hk_buf_upload(hk_ctx* ctx, int buf, const double* host);
+int hk_buf_download(hk_ctx* ctx, int buf, double* host);
+/* d_cu.<dst> .= d_cu.<src> (algorithms_outer.jl:9-10); both arrays of the same shape. */
+int hk_buf_copy(hk_ctx* ctx, int dst, int src);
+/* loglikelihood(d_cu, g_cu, d_cu.<qbuf>, d_cu.<pbuf>) (src/cuda/runners.jl:57-60). */
+int hk_loglik_buf(hk_ctx* ctx, int qbuf, int pbuf, double* ll);
+/* em!(d_cu, g_cu) / em_p!(d_cu, g_cu) (src/cuda/runners.jl:89-99): unnormalised EM sums from d_cu.q, d_cu.p
+ * into d_cu.q_next (skipped when fix_q != 0) and d_cu.p_next. */
+int hk_em_buf(hk_ctx* ctx, int fix_q);
+/* update_q_cuda!(d_cu, g_cu) / update_p_cuda!(d_cu, g_cu) (src/cuda/runners.jl:115-118, 133-136): curvature and
+ * gradient from d_cu.q, d_cu.p into d_cu.XtX_q, d_cu.Xtz_q / d_cu.XtX_p, d_cu.Xtz_p (Xtz not negated). */
+int hk_update_q_buf(hk_ctx* ctx);
+int hk_update_p_buf(hk_ctx* ctx);
 
 /* Device-resident row steps of update_q! / update_p! (src/algorithms_inner.jl:97-211, 224-355):
  * pass + per-row QP + step + projection on the current parameters; q_next / p_next are returned.

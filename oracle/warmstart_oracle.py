@@ -46,15 +46,17 @@ def dosages(codes: np.ndarray) -> np.ndarray:
     return np.where(codes == 1, np.nan, d)
 
 
-def features(codes: np.ndarray, n_snp: int, seed: int, D: int = D_FEAT) -> np.ndarray:
+def features(codes: np.ndarray, n_snp: int, seed: int, D: int = D_FEAT, rows=None) -> np.ndarray:
     """F (I x D) = S^-1/2 sum_{s observed} (g_is - mu_s) R_s over the first n_snp SNPs.
 
-    codes: (n_snp_total x I) PLINK codes, SNP-major."""
-    G = dosages(codes[:n_snp]).T  # I x S
-    obs = ~np.isnan(G)
-    cnt = obs.sum(axis=0)
-    mu = np.where(cnt > 0, np.nansum(G, axis=0) / np.maximum(cnt, 1), 0.0)
-    Z = np.where(obs, G - mu, 0.0)
+    codes: (n_snp_total x I) PLINK codes, SNP-major.  mu_s is the observed mean dosage of SNP s over all
+    individuals; rows (optional) selects the individuals whose features are returned."""
+    C = codes[:n_snp]
+    cnt = np.count_nonzero(C != 1, axis=1)
+    tot = np.count_nonzero(C == 2, axis=1) + 2 * np.count_nonzero(C == 3, axis=1)
+    mu = np.where(cnt > 0, tot / np.maximum(cnt, 1), 0.0)
+    G = dosages(C if rows is None else C[:, rows]).T  # I x S
+    Z = np.where(np.isnan(G), 0.0, G - mu)
     return Z @ gauss_matrix(n_snp, seed, D) / np.sqrt(n_snp)
 
 

@@ -1,4 +1,4 @@
-"""Shared test-data helpers (seeded synthetic BED + starting parameters)."""
+"""Shared test-data helpers (seeded synthetic BED + starting parameters) and the element-wise check."""
 import numpy as np
 
 from haplo_amd.sim import init_params, simulate
@@ -20,8 +20,24 @@ def unpack_codes(bed, I, nsnp):
     return out[:, :I]
 
 
-def rel(a, b):
+def rel(a, b, floor=0.0):
+    """Largest element-wise error max_i |a_i - b_i| / max(|b_i|, floor).
+
+    floor = 0 checks every element against its own magnitude.  The pass outputs (log-likelihood rows,
+    gradients, curvature, EM sums) are sums of same-signed terms, so a relative bound per element is what
+    their arithmetic guarantees, down to the smallest entry.  floor = 1 turns the check into an absolute
+    one for values of order one or below (parameters on the simplex, log-likelihood traces scaled by
+    their own first value are not meant here).  An element whose expected value is 0 (floor 0) must
+    match exactly."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
-    den = max(np.max(np.abs(b)), 1e-300)
-    return float(np.max(np.abs(a - b)) / den)
+    if a.shape != b.shape:
+        raise ValueError(f"shape mismatch {a.shape} vs {b.shape}")
+    if a.size == 0:
+        return 0.0
+    d = np.abs(a - b)
+    den = np.maximum(np.abs(b), floor)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.where(d == 0, 0.0, d / den)
+    r = np.where(np.isnan(r), np.inf, r)
+    return float(np.max(r))
