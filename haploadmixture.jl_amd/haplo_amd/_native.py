@@ -107,6 +107,7 @@ def hip():
     L.hk_em_buf.argtypes = [_vp, _int]
     L.hk_update_q_buf.argtypes = [_vp]
     L.hk_update_p_buf.argtypes = [_vp]
+    L.hk_loglik_noalloc.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.hk_q_step.argtypes = [_vp, _int, _dbl, _vp]
     L.hk_p_step.argtypes = [_vp, _vp, _int, _dbl, _vp]
     L.hk_fit_opts_default.argtypes = [ctypes.POINTER(FitOpts)]

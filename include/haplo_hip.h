@@ -201,6 +201,16 @@ int hk_get_tuning(hk_ctx* ctx, int* q_chunks, int* p_chunks);
 /* Wait for all work of the context's stream. */
 int hk_sync(hk_ctx* ctx);
 
+/* No counterpart in the reference: hk_loglik for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  q (K x I), p (K x 4J) and ll (one
+ * double) are DEVICE pointers; rows (I doubles, device) may be NULL.  The work is enqueued on the context's
+ * stream and not waited for (hk_sync).  It calls no hipMalloc / hipFree: the staged parameters and partial
+ * sums live in a fixed 64 MiB device array that each GPU holds for the life of the process, shared by every
+ * call on that GPU -- calls that overlap on one GPU race.  A problem that needs more than that (about
+ * K * 4J * 4 bytes for the MIXED engine, 8 for FP64, plus one partial row of I doubles) is rejected with
+ * HK_EINVAL. */
+int hk_loglik_noalloc(hk_ctx* ctx, const double* q, const double* p, double* ll, double* rows);
+
 /* Time one pass kind on the current parameters: kind 0 = log-likelihood, 1 = update_q pass,
  * 2 = update_p pass, 3 = EM pass; reps launches timed with HIP events on the context stream.
  * ms_per_launch receives the mean device time of the pass's main kernel. */
