@@ -109,3 +109,37 @@ def test_product_path_has_no_oracle_dependency():
             if fn.endswith((".py", ".hip", ".h", ".c")):
                 txt = open(os.path.join(dirpath, fn)).read()
                 assert "hko_" not in txt and "liboracle" not in txt, fn
+
+
+def test_julia_binding_covers_the_runner_level():
+    """integration/julia/HipBackend.jl: every library function it calls is declared in the header and
+    exported; it defines each name the reference's host code calls with d_cu !== nothing (cuda/runners.jl's
+    runners, _cu_admixture_base, copyto_sync! and `.=` on the device arrays); and HipAdmixData carries every
+    d_cu field that algorithms_outer.jl / algorithms_inner.jl read (q, p, q_next, p_next: outer :9-11,50,
+    122,149, inner :36,42,80,124,182,250,326; XtX_q, Xtz_q, XtX_p, Xtz_p: inner :126,252)."""
+    import re
+
+    path = os.path.join(_native.REPO_ROOT, "integration", "julia", "HipBackend.jl")
+    src = open(path).read()
+    called = set(re.findall(r"ccall\(\(:(hk_[a-z0-9_]+)", src))
+    assert called and called <= set(_native.header_symbols()), called - set(_native.header_symbols())
+    L = _native.hip()
+    assert all(hasattr(L, s) for s in called)
+    for sig in ("function loglikelihood(d::HipAdmixData, ::Nothing, q::HipBuf{2}, p::HipBuf{2})",
+                "function em!(d::HipAdmixData, ::Nothing)", "function em_p!(d::HipAdmixData, ::Nothing)",
+                "update_q_cuda!(d::HipAdmixData, ::Nothing)", "update_p_cuda!(d::HipAdmixData, ::Nothing)",
+                "function _cu_admixture_base(d::AdmixData2{T,T2}, g_la::SnpLinAlg{T2}, I::Int, J::Int)",
+                "function OpenADMIXTURE.copyto_sync!(dsts::Vector{<:HipBuf}, srcs::Vector{<:AbstractArray})",
+                "function OpenADMIXTURE.copyto_sync!(dsts::Vector{<:AbstractArray}, srcs::Vector{<:HipBuf})",
+                "function Base.copyto!(dst::HipBuf, bc::Broadcast.Broadcasted{HipStyle})"):
+        assert sig in src, sig
+    struct = src[src.index("mutable struct HipAdmixData"):]
+    struct = struct[: struct.index("\nend\n")]
+    for f in ("q", "p", "q_next", "p_next", "XtX_q", "Xtz_q", "XtX_p", "Xtz_p"):
+        assert re.search(rf"^\s+{f}::HipBuf\{{[23]\}}$", struct, re.M), f
+    # buffer ids agree with the header's enum
+    hdr = dict((n, int(v)) for n, v in re.findall(r"(HK_BUF_[A-Z_]+) = (\d+)", open(_native.HEADER).read()))
+    jl = {}
+    for names, vals in re.findall(r"^const (HK_BUF_[A-Z_, ]+) = (.*)$", src, re.M):
+        jl.update(zip(names.split(", "), (int(v) for v in re.findall(r"Cint\((\d+)\)", vals))))
+    assert jl == hdr, (jl, hdr)
