@@ -258,7 +258,7 @@ static int init_fit_buffers(hk_ctx* c) {
   HIPC(c, hipMalloc(&c->res_p, sizeof(double) * 4 * g.Jp * NT));
   HIPC(c, hipMalloc(&c->ll_dev, sizeof(double) * 4));
   HIPC(c, hipMalloc(&c->ll_rows, sizeof(double) * I));
-  HIPC(c, hipMalloc(&c->fails, sizeof(int)));
+  HIPC(c, hipMalloc(&c->fails, 2 * sizeof(int)));  // [QP failures, QPs stopped at the iteration limit]
   HIPC(c, hipMalloc(&c->redo, sizeof(int) * (c->J + 1)));
   HIPC(c, hipMalloc(&c->redo_late, sizeof(int) * (c->J + 1)));
   HIPC(c, hipMalloc(&c->pflag, c->J));
@@ -269,6 +269,7 @@ static int init_fit_buffers(hk_ctx* c) {
   HIPC(c, hipMalloc(&c->zal_coef, sizeof(double) * 8));
   plan_chunks(c);
   CHK(alloc_parts(c));
+  HIPC(c, prime_p_step_caps(K));  // runtime occupancy queries here, never inside a captured iteration
   HIPC(c, hipStreamSynchronize(c->s));
   return HK_OK;
 }
@@ -500,6 +501,7 @@ int hk_q_step(hk_ctx* c, int penalty, double rho, double* q_next) {
   CHK(run_qpass(c, MODE_CURV, qof(c, x), pof(c, x)));
   StepOpts so{penalty, rho, 1e-5};
   HIPC(c, launch_set_int(c->fails, 0, c->s));
+  HIPC(c, launch_set_int(c->fails + 1, 0, c->s));
   HIPC(c, launch_q_step(c->K, c->I, c->res_q, qof(c, x), qof(c, xn), so, c->fails, c->s));
   int nf = 0;
   HIPC(c, hipMemcpyAsync(&nf, c->fails, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -521,6 +523,7 @@ int hk_p_step(hk_ctx* c, const double* q_pass, int penalty, double rho, double* 
   CHK(run_ppass(c, MODE_CURV, qp, pof(c, x)));
   StepOpts so{penalty, rho, 1e-5};
   HIPC(c, launch_set_int(c->fails, 0, c->s));
+  HIPC(c, launch_set_int(c->fails + 1, 0, c->s));
   HIPC(c, launch_p_step(c->K, c->J, c->res_p, pof(c, x), pof(c, xn), so, c->fails, p_lists(c), c->s));
   int nf = 0;
   HIPC(c, hipMemcpyAsync(&nf, c->fails, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -929,6 +932,7 @@ static int fit_prepare(FitRun& f, double ll_start) {
   f.tm.s = c->s;
   CHK(ensure_fit_buffers(c, o->Qn));
   HIPC(c, launch_set_int(c->fails, 0, c->s));
+  HIPC(c, launch_set_int(c->fails + 1, 0, c->s));
   HIPC(c, launch_set_int(c->d_iter, 1, c->s));
   return HK_OK;
 }
@@ -985,13 +989,14 @@ static int fit_after_iteration(FitRun& f) {
 
 static int fit_finish(FitRun& f) {
   hk_ctx* c = f.c;
-  int nf = 0;
-  HIPC(c, hipMemcpyAsync(&nf, c->fails, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  int nf[2] = {0, 0};
+  HIPC(c, hipMemcpyAsync(nf, c->fails, 2 * sizeof(int), hipMemcpyDeviceToHost, c->s));
   HIPC(c, hipStreamSynchronize(c->s));
   hk_fit_result* r = f.res;
   r->ll = f.ll_new;
   r->iters = f.iter;
-  r->qp_failures = nf;
+  r->qp_failures = nf[0];
+  r->qp_iter_limit = nf[1];
   if (f.tm.on) {
     double t[T_NCAT] = {0};
     int n[T_NCAT] = {0};

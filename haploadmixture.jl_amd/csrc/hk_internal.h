@@ -74,6 +74,9 @@ struct PStepLists {
   const unsigned char* flags;
   int* late;
 };
+// Occupancy caps of the P-step kernels for K, queried once per process at context set-up (outside any capture);
+// launch_p_step returns hipErrorNotReady for a K that was never primed.
+hipError_t prime_p_step_caps(int K);
 hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
                          int* fails, const PStepLists& lists, hipStream_t s, double* dp = nullptr);
 hipError_t launch_project_q(int K, int64_t I, double* q, double eps, hipStream_t s);

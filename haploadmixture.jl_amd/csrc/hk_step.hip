@@ -13,6 +13,8 @@
 // active-set method on ONE wavefront: lane r owns row r of the (masked) curvature matrix in
 // registers, pivots and columns are broadcast with v_readlane, and the two triangular solves use
 // one LDS transpose.  Every reduction runs in a fixed order; the only atomic is the failure counter.
+#include <mutex>
+
 #include "hk_device.h"
 #include "hk_internal.h"
 
@@ -330,6 +332,8 @@ __global__ __launch_bounds__(64) void k_q_step(int64_t I, const double* __restri
   else it = wave_qp<K, 1>(Hrow, g, lo, hi, 0, x, lds);
   if (it < 0) x = 0.0;
   if (it < 0 && lane == 0) atomicAdd(fails, 1);
+  // iteration limit reached: the last (feasible) iterate is applied and counted in fails[1]
+  if (it >= 4 * K + 20 && lane == 0) atomicAdd(fails + 1, 1);
   if (dq && lane < K) dq[lane + (int64_t)K * i] = x;  // raw step, for step halving
   // q_next = q + dq, then project_q! (lane 0 gathers the K values).
   double v[K];
@@ -381,6 +385,7 @@ __device__ __forceinline__ void p_step_dense(int64_t j, const double* __restrict
   else it = wave_qp<N, K>(Hrow, g, lo, hi, k, x, lds);
   if (it < 0) x = 0.0;
   if (it < 0 && lane == 0) atomicAdd(fails, 1);
+  if (it >= 4 * N + 20 && lane == 0) atomicAdd(fails + 1, 1);  // iteration limit: last iterate applied
   if (dp && act) dp[k + (int64_t)K * (4 * j + l)] = x;  // raw step, for step halving
   // p_next = p + dp, then project_p! on each (k, j) 4-vector: lane k gathers haplotypes 0..3.
   const double nv = pv + x;
@@ -744,6 +749,9 @@ __device__ __forceinline__ void p_step4_body(int64_t J, const double* __restrict
       if (l == 0) late[1 + atomicAdd(late, 1)] = (int)j;
       return;
     }
+    // stopped at the iteration limit (it == maxit; a solved quad stops below it): the last feasible
+    // iterate is applied and counted in fails[1]
+    if (it >= maxit && l == 0) atomicAdd(fails + 1, 1);
     if (dp) {
       #pragma unroll
       for (int k = 0; k < K; ++k) dp[k + (int64_t)K * (4 * j + l)] = x[k];
@@ -1102,8 +1110,8 @@ hipError_t launch_q_step(int K, int64_t I, const double* res_q, const double* q,
   });
 }
 
-// Workgroups of k_p_step4 resident on the whole device at once (occupancy x compute units), computed once
-// per kernel instance: the grid never exceeds it, each slot then solves several quads in turn.
+// Workgroups of k_p_step4 resident on the whole device at once (occupancy x compute units): the grid never
+// exceeds it, each slot then solves several quads in turn.
 template <typename F>
 static unsigned resident_blocks(F kernel, int block) {
   int per_cu = 0, dev = 0, cus = 0;
@@ -1114,6 +1122,37 @@ static unsigned resident_blocks(F kernel, int block) {
   return (unsigned)(per_cu * cus);
 }
 
+// Four-lane P-step kernels of one K and constraint mode: (one quad per slot, several quads per slot, dense
+// list kernel).  K = 6 takes the striding form for both (the one-quad-per-slot form would spill at 256
+// registers); K <= 5 packs two quads per wave (the _w2 forms).
+template <int KT, bool PEN>
+struct PStepKernels {
+  static auto single() {
+    if constexpr (KT == 6 && PEN) return k_p_step4_w2<KT, PEN, true>;
+    else if constexpr (KT <= 5) return k_p_step4_w2<KT, PEN, false>;
+    else return k_p_step4<KT, PEN, false>;
+  }
+  static auto multi() {
+    if constexpr (KT <= 6 && (KT <= 5 || PEN)) return k_p_step4_w2<KT, PEN, true>;
+    else return k_p_step4<KT, PEN, true>;
+  }
+};
+
+// Resident-slot caps per (constraint mode, K), queried by prime_p_step_caps at context set-up so that the
+// launcher itself makes no runtime queries (it may run inside a stream capture).
+static unsigned g_pcap[2][HK_MAXK + 1];
+static std::mutex g_pcap_mu;
+
+hipError_t prime_p_step_caps(int K) {
+  if (K < 1 || K > HK_MAXK) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lk(g_pcap_mu);
+  return with_k(K, [&](auto kc) {
+    constexpr int KT = decltype(kc)::value;
+    if (!g_pcap[1][KT]) g_pcap[1][KT] = resident_blocks(PStepKernels<KT, true>::multi(), 256);
+    if (!g_pcap[0][KT]) g_pcap[0][KT] = resident_blocks(PStepKernels<KT, false>::multi(), 256);
+  });
+}
+
 // One launch for the P row QPs: the first NDB workgroups run the dense one-wave solver over the early list
 // (blocks launch_finish_p marked), the rest the four-lane QPs of all other blocks (several quads per slot
 // when J exceeds the slots the device holds), so the dense solver's long active-set loops overlap the
@@ -1121,45 +1160,31 @@ static unsigned resident_blocks(F kernel, int block) {
 // the same) go to the late list, solved after it.
 hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
                          int* fails, const PStepLists& lists, hipStream_t s, double* dp) {
+  if (K < 1 || K > HK_MAXK) return hipErrorInvalidValue;
+  const unsigned cap = g_pcap[o.penalty ? 1 : 0][K];
+  if (!cap) return hipErrorNotReady;  // prime_p_step_caps(K) was not called at set-up
   hipError_t e = launch_set_int(lists.late, 0, s);
   if (e != hipSuccess) return e;
   constexpr int NDB = 64;  // 256 wavefronts for the early list (it holds a few blocks in the usual case)
   const unsigned nq = (unsigned)((4 * J + 255) / 256), nd = (unsigned)(J < 1024 ? J : 1024);
   return with_k(K, [&](auto kc) {
     constexpr int KT = decltype(kc)::value;
-    auto run = [&](auto single, auto multi, auto list_kernel, unsigned cap) {
+    auto run = [&](auto single, auto multi, auto list_kernel) {
       // the dense workgroups exit at once when their list is empty (the usual case), so the four-lane grid
       // keeps the whole device's worth of slots
-      const unsigned room = cap;
-      if (nq <= room)
+      if (nq <= cap)
         hipLaunchKernelGGL(single, dim3(NDB + nq), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp,
                            lists.flags, lists.dense, NDB, fails, lists.late);
       else
-        hipLaunchKernelGGL(multi, dim3(NDB + room), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp,
+        hipLaunchKernelGGL(multi, dim3(NDB + cap), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp,
                            lists.flags, lists.dense, NDB, fails, lists.late);
       hipLaunchKernelGGL(list_kernel, dim3(nd), dim3(64), 0, s, res_p, p, p_next, o.rho, o.eps, fails, dp,
                          lists.late);
     };
-    if (o.penalty) {
-      if constexpr (KT == 6) {  // the one-quad-per-slot form would spill at 256 registers: the striding one for both
-        static const unsigned cap = resident_blocks(k_p_step4_w2<KT, true, true>, 256);
-        run(k_p_step4_w2<KT, true, true>, k_p_step4_w2<KT, true, true>, k_p_step_list<KT, true>, cap);
-      } else if constexpr (KT <= 5) {
-        static const unsigned cap = resident_blocks(k_p_step4_w2<KT, true, true>, 256);
-        run(k_p_step4_w2<KT, true, false>, k_p_step4_w2<KT, true, true>, k_p_step_list<KT, true>, cap);
-      } else {
-        static const unsigned cap = resident_blocks(k_p_step4<KT, true, true>, 256);
-        run(k_p_step4<KT, true, false>, k_p_step4<KT, true, true>, k_p_step_list<KT, true>, cap);
-      }
-    } else {
-      if constexpr (KT <= 5) {
-        static const unsigned cap = resident_blocks(k_p_step4_w2<KT, false, true>, 256);
-        run(k_p_step4_w2<KT, false, false>, k_p_step4_w2<KT, false, true>, k_p_step_list<KT, false>, cap);
-      } else {
-        static const unsigned cap = resident_blocks(k_p_step4<KT, false, true>, 256);
-        run(k_p_step4<KT, false, false>, k_p_step4<KT, false, true>, k_p_step_list<KT, false>, cap);
-      }
-    }
+    if (o.penalty)
+      run(PStepKernels<KT, true>::single(), PStepKernels<KT, true>::multi(), k_p_step_list<KT, true>);
+    else
+      run(PStepKernels<KT, false>::single(), PStepKernels<KT, false>::multi(), k_p_step_list<KT, false>);
   });
 }
 

@@ -55,7 +55,7 @@ class FitResult(ctypes.Structure):
         ("t_qpass_ms", _dbl), ("t_ppass_ms", _dbl), ("t_ll_ms", _dbl), ("t_qstep_ms", _dbl),
         ("t_pstep_ms", _dbl), ("t_em_ms", _dbl), ("t_qn_ms", _dbl),
         ("n_qpass", _int), ("n_ppass", _int), ("n_ll", _int), ("n_qstep", _int), ("n_pstep", _int),
-        ("n_em", _int), ("n_qn", _int),
+        ("n_em", _int), ("n_qn", _int), ("qp_iter_limit", _int),
     ]
 
     def as_dict(self):

@@ -167,6 +167,8 @@ typedef struct {
   /* profile != 0: summed device time (ms) and launch counts per kernel family */
   double t_qpass_ms, t_ppass_ms, t_ll_ms, t_qstep_ms, t_pstep_ms, t_em_ms, t_qn_ms;
   int n_qpass, n_ppass, n_ll, n_qstep, n_pstep, n_em, n_qn;
+  int qp_iter_limit;     /* row QPs stopped at the active-set iteration limit (4n + 20 for n variables):
+                            their last iterate, feasible by construction, was applied, not a solution of the QP */
 } hk_fit_result;
 
 void hk_fit_opts_default(hk_fit_opts* o);

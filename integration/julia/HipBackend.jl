@@ -200,6 +200,7 @@ struct HkFitResult                                    # hk_fit_result
     t_qpass_ms::Cdouble; t_ppass_ms::Cdouble; t_ll_ms::Cdouble; t_qstep_ms::Cdouble; t_pstep_ms::Cdouble
     t_em_ms::Cdouble; t_qn_ms::Cdouble
     n_qpass::Cint; n_ppass::Cint; n_ll::Cint; n_qstep::Cint; n_pstep::Cint; n_em::Cint; n_qn::Cint
+    qp_iter_limit::Cint
 end
 
 """

@@ -143,3 +143,25 @@ def test_julia_binding_covers_the_runner_level():
     for names, vals in re.findall(r"^const (HK_BUF_[A-Z_, ]+) = (.*)$", src, re.M):
         jl.update(zip(names.split(", "), (int(v) for v in re.findall(r"Cint\((\d+)\)", vals))))
     assert jl == hdr, (jl, hdr)
+
+
+def test_result_and_option_structs_match_the_header(tmp_path):
+    """ctypes' FitOpts / FitResult have the C structs' size and field offsets (compiled from the header)."""
+    fields_r = [f for f, _ in _native.FitResult._fields_]
+    fields_o = [f for f, _ in _native.FitOpts._fields_]
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "haplo_hip.h"', "int main(void) {",
+             'printf("%zu %zu\\n", sizeof(hk_fit_result), sizeof(hk_fit_opts));']
+    lines += [f'printf("%zu\\n", offsetof(hk_fit_result, {f}));' for f in fields_r]
+    lines += [f'printf("%zu\\n", offsetof(hk_fit_opts, {f}));' for f in fields_o]
+    lines += ["return 0;", "}"]
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.dirname(_native.HEADER), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    vals = [int(v) for v in out]
+    assert vals[0] == ctypes.sizeof(_native.FitResult) and vals[1] == ctypes.sizeof(_native.FitOpts)
+    offs_r = vals[2: 2 + len(fields_r)]
+    offs_o = vals[2 + len(fields_r):]
+    assert offs_r == [getattr(_native.FitResult, f).offset for f in fields_r]
+    assert offs_o == [getattr(_native.FitOpts, f).offset for f in fields_o]

@@ -37,7 +37,7 @@ def test_c1_fp64_fit_matches_oracle(oracle):
         e.set_params(q0, p0)
         r = e.fit(rtol=1e-7, n_iter=1000)
         gq, gp = e.get_params()
-    assert r["converged"] and r["qp_failures"] == 0
+    assert r["converged"] and r["qp_failures"] == 0 and r["qp_iter_limit"] == 0
     assert r["iters"] == oit, (r["iters"], oit)
     assert rel(r["trace"], otr) <= 1e-10
     assert abs(r["ll"] - oll) <= 1e-9 * abs(oll)
@@ -81,7 +81,7 @@ def test_c4_batch_equals_single_fits_and_aic_picks_true_k():
             e.set_params(q0, p0)
             ra = e.fit(o, trace=False)
             assert ra["iters"] == rb["iters"] and ra["ll"] == rb["ll"], sp
-            assert rb["converged"] and rb["qp_failures"] == 0, sp
+            assert rb["converged"] and rb["qp_failures"] == 0 and rb["qp_iter_limit"] == 0, sp
     finally:
         for e in engines[1:]:
             e.close()

@@ -182,7 +182,7 @@ def test_fit_matches_oracle_fit(prec, oracle):
     oq, op, oll, oit, otr = oracle.fit(s.bed, I, J, q, p, rtol=1e-9, n_iter=300)
     with _eng(s, q, p, prec) as e:
         r = e.fit(rtol=1e-9, n_iter=300)
-        assert r["qp_failures"] == 0
+        assert r["qp_failures"] == 0 and r["qp_iter_limit"] == 0
         tol = 1e-9 if prec == "fp64" else 1e-6
         assert abs(r["ll"] - oll) <= tol * abs(oll), (r["ll"], oll, r["iters"], oit)
         # the log-likelihood never decreases across accepted iterations beyond round-off (MIXED: the
