@@ -111,10 +111,13 @@ def select_bench(a, w, prec):
     Ks = list(range(2, 9))
     log("warm-up (kernel loading, graph capture)")
     select_k(data.bed, I, J, Ks=Ks, restarts=1, precision=prec, rtol=a.rtol, n_iter=3, autotune=False)
-    log("concurrent batch")
+    log("concurrent batch: one captured graph per round")
     b = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, batched=True,
-                 use_graph=not a.eager)
+                 use_graph=not a.eager, batch_rounds=not a.eager)
     log(f"batch wall {b.wall_s:.2f}s best K={b.best_k}")
+    ev = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, batched=True,
+                  use_graph=not a.eager, batch_rounds=False)
+    log(f"event-driven batch wall {ev.wall_s:.2f}s best K={ev.best_k}")
     sq = select_k(data.bed, I, J, Ks=Ks, restarts=4, precision=prec, rtol=a.rtol, n_iter=a.max_iter, batched=False,
                   use_graph=not a.eager)
     log(f"sequential wall {sq.wall_s:.2f}s best K={sq.best_k}")
@@ -127,7 +130,9 @@ def select_bench(a, w, prec):
         "vs_baseline": None, "dtype": "f32-products/f64-accum" if prec == "mixed" else "f64",
         "data": "synthetic (seeded generator, true K=5)",
         "config": {"workload": a.workload, "I": I, "J": J, "Ks": Ks, "restarts": 4, "fits": len(b.table),
-                   "precision": prec, "rtol": a.rtol, "graphs": not a.eager},
+                   "precision": prec, "rtol": a.rtol,
+                   "batch": "eager per-fit enqueue" if a.eager else "one captured graph per round, device stops"},
+        "event_driven_batch_wall_s": round(ev.wall_s, 3),
         "sequential_wall_s": round(sq.wall_s, 3),
         "speedup_batch_over_sequential": round(sq.wall_s / b.wall_s, 3),
         "per_fit_latency_batch_s": {"p50": round(float(np.percentile(lat_b, 50)), 3),

@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <utility>
 #include <vector>
@@ -95,14 +96,15 @@ static double* qof(hk_ctx* c, double* x) { return x; }
 static double* pof(hk_ctx* c, double* x) { return x + (int64_t)c->K * c->I; }
 
 // Pointer handed to a pass for parameters (q, p): FP64 directly, or the FP32 mirror.
-static int pass_params(hk_ctx* c, const double* q, const double* p, const void** qq, const void** pp) {
+static int pass_params(hk_ctx* c, const double* q, const double* p, const void** qq, const void** pp,
+                       const int* gate = nullptr) {
   if (c->prec == PREC_FP64) {
     *qq = q;
     *pp = p;
     return HK_OK;
   }
   const int64_t nq = (int64_t)c->K * c->I, np = (int64_t)c->K * 4 * c->g.Jp;
-  HIPC(c, launch_pair_to_f32(q, nq, p, np, c->x32, c->s));
+  HIPC(c, launch_pair_to_f32(q, nq, p, np, c->x32, c->s, gate));
   *qq = c->x32;
   *pp = c->x32 + nq;
   return HK_OK;
@@ -157,39 +159,45 @@ enum { T_QPASS, T_PPASS, T_LL, T_QSTEP, T_PSTEP, T_EM, T_QN, T_NCAT };
 
 }  // namespace
 
-// Q-side pass (individual-parallel): MODE_LL -> ll_dev / ll_rows, else res_q.
+// Q-side pass (individual-parallel): MODE_LL -> ll_dev / ll_rows, else res_q.  gate: the fit's device gate
+// when the pass is part of a batch graph (every kernel then returns at once after the fit stopped).
 static int run_qpass(hk_ctx* c, int mode, const double* q, const double* p, Timer* tm = nullptr, int cat = 0,
-                     double* ll_out = nullptr) {
+                     double* ll_out = nullptr, const int* gate = nullptr) {
+  PassGeom g = c->g;
+  g.gate = gate;
   const void *qq, *pp;
-  CHK(pass_params(c, q, p, &qq, &pp));
-  if (mode == MODE_CURV) HIPC(c, launch_curv_tables(c->K, c->prec, 0, c->g, qq, pp, c->s));
+  CHK(pass_params(c, q, p, &qq, &pp, gate));
+  if (mode == MODE_CURV) HIPC(c, launch_curv_tables(c->K, c->prec, 0, g, qq, pp, c->s));
   if (tm) tm->begin(cat);
-  HIPC(c, launch_qside(c->K, c->prec, mode, c->g, qq, pp, c->part_q, c->s));
+  HIPC(c, launch_qside(c->K, c->prec, mode, g, qq, pp, c->part_q, c->s));
   if (tm) tm->end();
   if (mode == MODE_LL)
-    HIPC(c, launch_reduce_ll(c->part_q, c->g.qchunks, c->g, c->ll_rows, ll_out ? ll_out : c->ll_dev, c->s));
-  else if (mode == MODE_CURV) HIPC(c, launch_reduce_q_curv(c->part_q, c->g.qchunks, c->K, c->g, q, c->res_q, c->s));
-  else HIPC(c, launch_reduce_q(c->part_q, c->g.qchunks, nacc(c->K, mode), c->g, c->res_q, c->s));
+    HIPC(c, launch_reduce_ll(c->part_q, g.qchunks, g, c->ll_rows, ll_out ? ll_out : c->ll_dev, c->s));
+  else if (mode == MODE_CURV) HIPC(c, launch_reduce_q_curv(c->part_q, g.qchunks, c->K, g, q, c->res_q, c->s));
+  else HIPC(c, launch_reduce_q(c->part_q, g.qchunks, nacc(c->K, mode), g, c->res_q, c->s));
   return HK_OK;
 }
 
 // P-side pass (block-parallel) -> res_p [4Jp][NT].
-static int run_ppass(hk_ctx* c, int mode, const double* q, const double* p, Timer* tm = nullptr, int cat = 0) {
+static int run_ppass(hk_ctx* c, int mode, const double* q, const double* p, Timer* tm = nullptr, int cat = 0,
+                     const int* gate = nullptr) {
+  PassGeom g = c->g;
+  g.gate = gate;
   const void *qq, *pp;
-  CHK(pass_params(c, q, p, &qq, &pp));
+  CHK(pass_params(c, q, p, &qq, &pp, gate));
   const int NT = nacc(c->K, mode);
-  if (mode == MODE_CURV) HIPC(c, launch_curv_tables(c->K, c->prec, 1, c->g, qq, pp, c->s));
-  if (c->g.pchunks == 1) {
+  if (mode == MODE_CURV) HIPC(c, launch_curv_tables(c->K, c->prec, 1, g, qq, pp, c->s));
+  if (g.pchunks == 1) {
     if (tm) tm->begin(cat);
-    HIPC(c, launch_pside(c->K, c->prec, mode, c->g, qq, pp, c->res_p, c->s));
+    HIPC(c, launch_pside(c->K, c->prec, mode, g, qq, pp, c->res_p, c->s));
     if (tm) tm->end();
   } else {
     if (tm) tm->begin(cat);
-    HIPC(c, launch_pside(c->K, c->prec, mode, c->g, qq, pp, c->part_p, c->s));
+    HIPC(c, launch_pside(c->K, c->prec, mode, g, qq, pp, c->part_p, c->s));
     if (tm) tm->end();
-    HIPC(c, launch_reduce_p(c->part_p, c->g.pchunks, NT, c->g, c->res_p, c->s));
+    HIPC(c, launch_reduce_p(c->part_p, g.pchunks, NT, g, c->res_p, c->s));
   }
-  if (mode == MODE_CURV) HIPC(c, launch_finish_p(c->K, c->g, p, c->res_p, c->pflag, c->redo, c->s));
+  if (mode == MODE_CURV) HIPC(c, launch_finish_p(c->K, g, p, c->res_p, c->pflag, c->redo, c->s));
   return HK_OK;
 }
 
@@ -699,6 +707,7 @@ void hk_fit_opts_default(hk_fit_opts* o) {
   o->em_fix_q = 0;
   o->guarantee_increase = 0;
   o->qn_mode = 0;
+  o->batch_rounds = 1;
 }
 
 }  // extern "C"
@@ -750,8 +759,11 @@ static int halve_step(hk_ctx* c, const hk_fit_opts* o, Timer* tm, bool is_q, con
   }
 }
 
-// ll_prev: the iteration's starting log-likelihood (needed by guarantee_increase only).
-static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm, double ll_prev = NAN) {
+// ll_prev: the iteration's starting log-likelihood (needed by guarantee_increase only).  gate / ctl / trace:
+// set when the iteration is part of a batch graph -- every kernel is gated on the fit's device gate and the
+// device-side stopping rule (launch_fit_control) closes the iteration.
+static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm, double ll_prev = NAN, int* gate = nullptr,
+                             FitCtl* ctl = nullptr, double* trace = nullptr) {
   const int K = c->K;
   const int64_t I = c->I, J = c->J;
   const int64_t N = (int64_t)K * (I + 4 * J), nq = (int64_t)K * I, np = (int64_t)K * 4 * J;
@@ -759,8 +771,9 @@ static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm, double 
   double* xn = c->xb[1];
   double* xn2 = c->xb[2];
   double* xt = c->xb[3];
-  const StepOpts qso{o->q_penalty, o->rho, o->eps};
-  const StepOpts pso{o->p_penalty, o->rho, o->eps};
+  StepOpts qso{o->q_penalty, o->rho, o->eps};
+  StepOpts pso{o->p_penalty, o->rho, o->eps};
+  qso.gate = pso.gate = gate;
   // guarantee_increase: the raw QP steps live in x_tmp (d.q_tmp / d.p_tmp in the reference) until ZAL
   // overwrites it; every step is then line-searched on the host, so the iteration cannot be captured.
   const bool gi = o->guarantee_increase != 0;
@@ -768,7 +781,7 @@ static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm, double 
   double* dp = gi ? pof(c, xt) : nullptr;
   double ll_tmp = ll_prev;
   auto qstep = [&](const double* q, const double* p, double* q_out) -> int {
-    CHK(run_qpass(c, MODE_CURV, q, p, tm, T_QPASS));
+    CHK(run_qpass(c, MODE_CURV, q, p, tm, T_QPASS, nullptr, gate));
     if (tm) tm->begin(T_QSTEP);
     HIPC(c, launch_q_step(K, I, c->res_q, q, q_out, qso, c->fails, c->s, dq));
     if (tm) tm->end();
@@ -776,7 +789,7 @@ static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm, double 
     return HK_OK;
   };
   auto pstep = [&](const double* q, const double* p, double* p_out) -> int {
-    CHK(run_ppass(c, MODE_CURV, q, p, tm, T_PPASS));
+    CHK(run_ppass(c, MODE_CURV, q, p, tm, T_PPASS, gate));
     if (tm) tm->begin(T_PSTEP);
     HIPC(c, launch_p_step(K, J, c->res_p, p, p_out, pso, c->fails, p_lists(c), c->s, dp));
     if (tm) tm->end();
@@ -785,22 +798,24 @@ static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm, double 
   };
   // F(x): q_next from (q, p), p_next from (q_next, p)  (update_q!, update_p!; Gauss-Seidel order)
   if (!o->fix_q) CHK(qstep(qof(c, x), pof(c, x), qof(c, xn)));
-  else HIPC(c, launch_copy_f64(nq, qof(c, x), qof(c, xn), c->s));
+  else HIPC(c, launch_copy_f64(nq, qof(c, x), qof(c, xn), c->s, gate));
   if (!o->fix_p) CHK(pstep(qof(c, xn), pof(c, x), pof(c, xn)));
-  else HIPC(c, launch_copy_f64(np, pof(c, x), pof(c, xn), c->s));
+  else HIPC(c, launch_copy_f64(np, pof(c, x), pof(c, xn), c->s, gate));
   // F(F(x)): q_next2 from (q_next, p_next), p_next2 from (q_next2, p_next)
   if (!o->fix_q) CHK(qstep(qof(c, xn), pof(c, xn), qof(c, xn2)));
-  else HIPC(c, launch_copy_f64(nq, qof(c, x), qof(c, xn2), c->s));
+  else HIPC(c, launch_copy_f64(nq, qof(c, x), qof(c, xn2), c->s, gate));
   if (!o->fix_p) CHK(pstep(qof(c, xn2), pof(c, xn), pof(c, xn2)));
-  else HIPC(c, launch_copy_f64(np, pof(c, x), pof(c, xn2), c->s));
-  CHK(run_qpass(c, MODE_LL, qof(c, xn2), pof(c, xn2), tm, T_LL, c->d_ll + 0));
+  else HIPC(c, launch_copy_f64(np, pof(c, x), pof(c, xn2), c->s, gate));
+  CHK(run_qpass(c, MODE_LL, qof(c, xn2), pof(c, xn2), tm, T_LL, c->d_ll + 0, gate));
   if (tm) tm->begin(T_QN);
-  HIPC(c, launch_zal(N, o->Qn, c->d_iter, c->U, c->V, x, xn, xn2, xt, c->zal_part, c->zal_coef, c->s, o->qn_mode));
-  HIPC(c, launch_project_p(K, J, pof(c, xt), o->eps, c->s));
-  HIPC(c, launch_project_q(K, I, qof(c, xt), o->eps, c->s));
+  HIPC(c, launch_zal(N, o->Qn, c->d_iter, c->U, c->V, x, xn, xn2, xt, c->zal_part, c->zal_coef, c->s, o->qn_mode,
+                     gate));
+  HIPC(c, launch_project_p(K, J, pof(c, xt), o->eps, c->s, gate));
+  HIPC(c, launch_project_q(K, I, qof(c, xt), o->eps, c->s, gate));
   if (tm) tm->end();
-  CHK(run_qpass(c, MODE_LL, qof(c, xt), pof(c, xt), tm, T_LL, c->d_ll + 1));
-  HIPC(c, launch_accept(N, c->d_ll, xn2, xt, x, c->d_iter, c->s));
+  CHK(run_qpass(c, MODE_LL, qof(c, xt), pof(c, xt), tm, T_LL, c->d_ll + 1, gate));
+  HIPC(c, launch_accept(N, c->d_ll, xn2, xt, x, c->d_iter, c->s, gate));
+  if (ctl) HIPC(c, launch_fit_control(c->d_ll, ctl, trace, gate, c->s));
   return HK_OK;
 }
 
@@ -1039,9 +1054,141 @@ extern "C" int hk_fit(hk_ctx* c, const hk_fit_opts* o, double ll_start, double* 
   return hk_fit_ex(c, o, ll_start, ll_trace, nullptr, res);
 }
 
-// Several fits in lockstep, each on its own context / stream: every round enqueues one iteration of each
-// active fit (captured graphs), then collects their log-likelihoods.  Contexts created with
-// hk_ctx_create_shared() read one genotype upload.
+namespace {
+// Device state of a round-graph batch: per-fit gates and stopping-rule state, the fork / join stream and
+// events, the captured round.  Released after the root stream synchronised.
+struct RoundBatch {
+  int n = 0;
+  int* gates = nullptr;      // device, n: 1 while the fit runs
+  FitCtl* ctls = nullptr;    // device, n
+  int* h_gates = nullptr;    // pinned, 2 x n: the gates at the last two checks
+  hipStream_t root = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipEvent_t chk[2] = {nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~RoundBatch() {
+    if (root) (void)hipStreamSynchronize(root);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : join)
+      if (e) (void)hipEventDestroy(e);
+    for (auto e : {fork, chk[0], chk[1]})
+      if (e) (void)hipEventDestroy(e);
+    if (root) (void)hipStreamDestroy(root);
+    if (gates) (void)hipFree(gates);
+    if (ctls) (void)hipFree(ctls);
+    if (h_gates) (void)hipHostFree(h_gates);
+  }
+};
+}  // namespace
+
+// QN iterations of every fit as ONE captured graph per round: a fork from a root stream to each fit's stream,
+// kRoundIters whole iterations of the fit (every kernel gated on the fit's device gate, each iteration closed
+// by the device-side stopping rule), a join.  The same executable graph is replayed round after round; a fit whose stopping rule fired skips every
+// kernel of later rounds, so nothing is re-captured as fits converge, and each fit runs exactly the
+// iterations hk_fit would run.  One round stays in flight while the host reads the gates of the previous one.
+// QN iterations of each fit per round: a fit that stops within a round skips the rest of it (gated).
+constexpr int kRoundIters = 1;
+
+static int fit_batch_rounds(hk_ctx** ctxs, int n, const hk_fit_opts* opts, std::vector<FitRun>& runs,
+                            double* seconds_at_finish, std::chrono::steady_clock::time_point t0) {
+  hk_ctx* c0 = ctxs[0];
+  RoundBatch rb;
+  rb.n = n;
+  rb.join.assign(n, nullptr);
+  HIPC(c0, hipSetDevice(c0->dev));
+  HIPC(c0, hipMalloc(&rb.gates, sizeof(int) * n));
+  HIPC(c0, hipMalloc(&rb.ctls, sizeof(FitCtl) * n));
+  HIPC(c0, hipHostMalloc((void**)&rb.h_gates, sizeof(int) * 2 * n, hipHostMallocDefault));
+  HIPC(c0, hipStreamCreateWithFlags(&rb.root, hipStreamNonBlocking));
+  HIPC(c0, hipEventCreateWithFlags(&rb.fork, hipEventDisableTiming));
+  HIPC(c0, hipEventCreateWithFlags(&rb.chk[0], hipEventDisableTiming));
+  HIPC(c0, hipEventCreateWithFlags(&rb.chk[1], hipEventDisableTiming));
+  for (int b = 0; b < n; ++b) HIPC(c0, hipEventCreateWithFlags(&rb.join[b], hipEventDisableTiming));
+  std::vector<int> g0(n);
+  std::vector<FitCtl> ctl(n);
+  for (int b = 0; b < n; ++b) {
+    g0[b] = runs[b].active ? 1 : 0;
+    ctl[b] = FitCtl{runs[b].ll_new, opts[b].rtol, opts[b].n_iter, 0, 0, 0};
+    HIPC(ctxs[b], hipStreamSynchronize(ctxs[b]->s));  // the warm-up and set-up work of every fit is done
+  }
+  HIPC(c0, hipMemcpy(rb.gates, g0.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  HIPC(c0, hipMemcpy(rb.ctls, ctl.data(), sizeof(FitCtl) * n, hipMemcpyHostToDevice));
+  // capture one round, the costliest fits (largest K x individuals x blocks) first: their branches set the
+  // length of a round, so they are dispatched before the cheap ones
+  std::vector<int> order(n);
+  for (int b = 0; b < n; ++b) order[b] = b;
+  auto cost = [&](int b) { return (double)ctxs[b]->K * ctxs[b]->K * (double)ctxs[b]->I * (double)ctxs[b]->J; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  HIPC(c0, hipStreamBeginCapture(rb.root, hipStreamCaptureModeThreadLocal));
+  int rc = HK_OK;
+  hipError_t e = hipEventRecord(rb.fork, rb.root);
+  for (int t = 0; t < n && e == hipSuccess && rc == HK_OK; ++t) {
+    const int b = order[t];
+    hk_ctx* c = ctxs[b];
+    e = hipStreamWaitEvent(c->s, rb.fork, 0);
+    for (int r = 0; r < kRoundIters && e == hipSuccess && rc == HK_OK; ++r)
+      rc = enqueue_iteration(c, &opts[b], nullptr, NAN, rb.gates + b, rb.ctls + b, nullptr);
+    if (e == hipSuccess && rc == HK_OK) e = hipEventRecord(rb.join[b], c->s);
+    if (e == hipSuccess && rc == HK_OK) e = hipStreamWaitEvent(rb.root, rb.join[b], 0);
+  }
+  hipGraph_t gr = nullptr;
+  const hipError_t ee = hipStreamEndCapture(rb.root, &gr);
+  rb.graph = gr;
+  if (rc != HK_OK) return rc;
+  HIPC(c0, e);
+  HIPC(c0, ee);
+  HIPC(c0, hipGraphInstantiate(&rb.exec, rb.graph, nullptr, nullptr, 0));
+  // replay: launch round r + 1 before reading the gates after round r
+  std::vector<char> stopped(n, 0);
+  auto now_s = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  for (int b = 0; b < n; ++b)
+    if (!runs[b].active) {
+      stopped[b] = 1;
+      if (seconds_at_finish) seconds_at_finish[b] = now_s();
+    }
+  auto launch_round = [&](int slot) -> int {
+    HIPC(c0, hipGraphLaunch(rb.exec, rb.root));
+    HIPC(c0, hipMemcpyAsync(rb.h_gates + slot * n, rb.gates, sizeof(int) * n, hipMemcpyDeviceToHost, rb.root));
+    HIPC(c0, hipEventRecord(rb.chk[slot], rb.root));
+    return HK_OK;
+  };
+  CHK(launch_round(0));
+  for (int round = 1;; ++round) {
+    CHK(launch_round(round & 1));
+    const int prev = (round - 1) & 1;
+    HIPC(c0, hipEventSynchronize(rb.chk[prev]));
+    int active = 0;
+    for (int b = 0; b < n; ++b) {
+      if (rb.h_gates[prev * n + b]) {
+        ++active;
+      } else if (!stopped[b]) {
+        stopped[b] = 1;
+        if (seconds_at_finish) seconds_at_finish[b] = now_s();
+      }
+    }
+    if (active == 0) break;
+  }
+  HIPC(c0, hipStreamSynchronize(rb.root));
+  HIPC(c0, hipMemcpy(ctl.data(), rb.ctls, sizeof(FitCtl) * n, hipMemcpyDeviceToHost));
+  for (int b = 0; b < n; ++b) {
+    FitRun& f = runs[b];
+    if (ctl[b].status) return fail(ctxs[b], HK_ENAN, "log-likelihood became non-finite at iteration %d", ctl[b].iter);
+    if (ctl[b].iter > 0) {
+      f.iter = ctl[b].iter;
+      f.ll_new = ctl[b].ll_prev;
+      f.res->converged = ctl[b].converged;
+    }
+    f.active = false;
+  }
+  return HK_OK;
+}
+
+// Several fits of contexts over one genotype upload (hk_ctx_create_shared), each with its own buffers and
+// stream: the EM warm-up in lockstep, then the QN iterations either as one captured graph per round
+// (batch_rounds) or each fit's iteration enqueued as soon as its previous one completed.
 extern "C" int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const double* ll_start,
                             hk_fit_result* res, double* seconds_at_finish) {
   if (!ctxs || n < 1 || !opts || !res) return fail(nullptr, HK_EINVAL, "hk_fit_batch: invalid arguments");
@@ -1075,6 +1222,14 @@ extern "C" int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const
     runs[b].o = &oz;
     CHK(fit_em(runs[b], opts[b].em_iters > 0 ? ll_em[b] : (ll_start ? ll_start[b] : NAN)));
     runs[b].o = &opts[b];
+  }
+  bool rounds = opts[0].batch_rounds != 0;
+  for (int b = 0; b < n; ++b)
+    rounds = rounds && opts[b].use_graph && !opts[b].guarantee_increase && !opts[b].profile && !opts[b].verbose;
+  if (rounds) {
+    CHK(fit_batch_rounds(ctxs, n, opts, runs, seconds_at_finish, t0));
+    for (int b = 0; b < n; ++b) CHK(fit_finish(runs[b]));
+    return HK_OK;
   }
   // QN iterations, event-driven: a fit's next iteration is enqueued as soon as its previous one (and the
   // copy of its log-likelihoods) completed, so no fit waits for the slowest one of a round.  A fit's

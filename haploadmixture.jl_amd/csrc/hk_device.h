@@ -7,6 +7,11 @@
 #define HK_QP_NMAX 32      // 4 * HK_MAXK variables in a P-side QP
 #define HK_WAVE 64
 
+// Gate of a fit's iteration kernels: a fit whose device-side stopping rule has fired (*gate == 0) skips every
+// kernel of later iterations (they return at once), so a graph holding several fits' iterations can be
+// replayed until all of them stopped.  nullptr = always run.
+__device__ __forceinline__ bool gate_off(const int* gate) { return gate != nullptr && *gate == 0; }
+
 // ---------------------------------------------------------------------------------------------
 // Genotype nibble: bits 0-1 = BED code of SNP 2j, bits 2-3 = BED code of SNP 2j+1.
 // BED codes: 0 hom allele-1, 1 missing, 2 het, 3 hom allele-2 (src/loops.jl:63 maps them to

@@ -24,6 +24,7 @@ struct PassGeom {
   int acc32;           // reference Float32 semantics: no FP64 flushes, chunk partials summed in FP32
   void* qq;            // staging table of the P-side curvature pass (TP; rows per individual)
   void* pp;            // staging table of the Q-side curvature pass (TP; rows per block)
+  const int* gate;     // device gate of the fit's iteration kernels (nullptr: always run)
 };
 
 int nacc(int K, int mode);
@@ -62,6 +63,7 @@ struct StepOpts {
   int penalty;  // Q: H + rho 11' without equality; P: rho on the (l, l2) block diagonals
   double rho;
   double eps;   // pseudocount of the simplex projections
+  const int* gate = nullptr;  // device gate (PassGeom::gate)
 };
 // dq / dp (optional): the raw QP steps before projection, for the step-halving line search.
 hipError_t launch_q_step(int K, int64_t I, const double* res_q, const double* q, double* q_next, const StepOpts& o,
@@ -79,20 +81,35 @@ struct PStepLists {
 hipError_t prime_p_step_caps(int K);
 hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
                          int* fails, const PStepLists& lists, hipStream_t s, double* dp = nullptr);
-hipError_t launch_project_q(int K, int64_t I, double* q, double eps, hipStream_t s);
-hipError_t launch_project_p(int K, int64_t J, double* p, double eps, hipStream_t s);
+hipError_t launch_project_q(int K, int64_t I, double* q, double eps, hipStream_t s, const int* gate = nullptr);
+hipError_t launch_project_p(int K, int64_t J, double* p, double eps, hipStream_t s, const int* gate = nullptr);
 hipError_t launch_em_q(int K, int64_t I, int64_t J, const double* res_q, int NT, const int64_t* dm, const double* q,
                        double* q_next, double eps, hipStream_t s);
 hipError_t launch_em_p(int K, int64_t J, const double* res_p, int NT, const double* p, double* p_next, double eps,
                        hipStream_t s);
-hipError_t launch_pair_to_f32(const double* q, int64_t nq, const double* p, int64_t np, float* dst, hipStream_t s);
+hipError_t launch_pair_to_f32(const double* q, int64_t nq, const double* p, int64_t np, float* dst, hipStream_t s,
+                              const int* gate = nullptr);
 // Quasi-Newton extrapolation of admixture_qn! (mode 0: ZAL, 1: LBQN); dots_part holds 512 x 64 doubles.
 hipError_t launch_zal(int64_t N, int Qn, const int* d_iter, double* U, double* V, const double* x0, const double* x1,
-                      const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s, int mode);
+                      const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s, int mode,
+                      const int* gate = nullptr);
 hipError_t launch_accept(int64_t N, double* ll, const double* x_basic, const double* x_qn, double* x, int* d_iter,
-                         hipStream_t s);
-hipError_t launch_copy_f64(int64_t n, const double* src, double* dst, hipStream_t s);
-hipError_t launch_set_int(int* p, int v, hipStream_t s);
+                         hipStream_t s, const int* gate = nullptr);
+hipError_t launch_copy_f64(int64_t n, const double* src, double* dst, hipStream_t s, const int* gate = nullptr);
+// Device-side stopping rule of a fit run as part of a batch graph (admixture_qn!'s reldiff < rtol or
+// n_iter iterations, algorithms_outer.jl:161-165), evaluated after the accept step of every iteration.
+struct FitCtl {
+  double ll_prev;  // ll_new of the previous iteration (before the first: the QN loop's start value)
+  double rtol;
+  int n_iter;
+  int iter;        // QN iterations run
+  int converged;   // 1 once the relative change fell below rtol
+  int status;      // 0, or 1 when the log-likelihood became non-finite
+};
+// Appends ll[2] (ll_new) to trace, updates ctl and clears *gate when the fit stops.
+hipError_t launch_fit_control(const double* ll, FitCtl* ctl, double* trace, int* gate, hipStream_t s);
+
+hipError_t launch_set_int(int* p, int v, hipStream_t s, const int* gate = nullptr);
 hipError_t launch_axpy_step(int64_t n, const double* a, const double* d, double f, double* out, hipStream_t s);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy16(const void* src, void* dst, int64_t n16, hipStream_t s);

@@ -241,7 +241,8 @@ struct NAcc {
 template <int K, typename TP, int MODE>
 __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
                                                int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
-                                               double* __restrict__ part, int flush) {
+                                               double* __restrict__ part, int flush, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   constexpr int NT = NAcc<K, MODE>::v;
   constexpr bool F32 = sizeof(TP) == 4;
   __shared__ __attribute__((aligned(16))) TP lut[128];
@@ -335,7 +336,8 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
 template <int K, typename TP>
 __global__ __launch_bounds__(256) void k_pside_grad(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8,
                                                     int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
-                                                    double* __restrict__ out, int flush) {
+                                                    double* __restrict__ out, int flush, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   constexpr bool F32 = sizeof(TP) == 4;
   __shared__ __attribute__((aligned(16))) TP lut[128];
   fill_pair_lut<TP>(lut);
@@ -474,7 +476,8 @@ struct CurvShape {
 // Rows c >= ncol / sub repeat the last column group.
 template <int K, typename TP>
 __global__ void k_stage_rows(const TP* __restrict__ x, int64_t ncol, int64_t nrow, int sub, int QS, int RG,
-                             TP* __restrict__ out) {
+                             TP* __restrict__ out, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nrow * RG) return;
   const int64_t c = idx / RG;
@@ -701,14 +704,16 @@ __device__ __forceinline__ void pside_mm_body(const uint32_t* __restrict__ gp, i
 template <int K, typename TP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pside_mm_pf(
     const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8, int wpc, const TP* __restrict__ q,
-    const TP* __restrict__ p, const TP* __restrict__ qq, double* __restrict__ out, int flush) {
+    const TP* __restrict__ p, const TP* __restrict__ qq, double* __restrict__ out, int flush, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   pside_mm_body<K, TP, true>(gp, Jp, I, I8, wpc, q, p, qq, out, flush);
 }
 // waves_per_eu(2) keeps K = 8 at two waves per SIMD (and K = 7 at three) without spills.
 template <int K, typename TP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pside_mm(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8,
                                                   int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
-                                                  const TP* __restrict__ qq, double* __restrict__ out, int flush) {
+                                                  const TP* __restrict__ qq, double* __restrict__ out, int flush, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   pside_mm_body<K, TP, false>(gp, Jp, I, I8, wpc, q, p, qq, out, flush);
 }
 
@@ -892,13 +897,15 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
 template <int K, typename TP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_qside_mm_pf(
     const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8, int wpc, const TP* __restrict__ q,
-    const TP* __restrict__ p, const TP* __restrict__ pp, double* __restrict__ part, int flush) {
+    const TP* __restrict__ p, const TP* __restrict__ pp, double* __restrict__ part, int flush, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   qside_mm_body<K, TP, true>(gq, Ip, I, J8, wpc, q, p, pp, part, flush);
 }
 template <int K, typename TP>
 __global__ __launch_bounds__(256) void k_qside_mm(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
                                                   int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
-                                                  const TP* __restrict__ pp, double* __restrict__ part, int flush) {
+                                                  const TP* __restrict__ pp, double* __restrict__ part, int flush, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   qside_mm_body<K, TP, false>(gq, Ip, I, J8, wpc, q, p, pp, part, flush);
 }
 
@@ -926,7 +933,8 @@ __device__ __forceinline__ double chunk_sum(const double* __restrict__ part, int
 // part[c][t][Ip] -> res[i][NT] for i < I, slots t0 + blockIdx.y (thread per (i, t)).
 // acc32: the chunk partials are summed in FP32 (the reference's Float32 accumulators).
 __global__ void k_reduce_q(const double* __restrict__ part, int nchunk, int NT, int t0, int64_t Ip, int64_t I,
-                           double* __restrict__ res, int acc32) {
+                           double* __restrict__ res, int acc32, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int t = t0 + blockIdx.y;
   if (i >= I) return;
@@ -941,7 +949,8 @@ __global__ void k_reduce_q(const double* __restrict__ part, int nchunk, int NT, 
 }
 
 // Curvature pass: Xtz = XtX q (FP64 q) into slots 0..K-1 of res[i][NT], after k_reduce_q filled the XtX slots.
-__global__ void k_xtz_q(int K, int64_t I, const double* __restrict__ q, double* __restrict__ res) {
+__global__ void k_xtz_q(int K, int64_t I, const double* __restrict__ q, double* __restrict__ res, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= I) return;
   const int NT = K + K * (K + 1) / 2;
@@ -965,7 +974,8 @@ __global__ void k_xtz_q(int K, int64_t I, const double* __restrict__ q, double* 
 // lanes of a block are adjacent (4J is a multiple of 4), so the quad's result is a DPP maximum.
 template <int K>
 __global__ void k_finish_p(int64_t J, const double* __restrict__ p, double* __restrict__ res,
-                           unsigned char* __restrict__ flags, int* __restrict__ dense) {
+                           unsigned char* __restrict__ flags, int* __restrict__ dense, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   const int64_t jh = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (jh >= 4 * J) return;
   constexpr int NT = K + K * (K + 1) / 2;
@@ -1028,7 +1038,8 @@ __global__ void k_finish_p(int64_t J, const double* __restrict__ p, double* __re
 
 // part[c][4Jp][NT] -> res[jh][NT] for jh < 4J
 __global__ void k_reduce_p(const double* __restrict__ part, int nchunk, int NT, int64_t Jp, int64_t J,
-                           double* __restrict__ res, int acc32) {
+                           double* __restrict__ res, int acc32, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= 4 * J * NT) return;
   if (acc32) {
@@ -1045,7 +1056,8 @@ __global__ void k_reduce_p(const double* __restrict__ part, int nchunk, int NT, 
 // Log-likelihood, stage 1: rows[i] = sum_c part[c][i]; bpart[b] = sum of the rows of workgroup b (tree).
 __global__ __launch_bounds__(256) void k_reduce_ll_rows(const double* __restrict__ part, int nchunk, int64_t Ip,
                                                         int64_t I, double* __restrict__ rows,
-                                                        double* __restrict__ bpart) {
+                                                        double* __restrict__ bpart, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   __shared__ double sh[256];
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   double r = 0.0;
@@ -1059,7 +1071,8 @@ __global__ __launch_bounds__(256) void k_reduce_ll_rows(const double* __restrict
 
 // Stage 2: total = sum of the nb workgroup sums (fixed order).
 __global__ __launch_bounds__(256) void k_reduce_ll_total(const double* __restrict__ bpart, int nb,
-                                                         double* __restrict__ total) {
+                                                         double* __restrict__ total, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   __shared__ double sh[256];
   double s = 0.0;
   for (int b = threadIdx.x; b < nb; b += 256) s += bpart[b];
@@ -1106,18 +1119,18 @@ static hipError_t qside_k(int mode, const PassGeom& g, const void* q, const void
   const TP* pp = (const TP*)p;
   if (mode == MODE_LL)
     hipLaunchKernelGGL((k_qside<K, TP, MODE_LL>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part,
-                       g.flush_grad);
+                       g.flush_grad, g.gate);
   else if (mode == MODE_GRAD)
     hipLaunchKernelGGL((k_qside<K, TP, MODE_GRAD>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part,
-                       g.flush_grad);
+                       g.flush_grad, g.gate);
   else {
     const dim3 gmm((unsigned)(g.Ip / 128), (unsigned)g.qchunks);
     if constexpr (K <= 6 && sizeof(TP) == 4)
       hipLaunchKernelGGL((k_qside_mm_pf<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
-                         (const TP*)g.pp, part, g.flush_curv);
+                         (const TP*)g.pp, part, g.flush_curv, g.gate);
     else
       hipLaunchKernelGGL((k_qside_mm<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
-                         (const TP*)g.pp, part, g.flush_curv);
+                         (const TP*)g.pp, part, g.flush_curv, g.gate);
   }
   return hipGetLastError();
 }
@@ -1130,15 +1143,15 @@ static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void
   const TP* pp = (const TP*)p;
   if (mode == MODE_GRAD)
     hipLaunchKernelGGL((k_pside_grad<K, TP>), dim3((unsigned)(g.Jp / 256 + (g.Jp % 256 ? 1 : 0)), (unsigned)g.pchunks),
-                       block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out, g.flush_grad);
+                       block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out, g.flush_grad, g.gate);
   else {
     const dim3 gmm((unsigned)(g.Jp / 64), (unsigned)g.pchunks);
     if constexpr (K <= 6 && sizeof(TP) == 4)
       hipLaunchKernelGGL((k_pside_mm_pf<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
-                         (const TP*)g.qq, out, g.flush_curv);
+                         (const TP*)g.qq, out, g.flush_curv, g.gate);
     else
       hipLaunchKernelGGL((k_pside_mm<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
-                         (const TP*)g.qq, out, g.flush_curv);
+                         (const TP*)g.qq, out, g.flush_curv, g.gate);
   }
   return hipGetLastError();
 }
@@ -1151,11 +1164,11 @@ static hipError_t tables_k(int side, const PassGeom& g, const void* x, hipStream
   if (side == 0) {
     const int64_t nrow = g.Jp + S::PAD_ROWS;
     hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGQ + 255) / 256)), dim3(256), 0, s,
-                       (const TP*)x, 4 * g.Jp, nrow, 4, S::QS, S::RGQ, (TP*)g.pp);
+                       (const TP*)x, 4 * g.Jp, nrow, 4, S::QS, S::RGQ, (TP*)g.pp, g.gate);
   } else {
     const int64_t nrow = g.Ip + S::PAD_ROWS;
     hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGP + 255) / 256)), dim3(256), 0, s,
-                       (const TP*)x, g.I, nrow, 1, S::QS, S::RGP, (TP*)g.qq);
+                       (const TP*)x, g.I, nrow, 1, S::QS, S::RGP, (TP*)g.qq, g.gate);
   }
   return hipGetLastError();
 }
@@ -1252,7 +1265,7 @@ hipError_t launch_decode(const PassGeom& g, uint8_t* codes, hipStream_t s) {
 
 hipError_t launch_reduce_q(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s) {
   hipLaunchKernelGGL(k_reduce_q, dim3((unsigned)((g.I + 255) / 256), (unsigned)NT), dim3(256), 0, s, part, nchunk, NT, 0,
-                     g.Ip, g.I, res, g.acc32);
+                     g.Ip, g.I, res, g.acc32, g.gate);
   return hipGetLastError();
 }
 
@@ -1261,27 +1274,27 @@ hipError_t launch_reduce_q_curv(const double* part, int nchunk, int K, const Pas
   const int NT = K + K * (K + 1) / 2;
   const unsigned nb = (unsigned)((g.I + 255) / 256);
   hipLaunchKernelGGL(k_reduce_q, dim3(nb, (unsigned)(NT - K)), dim3(256), 0, s, part, nchunk, NT, K, g.Ip, g.I, res,
-                     g.acc32);
-  hipLaunchKernelGGL(k_xtz_q, dim3(nb), dim3(256), 0, s, K, g.I, q, res);
+                     g.acc32, g.gate);
+  hipLaunchKernelGGL(k_xtz_q, dim3(nb), dim3(256), 0, s, K, g.I, q, res, g.gate);
   return hipGetLastError();
 }
 
 hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* res, unsigned char* flags, int* dense,
                            hipStream_t s) {
   if (flags) {
-    const hipError_t e = launch_set_int(dense, 0, s);
+    const hipError_t e = launch_set_int(dense, 0, s, g.gate);
     if (e != hipSuccess) return e;
   }
   const dim3 grid((unsigned)((4 * g.J + 255) / 256));
   switch (K) {
-    case 1: hipLaunchKernelGGL(k_finish_p<1>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
-    case 2: hipLaunchKernelGGL(k_finish_p<2>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
-    case 3: hipLaunchKernelGGL(k_finish_p<3>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
-    case 4: hipLaunchKernelGGL(k_finish_p<4>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
-    case 5: hipLaunchKernelGGL(k_finish_p<5>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
-    case 6: hipLaunchKernelGGL(k_finish_p<6>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
-    case 7: hipLaunchKernelGGL(k_finish_p<7>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
-    default: hipLaunchKernelGGL(k_finish_p<8>, grid, dim3(256), 0, s, g.J, p, res, flags, dense); break;
+    case 1: hipLaunchKernelGGL(k_finish_p<1>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 2: hipLaunchKernelGGL(k_finish_p<2>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 3: hipLaunchKernelGGL(k_finish_p<3>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 4: hipLaunchKernelGGL(k_finish_p<4>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 5: hipLaunchKernelGGL(k_finish_p<5>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 6: hipLaunchKernelGGL(k_finish_p<6>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 7: hipLaunchKernelGGL(k_finish_p<7>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    default: hipLaunchKernelGGL(k_finish_p<8>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
   }
   return hipGetLastError();
 }
@@ -1289,7 +1302,7 @@ hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* re
 hipError_t launch_reduce_p(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s) {
   int64_t n = 4 * g.J * NT;
   hipLaunchKernelGGL(k_reduce_p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nchunk, NT, g.Jp, g.J, res,
-                     g.acc32);
+                     g.acc32, g.gate);
   return hipGetLastError();
 }
 
@@ -1299,8 +1312,8 @@ hipError_t launch_reduce_ll(const double* part, int nchunk, const PassGeom& g, d
                             hipStream_t s) {
   const int nb = (int)((g.I + 255) / 256);
   double* bpart = const_cast<double*>(part) + (int64_t)nchunk * g.Ip;
-  hipLaunchKernelGGL(k_reduce_ll_rows, dim3((unsigned)nb), dim3(256), 0, s, part, nchunk, g.Ip, g.I, rows, bpart);
-  hipLaunchKernelGGL(k_reduce_ll_total, dim3(1), dim3(256), 0, s, bpart, nb, total);
+  hipLaunchKernelGGL(k_reduce_ll_rows, dim3((unsigned)nb), dim3(256), 0, s, part, nchunk, g.Ip, g.I, rows, bpart, g.gate);
+  hipLaunchKernelGGL(k_reduce_ll_total, dim3(1), dim3(256), 0, s, bpart, nb, total, g.gate);
   return hipGetLastError();
 }
 

@@ -305,7 +305,8 @@ __device__ int wave_qp(const double (&Hrow)[N], double g, double lo, double hi, 
 template <int K, bool PEN>
 __global__ __launch_bounds__(64) void k_q_step(int64_t I, const double* __restrict__ res, const double* __restrict__ q,
                                                double* __restrict__ q_next, double rho, double eps,
-                                               int* __restrict__ fails, double* __restrict__ dq) {
+                                               int* __restrict__ fails, double* __restrict__ dq, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   constexpr bool penalty = PEN;
   constexpr int NT = K + K * (K + 1) / 2;
   __shared__ double lds[K * K > 64 ? K * K : 64];
@@ -404,7 +405,8 @@ template <int K, bool PEN>
 __global__ __launch_bounds__(64) void k_p_step_list(const double* __restrict__ res, const double* __restrict__ p,
                                                     double* __restrict__ p_next, double rho, double eps,
                                                     int* __restrict__ fails, double* __restrict__ dp,
-                                                    const int* __restrict__ redo) {
+                                                    const int* __restrict__ redo, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   __shared__ double lds[4 * K * 4 * K];
   const int n = redo[0];
   for (int idx = blockIdx.x; idx < n; idx += gridDim.x)
@@ -807,7 +809,8 @@ template <int K, bool PEN, bool MULTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p_step4_w2(
     int64_t J, const double* __restrict__ res, const double* __restrict__ p, double* __restrict__ p_next, double rho,
     double eps, double* __restrict__ dp, const unsigned char* __restrict__ flags, const int* __restrict__ dense_list,
-    int ndb, int* __restrict__ fails, int* __restrict__ late) {
+    int ndb, int* __restrict__ fails, int* __restrict__ late, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   p_step4_body<K, PEN, MULTI>(J, res, p, p_next, rho, eps, dp, flags, dense_list, ndb, fails, late);
 }
 template <int K, bool PEN, bool MULTI>
@@ -816,7 +819,8 @@ __global__ __launch_bounds__(256) void k_p_step4(int64_t J, const double* __rest
                                                  double rho, double eps, double* __restrict__ dp,
                                                  const unsigned char* __restrict__ flags,
                                                  const int* __restrict__ dense_list, int ndb, int* __restrict__ fails,
-                                                 int* __restrict__ late) {
+                                                 int* __restrict__ late, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   p_step4_body<K, PEN, MULTI>(J, res, p, p_next, rho, eps, dp, flags, dense_list, ndb, fails, late);
 }
 
@@ -824,7 +828,8 @@ __global__ __launch_bounds__(256) void k_p_step4(int64_t J, const double* __rest
 // Projections and EM normalisation (thread per column / per (k, j)).
 // ------------------------------------------------------------------------------------------------
 template <int K>
-__global__ void k_project_q(int64_t I, double* __restrict__ q, double eps) {
+__global__ void k_project_q(int64_t I, double* __restrict__ q, double eps, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= I) return;
   double v[K];
@@ -835,7 +840,8 @@ __global__ void k_project_q(int64_t I, double* __restrict__ q, double eps) {
   for (int c = 0; c < K; ++c) q[c + (int64_t)K * i] = v[c];
 }
 
-__global__ void k_project_p(int K, int64_t J, double* __restrict__ p, double eps) {
+__global__ void k_project_p(int K, int64_t J, double* __restrict__ p, double eps, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)K * J) return;
   const int64_t j = idx / K;
@@ -883,7 +889,8 @@ __global__ void k_em_p(int K, int64_t J, const double* __restrict__ res, int NT,
 
 // FP32 mirror of a (q, p) pair in one launch: dst = [q (nq) | p (np)].
 __global__ void k_pair_to_f32(const double* __restrict__ q, int64_t nq, const double* __restrict__ p, int64_t np,
-                              float* __restrict__ dst) {
+                              float* __restrict__ dst, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nq) dst[i] = (float)q[i];
   else if (i < nq + np) dst[i] = (float)p[i - nq];
@@ -925,7 +932,8 @@ __device__ __forceinline__ void zal_cols(const int* it, int Qn, int& col, int& n
 __global__ __launch_bounds__(256) void k_zal_dots(int64_t N, const int* __restrict__ d_iter, int Qn,
                                                   double* __restrict__ U, double* __restrict__ V,
                                                   const double* __restrict__ x0, const double* __restrict__ x1,
-                                                  const double* __restrict__ x2, double* __restrict__ part) {
+                                                  const double* __restrict__ x2, double* __restrict__ part, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   __shared__ double sh[256];
   int col, ncols;
   zal_cols(d_iter, Qn, col, ncols);
@@ -976,7 +984,8 @@ __global__ __launch_bounds__(256) void k_zal_dots(int64_t N, const int* __restri
 #define ZAL_NG 16
 __global__ __launch_bounds__(64 * ZAL_NG) void k_zal_solve(const int* __restrict__ d_iter, int Qn,
                                                           const double* __restrict__ part,
-                                                          double* __restrict__ coef, int mode) {
+                                                          double* __restrict__ coef, int mode, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   static_assert(ZAL_NDOT <= 64 && ZAL_NB % ZAL_NG == 0, "one dot product per lane of a group");
   __shared__ double red[ZAL_NG][64];
   __shared__ double dots[ZAL_NDOT];
@@ -1042,7 +1051,8 @@ __global__ __launch_bounds__(64 * ZAL_NG) void k_zal_solve(const int* __restrict
 }
 
 __global__ void k_zal_apply(int64_t N, const int* __restrict__ d_iter, int Qn, const double* __restrict__ V,
-                            const double* __restrict__ x1, const double* __restrict__ coef, double* __restrict__ x_out) {
+                            const double* __restrict__ x1, const double* __restrict__ coef, double* __restrict__ x_out, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N) return;
   int col, ncols;
@@ -1060,7 +1070,8 @@ __global__ void k_zal_apply(int64_t N, const int* __restrict__ d_iter, int Qn, c
 // winning parameter vector is copied into x; block 0 / thread 0 then advances the iteration counter.
 // ------------------------------------------------------------------------------------------------
 __global__ void k_accept(int64_t N, double* __restrict__ ll, const double* __restrict__ x_basic,
-                         const double* __restrict__ x_qn, double* __restrict__ x, int* __restrict__ d_iter) {
+                         const double* __restrict__ x_qn, double* __restrict__ x, int* __restrict__ d_iter, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   const bool take_qn = ll[0] < ll[1];
   const double* src = take_qn ? x_qn : x_basic;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1071,12 +1082,37 @@ __global__ void k_accept(int64_t N, double* __restrict__ ll, const double* __res
   }
 }
 
-__global__ void k_copy_f64(int64_t n, const double* __restrict__ src, double* __restrict__ dst) {
+// The stopping rule of admixture_qn! (algorithms_outer.jl:161-165) on the device: ll[2] holds the iteration's
+// ll_new (k_accept); the same floating-point expression as the host's decides, so a batch graph stops each
+// fit after exactly the iterations hk_fit would run.
+__global__ void k_fit_control(const double* __restrict__ ll, FitCtl* __restrict__ ctl, double* __restrict__ trace,
+                              int* __restrict__ gate) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || *gate == 0) return;
+  const double ll_new = ll[2];
+  const int it = ctl->iter + 1;
+  ctl->iter = it;
+  if (trace) trace[it - 1] = ll_new;
+  const double reldiff = fabs((ll_new - ctl->ll_prev) / ctl->ll_prev);
+  ctl->ll_prev = ll_new;
+  if (!isfinite(ll_new)) {
+    ctl->status = 1;
+    *gate = 0;
+  } else if (reldiff < ctl->rtol) {
+    ctl->converged = 1;
+    *gate = 0;
+  } else if (it >= ctl->n_iter) {
+    *gate = 0;
+  }
+}
+
+__global__ void k_copy_f64(int64_t n, const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) dst[t] = src[t];
 }
 
-__global__ void k_set_int(int* __restrict__ p, int v) {
+__global__ void k_set_int(int* __restrict__ p, int v, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
   if (threadIdx.x == 0) *p = v;
 }
 
@@ -1103,10 +1139,10 @@ hipError_t launch_q_step(int K, int64_t I, const double* res_q, const double* q,
     constexpr int KT = decltype(kc)::value;
     if (o.penalty)
       hipLaunchKernelGGL((k_q_step<KT, true>), dim3((unsigned)I), dim3(64), 0, s, I, res_q, q, q_next, o.rho, o.eps,
-                         fails, dq);
+                         fails, dq, o.gate);
     else
       hipLaunchKernelGGL((k_q_step<KT, false>), dim3((unsigned)I), dim3(64), 0, s, I, res_q, q, q_next, o.rho, o.eps,
-                         fails, dq);
+                         fails, dq, o.gate);
   });
 }
 
@@ -1163,7 +1199,7 @@ hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p,
   if (K < 1 || K > HK_MAXK) return hipErrorInvalidValue;
   const unsigned cap = g_pcap[o.penalty ? 1 : 0][K];
   if (!cap) return hipErrorNotReady;  // prime_p_step_caps(K) was not called at set-up
-  hipError_t e = launch_set_int(lists.late, 0, s);
+  hipError_t e = launch_set_int(lists.late, 0, s, o.gate);
   if (e != hipSuccess) return e;
   constexpr int NDB = 64;  // 256 wavefronts for the early list (it holds a few blocks in the usual case)
   const unsigned nq = (unsigned)((4 * J + 255) / 256), nd = (unsigned)(J < 1024 ? J : 1024);
@@ -1174,12 +1210,12 @@ hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p,
       // keeps the whole device's worth of slots
       if (nq <= cap)
         hipLaunchKernelGGL(single, dim3(NDB + nq), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp,
-                           lists.flags, lists.dense, NDB, fails, lists.late);
+                           lists.flags, lists.dense, NDB, fails, lists.late, o.gate);
       else
         hipLaunchKernelGGL(multi, dim3(NDB + cap), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps, dp,
-                           lists.flags, lists.dense, NDB, fails, lists.late);
+                           lists.flags, lists.dense, NDB, fails, lists.late, o.gate);
       hipLaunchKernelGGL(list_kernel, dim3(nd), dim3(64), 0, s, res_p, p, p_next, o.rho, o.eps, fails, dp,
-                         lists.late);
+                         lists.late, o.gate);
     };
     if (o.penalty)
       run(PStepKernels<KT, true>::single(), PStepKernels<KT, true>::multi(), k_p_step_list<KT, true>);
@@ -1188,16 +1224,16 @@ hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p,
   });
 }
 
-hipError_t launch_project_q(int K, int64_t I, double* q, double eps, hipStream_t s) {
+hipError_t launch_project_q(int K, int64_t I, double* q, double eps, hipStream_t s, const int* gate) {
   return with_k(K, [&](auto kc) {
     constexpr int KT = decltype(kc)::value;
-    hipLaunchKernelGGL((k_project_q<KT>), dim3((unsigned)((I + 255) / 256)), dim3(256), 0, s, I, q, eps);
+    hipLaunchKernelGGL((k_project_q<KT>), dim3((unsigned)((I + 255) / 256)), dim3(256), 0, s, I, q, eps, gate);
   });
 }
 
-hipError_t launch_project_p(int K, int64_t J, double* p, double eps, hipStream_t s) {
+hipError_t launch_project_p(int K, int64_t J, double* p, double eps, hipStream_t s, const int* gate) {
   const int64_t n = (int64_t)K * J;
-  hipLaunchKernelGGL(k_project_p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, K, J, p, eps);
+  hipLaunchKernelGGL(k_project_p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, K, J, p, eps, gate);
   return hipGetLastError();
 }
 
@@ -1217,8 +1253,9 @@ hipError_t launch_em_p(int K, int64_t J, const double* res_p, int NT, const doub
   return hipGetLastError();
 }
 
-hipError_t launch_pair_to_f32(const double* q, int64_t nq, const double* p, int64_t np, float* dst, hipStream_t s) {
-  hipLaunchKernelGGL(k_pair_to_f32, dim3((unsigned)((nq + np + 255) / 256)), dim3(256), 0, s, q, nq, p, np, dst);
+hipError_t launch_pair_to_f32(const double* q, int64_t nq, const double* p, int64_t np, float* dst, hipStream_t s,
+                              const int* gate) {
+  hipLaunchKernelGGL(k_pair_to_f32, dim3((unsigned)((nq + np + 255) / 256)), dim3(256), 0, s, q, nq, p, np, dst, gate);
   return hipGetLastError();
 }
 
@@ -1229,19 +1266,24 @@ hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
 }
 
 hipError_t launch_accept(int64_t N, double* ll, const double* x_basic, const double* x_qn, double* x, int* d_iter,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_accept, dim3(1024), dim3(256), 0, s, N, ll, x_basic, x_qn, x, d_iter);
+                         hipStream_t s, const int* gate) {
+  hipLaunchKernelGGL(k_accept, dim3(1024), dim3(256), 0, s, N, ll, x_basic, x_qn, x, d_iter, gate);
   return hipGetLastError();
 }
 
-hipError_t launch_copy_f64(int64_t n, const double* src, double* dst, hipStream_t s) {
+hipError_t launch_copy_f64(int64_t n, const double* src, double* dst, hipStream_t s, const int* gate) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_copy_f64, dim3(1024), dim3(256), 0, s, n, src, dst);
+  hipLaunchKernelGGL(k_copy_f64, dim3(1024), dim3(256), 0, s, n, src, dst, gate);
   return hipGetLastError();
 }
 
-hipError_t launch_set_int(int* p, int v, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, p, v);
+hipError_t launch_fit_control(const double* ll, FitCtl* ctl, double* trace, int* gate, hipStream_t s) {
+  hipLaunchKernelGGL(k_fit_control, dim3(1), dim3(64), 0, s, ll, ctl, trace, gate);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_int(int* p, int v, hipStream_t s, const int* gate) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, p, v, gate);
   return hipGetLastError();
 }
 
@@ -1256,11 +1298,12 @@ hipError_t launch_axpy_step(int64_t n, const double* a, const double* d, double 
 }
 
 hipError_t launch_zal(int64_t N, int Qn, const int* d_iter, double* U, double* V, const double* x0, const double* x1,
-                      const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s, int mode) {
+                      const double* x2, double* x_out, double* dots_part, double* coef, hipStream_t s, int mode,
+                      const int* gate) {
   if (Qn < 1 || Qn > ZAL_QMAX || mode < 0 || mode > 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_zal_dots, dim3(ZAL_NB), dim3(256), 0, s, N, d_iter, Qn, U, V, x0, x1, x2, dots_part);
-  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(64 * ZAL_NG), 0, s, d_iter, Qn, dots_part, coef, mode);
-  hipLaunchKernelGGL(k_zal_apply, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, d_iter, Qn, V, x1, coef, x_out);
+  hipLaunchKernelGGL(k_zal_dots, dim3(ZAL_NB), dim3(256), 0, s, N, d_iter, Qn, U, V, x0, x1, x2, dots_part, gate);
+  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(64 * ZAL_NG), 0, s, d_iter, Qn, dots_part, coef, mode, gate);
+  hipLaunchKernelGGL(k_zal_apply, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, d_iter, Qn, V, x1, coef, x_out, gate);
   return hipGetLastError();
 }
 

@@ -42,7 +42,7 @@ class FitOpts(ctypes.Structure):
         ("em_iters", _int), ("n_iter", _int), ("rtol", _dbl), ("Qn", _int),
         ("fix_q", _int), ("fix_p", _int), ("q_penalty", _int), ("p_penalty", _int),
         ("rho", _dbl), ("eps", _dbl), ("verbose", _int), ("profile", _int), ("use_graph", _int),
-        ("em_fix_q", _int), ("guarantee_increase", _int), ("qn_mode", _int),
+        ("em_fix_q", _int), ("guarantee_increase", _int), ("qn_mode", _int), ("batch_rounds", _int),
     ]
 
 

@@ -2,9 +2,10 @@
 
 The reference fits one K per ``run_admixture`` call (src/driver.jl:38) and only mentions AIC for choosing
 K (README.md:9).  Here all fits share one genotype upload (``HipAdmixData.shared``), the chunk shapes of
-each K's passes are chosen by the autotuner (``hk_autotune``), and the fits run concurrently, one
-captured hipGraph launch per fit per iteration, each fit's next iteration enqueued as soon as its previous
-one completed (``hk_fit_batch``).
+each K's passes are chosen by the autotuner (``hk_autotune``), and the fits run concurrently
+(``hk_fit_batch``): every fit's QN iteration is captured into ONE graph per round, replayed until each
+fit's device-side stopping rule has switched its kernels off (with batch_rounds=False, each fit's
+iteration is enqueued on its own stream as soon as its previous one completed).
 
 AIC = -2 ll + 2 (I (K - 1) + 3 K J): I (K-1) free admixture proportions and 3 free haplotype frequencies
 per (population, block).
@@ -42,7 +43,11 @@ def _seed(K: int, r: int, base: int) -> int:
 
 
 def select_k(bed, I, J, Ks=(2, 3, 4, 5, 6, 7, 8), restarts=4, seed=1, precision="mixed", rtol=1e-7,
-             n_iter=1000, em_iters=5, Q=3, autotune=True, batched=True, use_graph=True) -> SelectResult:
+             n_iter=1000, em_iters=5, Q=3, autotune=True, batched=True, use_graph=True,
+             batch_rounds=True) -> SelectResult:
+    """batched: hk_fit_batch over all fits (batch_rounds: one captured graph per round holding every fit's
+    iteration, else each fit's iteration enqueued on its own as soon as its previous one completed);
+    otherwise the fits run one after another."""
     Ks = list(Ks)
     base = HipAdmixData(bed, I, J, Ks[0], precision=precision)
     engines, specs = [], []
@@ -64,7 +69,7 @@ def select_k(bed, I, J, Ks=(2, 3, 4, 5, 6, 7, 8), restarts=4, seed=1, precision=
                     if k == K and e is not first:
                         e.set_tuning(qc, pc)
         opts = [HipAdmixData.default_opts(em_iters=em_iters, n_iter=n_iter, rtol=rtol, Qn=Q,
-                                          use_graph=int(use_graph)) for _ in engines]
+                                          use_graph=int(use_graph), batch_rounds=int(batch_rounds)) for _ in engines]
         t0 = time.perf_counter()
         if batched:
             res, fin = fit_batch(engines, opts)

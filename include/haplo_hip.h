@@ -155,6 +155,11 @@ typedef struct {
                              (update_q!/update_p!, algorithms_inner.jl:170-199,314-343); eager path */
   int qn_mode;           /* admixture_qn!(mode=...) (algorithms_outer.jl:39,128-141): 0 = :ZAL (default),
                             1 = :LBQN, limited-memory Broyden of the second kind on the same secants     */
+  int batch_rounds;      /* hk_fit_batch (read from the first fit's options): 1 (default) = every fit's QN
+                            iteration in ONE captured graph per round, replayed with a device-side stopping
+                            rule per fit; 0 = each fit's iteration enqueued on its own stream as soon as its
+                            previous one completed.  Fits with guarantee_increase, profile or verbose, or
+                            use_graph = 0, always take the second form. */
 } hk_fit_opts;
 
 typedef struct {
@@ -183,9 +188,11 @@ int hk_fit(hk_ctx* ctx, const hk_fit_opts* opts, double ll_start, double* ll_tra
 int hk_fit_ex(hk_ctx* ctx, const hk_fit_opts* opts, double ll_start, double* ll_trace, double* t_trace,
               hk_fit_result* res);
 
-/* Fits of several contexts run concurrently, each on its own stream: a fit's next iteration (one captured
- * graph with use_graph) is enqueued as soon as its previous one completed, without waiting for the other
- * fits.  Each fit's results are those of hk_fit_ex on its context.  opts, ll_start (may be NULL) and res have n entries; seconds_at_finish
+/* Fits of several contexts (typically hk_ctx_create_shared over one upload, uneven K) run concurrently: the EM
+ * warm-up in lockstep, then the QN iterations of all fits as one captured graph per round whose kernels each
+ * fit's device-side stopping rule switches off once it stops (batch_rounds, the default), or each fit's
+ * iteration enqueued as soon as its previous one completed.  Each fit's results are those of hk_fit_ex on its
+ * context.  opts, ll_start (may be NULL) and res have n entries; seconds_at_finish
  * (may be NULL) receives the wall time at which each fit stopped. */
 int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const double* ll_start, hk_fit_result* res,
                  double* seconds_at_finish);

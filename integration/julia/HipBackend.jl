@@ -192,7 +192,7 @@ update_p_cuda!(d::HipAdmixData, ::Nothing) =
 struct HkFitOpts                                      # hk_fit_opts
     em_iters::Cint; n_iter::Cint; rtol::Cdouble; Qn::Cint; fix_q::Cint; fix_p::Cint
     q_penalty::Cint; p_penalty::Cint; rho::Cdouble; eps::Cdouble; verbose::Cint; profile::Cint
-    use_graph::Cint; em_fix_q::Cint; guarantee_increase::Cint; qn_mode::Cint
+    use_graph::Cint; em_fix_q::Cint; guarantee_increase::Cint; qn_mode::Cint; batch_rounds::Cint
 end
 
 struct HkFitResult                                    # hk_fit_result
@@ -213,7 +213,7 @@ function hip_fit!(d::AdmixData2, dh::HipAdmixData; em_iters=5, n_iter=1000, rtol
     hk_check(dh.h, ccall((:hk_set_params, libhk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
                          dh.h, Array{Float64}(d.q), Array{Float64}(d.p)))
     o = Ref(HkFitOpts(em_iters, n_iter, rtol, Q, fix_q, fix_p, 0, 1, 1e7, 1e-5, verbose, 0, 1, em_fix_q,
-                      guarantee_increase, mode == :LBQN ? 1 : 0))
+                      guarantee_increase, mode == :LBQN ? 1 : 0, 1))
     res = Ref{HkFitResult}()
     trace = zeros(Float64, max(n_iter, 1))
     hk_check(dh.h, ccall((:hk_fit, libhk), Cint, (Ptr{Cvoid}, Ref{HkFitOpts}, Cdouble, Ptr{Float64}, Ref{HkFitResult}),
@@ -260,7 +260,7 @@ function hip_select(dh::HipAdmixData, Ks, restarts; rtol=1e-7, seed=1, precision
             end
         end
         n = length(ctxs)
-        opts = [HkFitOpts(5, n_iter, rtol, 3, 0, 0, 0, 1, 1e7, 1e-5, 0, 0, 1, 0, 0, 0) for _ in 1:n]
+        opts = [HkFitOpts(5, n_iter, rtol, 3, 0, 0, 0, 1, 1e7, 1e-5, 0, 0, 1, 0, 0, 0, 1) for _ in 1:n]
         res = Vector{HkFitResult}(undef, n)
         fin = zeros(Float64, n)
         rc = ccall((:hk_fit_batch, libhk), Cint, (Ptr{Ptr{Cvoid}}, Cint, Ptr{HkFitOpts}, Ptr{Float64},

@@ -41,8 +41,11 @@ def test_shared_context_matches_own_context():
     assert np.array_equal(r1["trace"], r2["trace"])
 
 
-def test_batch_matches_sequential_fits():
-    """hk_fit_batch over uneven K gives the same fits as one-at-a-time hk_fit."""
+@pytest.mark.parametrize("rounds", [1, 0], ids=["round_graph", "event_driven"])
+def test_batch_matches_sequential_fits(rounds):
+    """hk_fit_batch over uneven K gives the same fits as one-at-a-time hk_fit, in both batch forms: one
+    captured graph per round with device-side stopping rules (fits of 3..29 iterations, so most rounds
+    replay converged fits' gated kernels), and each fit's iteration enqueued as its previous one completed."""
     I, J = 400, 500
     s = simulate(I, J, 4, fst=0.1, miss=0.02, seed=9)
     Ks = [2, 3, 5, 3]
@@ -51,7 +54,7 @@ def test_batch_matches_sequential_fits():
         engs = [base] + [HipAdmixData.shared(base, K, "mixed" if t % 2 else "fp64") for t, K in enumerate(Ks[1:], 1)]
         for e, (q0, p0) in zip(engs, inits):
             e.set_params(q0, p0)
-        opts = [HipAdmixData.default_opts(rtol=1e-8, n_iter=200) for _ in engs]
+        opts = [HipAdmixData.default_opts(rtol=1e-8, n_iter=200, batch_rounds=rounds) for _ in engs]
         res, fin = fit_batch(engs, opts)
         seq = [_fit(e, q0, p0, rtol=1e-8, n_iter=200) for e, (q0, p0) in zip(engs, inits)]
         for e in engs[1:]:
@@ -84,3 +87,26 @@ def test_select_k_prefers_true_k():
     lls = {K: out.best_by_k[K]["ll"] for K in (2, 3, 4, 5)}
     assert lls[3] > lls[2] and lls[4] >= lls[3] - 1e-6 * abs(lls[3])
     assert out.best_k == 3, [(K, out.best_by_k[K]["aic"]) for K in (2, 3, 4, 5)]
+
+
+def test_round_graph_batch_respects_n_iter_and_modes():
+    """Round-graph batch with uneven stopping: fits capped by n_iter (3 and 7 iterations), one with
+    rtol 0 run to its cap, fix_q and LBQN options, an inactive fit (n_iter = 0); each equals its single fit."""
+    I, J = 300, 400
+    s = simulate(I, J, 3, fst=0.1, miss=0.02, seed=5)
+    specs = [(2, dict(n_iter=3, rtol=0.0)), (3, dict(n_iter=7, rtol=1e-8)), (4, dict(n_iter=40, rtol=1e-9)),
+             (3, dict(n_iter=5, rtol=0.0, fix_q=1)), (3, dict(n_iter=6, rtol=1e-10, qn_mode=1)),
+             (2, dict(n_iter=0))]
+    with HipAdmixData(s.bed, I, J, 2, precision="fp64") as base:
+        engs = [base] + [HipAdmixData.shared(base, K, "fp64") for K, _ in specs[1:]]
+        inits = [init_params(I, J, K, seed=70 + t) for t, (K, _) in enumerate(specs)]
+        opts = [HipAdmixData.default_opts(batch_rounds=1, **kw) for _, kw in specs]
+        for e, (q0, p0) in zip(engs, inits):
+            e.set_params(q0, p0)
+        res, _ = fit_batch(engs, opts)
+        seq = [_fit(e, q0, p0, **kw) for e, (q0, p0), (_, kw) in zip(engs, inits, specs)]
+        for e in engs[1:]:
+            e.close()
+    for a, b, (K, kw) in zip(res, seq, specs):
+        assert a["iters"] == b["iters"] and a["ll"] == b["ll"] and a["converged"] == b["converged"], (K, kw)
+    assert res[0]["iters"] == 3 and res[1]["iters"] <= 7 and res[5]["iters"] == 0
