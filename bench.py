@@ -158,7 +158,7 @@ def warmstart_bench(a, w, prec):
     from haplo_amd import warmstart as ws
 
     I, J, K = w["I"], w["J"], w["K"]
-    mlp = ws.load_default(K)
+    mlp = ws.MLP.load(a.weights) if a.weights else ws.load_default(K)
     if mlp is None:
         raise SystemExit(f"no warm-start network for K={K} in {ws.DATA_DIR}")
     log("warm-up (kernel loading)")
@@ -316,8 +316,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3,
                     help="fits run after the timed ones with HIP events around every pass (per-kernel times)")
     ap.add_argument("--eager", action="store_true",
-                    help="model selection: enqueue every iteration's kernels directly instead of replaying one "
-                         "captured graph per fit (for kernel traces)")
+                    help="enqueue every iteration's kernels directly instead of replaying captured graphs (for "
+                         "kernel traces under rocprofv3)")
+    ap.add_argument("--weights", default=None, help="warm start: network weights (.npz) instead of the shipped ones")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the timed fits, run a plain copy kernel of this size (counter calibration)")
     a = ap.parse_args()
@@ -350,7 +351,7 @@ def main():
         # verbose=0: no per-EM-iteration progress lines, so the EM warm-up evaluates only the log-likelihood
         # the fit uses (the last one); every QN iteration's log-likelihoods are computed as always
         eng.set_params(q0, p0)
-        return eng.fit(rtol=a.rtol, n_iter=a.max_iter, profile=profile, verbose=0)
+        return eng.fit(rtol=a.rtol, n_iter=a.max_iter, profile=profile, verbose=0, use_graph=int(not a.eager))
 
     for wi in range(a.warmup):
         r = one_fit()

@@ -86,3 +86,77 @@ def test_features_centre_and_ignore_missing():
     F2 = ow.features(codes[:, full], S, 77)
     assert np.allclose(F2.sum(axis=0), 0, atol=1e-12)
     assert F.shape == (I, ow.D_FEAT)
+
+
+def _torch():
+    return pytest.importorskip("torch")
+
+
+@pytest.mark.parametrize("match", [False, True])
+def test_forward_and_gradient_match_torch_autograd(match):
+    """forward / loss_grad against torch's own softmax cross-entropy and autograd in float64, with the same
+    BF16-rounded weight matrices (the gradient w.r.t. the FP32 masters is the one w.r.t. the rounded
+    weights: the rounding passes the gradient straight through)."""
+    torch = _torch()
+    D, H, K, n = 9, 7, 4, 23
+    w, offs = _net(D, H, K, 11)
+    rng = np.random.default_rng(12)
+    X = rng.normal(size=(n, D))
+    T = rng.dirichlet(np.ones(K) * 0.7, size=n)
+    P, _ = ow.forward(w, offs, D, H, K, X)
+    loss, perm, g = ow.loss_grad(w, offs, D, H, K, X, T, match=match)
+    W1, b1, W2, b2, W3, b3 = ow.unpack(w, offs, D, H, K)
+    leaves = [torch.tensor(a, dtype=torch.float64, requires_grad=True)
+              for a in (ow.bf16(W1), b1, ow.bf16(W2), b2, ow.bf16(W3), b3)]
+    tW1, tb1, tW2, tb2, tW3, tb3 = leaves
+    Xt = torch.tensor(X, dtype=torch.float64)
+    Z = torch.relu(torch.relu(Xt @ tW1 + tb1) @ tW2 + tb2) @ tW3 + tb3
+    Pt = torch.softmax(Z, dim=1)
+    assert np.max(np.abs(Pt.detach().numpy() - P)) <= 1e-13
+    Tt = torch.tensor(T[:, perm], dtype=torch.float64)
+    lt = -(Tt * torch.log_softmax(Z, dim=1)).sum() / n
+    lt.backward()
+    assert abs(lt.item() - loss) <= 1e-12 * abs(loss)
+    for leaf, o in zip(leaves, offs):
+        gt = leaf.grad.numpy().ravel()
+        assert np.max(np.abs(g[o:o + gt.size] - gt)) <= 1e-12 * max(1.0, np.max(np.abs(gt)))
+    if match:  # the matched permutation is the one with the smallest torch loss among all relabellings
+        Zd = torch.log_softmax(Z.detach(), dim=1)
+        best = min(itertools.permutations(range(K)), key=lambda p: float(-(torch.tensor(T[:, list(p)]) * Zd).sum()))
+        assert tuple(perm) == best
+
+
+def test_adam_matches_torch_optimizer():
+    """Three bias-corrected Adam steps against torch.optim.Adam (same lr, betas, eps) in float64."""
+    torch = _torch()
+    rng = np.random.default_rng(13)
+    w = rng.normal(size=50)
+    grads = [rng.normal(size=50) * s for s in (1.0, 0.1, 3.0)]
+    t = torch.tensor(w, dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([t], lr=1e-3, betas=(0.9, 0.999), eps=1e-8)
+    m = np.zeros_like(w)
+    v = np.zeros_like(w)
+    for step, g in enumerate(grads, 1):
+        opt.zero_grad()
+        t.grad = torch.tensor(g, dtype=torch.float64)
+        opt.step()
+        w, m, v = ow.adam(w, g, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8)
+        assert np.max(np.abs(t.detach().numpy() - w)) <= 1e-15 * max(1.0, np.max(np.abs(w)))
+
+
+def test_shipped_network_has_its_training_record():
+    """The shipped K=5 weights come with the record of the run that produced them (haplo_amd.warmstart.
+    train_default): 200 plain-method fits, the loss per epoch, the device time of each stage."""
+    import json
+    import os
+
+    from haplo_amd import warmstart as ws
+
+    path = ws.default_weights_path(5)
+    rec = json.load(open(os.path.splitext(path)[0] + ".json"))
+    z = np.load(path)
+    assert rec["weights"] == os.path.basename(path) and int(z["K"]) == rec["K"] == 5
+    assert rec["datasets"]["n"] == 200 and rec["rows"] == 200 * rec["datasets"]["I"]
+    hist = rec["loss_per_epoch"]
+    assert len(hist) == rec["training"]["epochs"] and hist[-1] < hist[0]
+    assert all(v > 0 for v in rec["seconds"].values())
