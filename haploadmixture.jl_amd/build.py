@@ -39,6 +39,11 @@ def _run(cmd):
     return r.stdout
 
 
+# translation units with flags of their own: the log-likelihood pass writes its matrix-instruction results to
+# VGPRs (its per-pair arithmetic reads every value once; accumulation registers would need a move per value)
+EXTRA_FLAGS = {"hk_ll.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def build_hip(jobs: int = 4) -> str:
     os.makedirs(LIB, exist_ok=True)
     objdir = os.path.join(HERE, "_build")
@@ -48,7 +53,8 @@ def build_hip(jobs: int = 4) -> str:
     for src in HIP_SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, *HIP_FLAGS, *EXTRA_FLAGS.get(src, []), "-I", os.path.join(ROOT, "include"), "-c",
+                     os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         list(ex.map(_run, cmds))
     out = os.path.join(LIB, "libhaplo_hip.so")

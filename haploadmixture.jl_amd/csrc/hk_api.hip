@@ -74,7 +74,9 @@ static void plan_chunks(hk_ctx* c) {
 }
 
 static int alloc_parts(hk_ctx* c) {
-  const int NT = ntq(c->K);
+  // the curvature slots; at least 3 rows so that the two-set log-likelihood sweep's partials (2 x qchunks x Ip)
+  // and its workgroup sums fit
+  const int NT = ntq(c->K) > 3 ? ntq(c->K) : 3;
   const int64_t lq = (int64_t)c->g.qchunks * NT * c->g.Ip;
   const int64_t lp = c->g.pchunks > 1 ? (int64_t)c->g.pchunks * 4 * c->g.Jp * NT : 0;
   if (lq > c->part_q_len) {
@@ -97,16 +99,17 @@ static double* pof(hk_ctx* c, double* x) { return x + (int64_t)c->K * c->I; }
 
 // Pointer handed to a pass for parameters (q, p): FP64 directly, or the FP32 mirror.
 static int pass_params(hk_ctx* c, const double* q, const double* p, const void** qq, const void** pp,
-                       const int* gate = nullptr) {
+                       const int* gate = nullptr, float* mirror = nullptr) {
   if (c->prec == PREC_FP64) {
     *qq = q;
     *pp = p;
     return HK_OK;
   }
+  float* m = mirror ? mirror : c->x32;
   const int64_t nq = (int64_t)c->K * c->I, np = (int64_t)c->K * 4 * c->g.Jp;
-  HIPC(c, launch_pair_to_f32(q, nq, p, np, c->x32, c->s, gate));
-  *qq = c->x32;
-  *pp = c->x32 + nq;
+  HIPC(c, launch_pair_to_f32(q, nq, p, np, m, c->s, gate));
+  *qq = m;
+  *pp = m + nq;
   return HK_OK;
 }
 
@@ -169,7 +172,10 @@ static int run_qpass(hk_ctx* c, int mode, const double* q, const double* p, Time
   CHK(pass_params(c, q, p, &qq, &pp, gate));
   if (mode == MODE_CURV) HIPC(c, launch_curv_tables(c->K, c->prec, 0, g, qq, pp, c->s));
   if (tm) tm->begin(cat);
-  HIPC(c, launch_qside(c->K, c->prec, mode, g, qq, pp, c->part_q, c->s));
+  if (mode == MODE_LL && c->prec != PREC_FP64)
+    HIPC(c, launch_ll_mm(c->K, g, (const float*)qq, (const float*)pp, nullptr, nullptr, c->part_q, c->s));
+  else
+    HIPC(c, launch_qside(c->K, c->prec, mode, g, qq, pp, c->part_q, c->s));
   if (tm) tm->end();
   if (mode == MODE_LL)
     HIPC(c, launch_reduce_ll(c->part_q, g.qchunks, g, c->ll_rows, ll_out ? ll_out : c->ll_dev, c->s));
@@ -201,6 +207,29 @@ static int run_ppass(hk_ctx* c, int mode, const double* q, const double* p, Time
   return HK_OK;
 }
 
+// Log-likelihoods of two parameter sets -> ll1 and ll2 (device).  MIXED: one genotype sweep of the matrix-core
+// pass for both; FP64: two passes.
+static int run_ll_pair(hk_ctx* c, const double* q1, const double* p1, const double* q2, const double* p2, double* ll1,
+                       double* ll2, Timer* tm = nullptr, int cat = 0, const int* gate = nullptr) {
+  if (c->prec == PREC_FP64) {
+    CHK(run_qpass(c, MODE_LL, q1, p1, tm, cat, ll1, gate));
+    CHK(run_qpass(c, MODE_LL, q2, p2, tm, cat, ll2, gate));
+    return HK_OK;
+  }
+  PassGeom g = c->g;
+  g.gate = gate;
+  const void *qa, *pa, *qb, *pb;
+  CHK(pass_params(c, q1, p1, &qa, &pa, gate, c->x32));
+  CHK(pass_params(c, q2, p2, &qb, &pb, gate, c->x32b));
+  if (tm) tm->begin(cat);
+  HIPC(c, launch_ll_mm(c->K, g, (const float*)qa, (const float*)pa, (const float*)qb, (const float*)pb, c->part_q, c->s));
+  if (tm) tm->end();
+  // set 2 first: set 1's workgroup sums go where set 2's row partials were
+  HIPC(c, launch_reduce_ll(c->part_q + (int64_t)g.qchunks * g.Ip, g.qchunks, g, c->ll_rows, ll2, c->s));
+  HIPC(c, launch_reduce_ll(c->part_q, g.qchunks, g, c->ll_rows, ll1, c->s));
+  return HK_OK;
+}
+
 static int ensure_full(hk_ctx* c, int64_t n) {
   if (n <= c->full_len) return HK_OK;
   if (c->full_a) HIPC(c, hipFree(c->full_a));
@@ -223,7 +252,7 @@ int hk_ctx_destroy(hk_ctx* c) {
   if (c->s) (void)hipStreamSynchronize(c->s);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
-  void* owned[] = {c->qq, c->pp, c->xb[0], c->xb[1], c->xb[2], c->xb[3], c->x32, c->res_q, c->res_p,
+  void* owned[] = {c->qq, c->pp, c->xb[0], c->xb[1], c->xb[2], c->xb[3], c->x32, c->x32b, c->res_q, c->res_p,
                    c->part_q, c->part_p, c->ll_dev, c->ll_rows, c->U, c->V, c->zal_part, c->zal_coef, c->fails,
                    c->redo, c->redo_late, c->pflag, c->full_a, c->full_b, c->d_ll, c->d_iter,
                    c->named[0], c->named[1], c->named[2], c->named[3]};
@@ -260,7 +289,10 @@ static int init_fit_buffers(hk_ctx* c) {
     HIPC(c, launch_fill(c->xb[b], (int64_t)K * I, 1.0 / K, c->s));
     HIPC(c, launch_fill(c->xb[b] + (int64_t)K * I, (int64_t)K * 4 * g.Jp, 0.25, c->s));
   }
-  if (c->prec == HK_PREC_MIXED) HIPC(c, hipMalloc(&c->x32, sizeof(float) * c->xlen));
+  if (c->prec == HK_PREC_MIXED) {
+    HIPC(c, hipMalloc(&c->x32, sizeof(float) * c->xlen));
+    HIPC(c, hipMalloc(&c->x32b, sizeof(float) * c->xlen));
+  }
   const int NT = ntq(K);
   HIPC(c, hipMalloc(&c->res_q, sizeof(double) * I * NT));
   HIPC(c, hipMalloc(&c->res_p, sizeof(double) * 4 * g.Jp * NT));
@@ -806,14 +838,16 @@ static int enqueue_iteration(hk_ctx* c, const hk_fit_opts* o, Timer* tm, double 
   else HIPC(c, launch_copy_f64(nq, qof(c, x), qof(c, xn2), c->s, gate));
   if (!o->fix_p) CHK(pstep(qof(c, xn2), pof(c, xn), pof(c, xn2)));
   else HIPC(c, launch_copy_f64(np, pof(c, x), pof(c, xn2), c->s, gate));
-  CHK(run_qpass(c, MODE_LL, qof(c, xn2), pof(c, xn2), tm, T_LL, c->d_ll + 0, gate));
+  // The QN point needs only x, F(x), F(F(x)) (algorithms_outer.jl:128-144), so it is formed first and both
+  // log-likelihoods -- ll_basic of F(F(x)) (:119-126) and ll_qn of the projected QN point (:146-153) -- come
+  // from one genotype sweep.
   if (tm) tm->begin(T_QN);
   HIPC(c, launch_zal(N, o->Qn, c->d_iter, c->U, c->V, x, xn, xn2, xt, c->zal_part, c->zal_coef, c->s, o->qn_mode,
                      gate));
   HIPC(c, launch_project_p(K, J, pof(c, xt), o->eps, c->s, gate));
   HIPC(c, launch_project_q(K, I, qof(c, xt), o->eps, c->s, gate));
   if (tm) tm->end();
-  CHK(run_qpass(c, MODE_LL, qof(c, xt), pof(c, xt), tm, T_LL, c->d_ll + 1, gate));
+  CHK(run_ll_pair(c, qof(c, xn2), pof(c, xn2), qof(c, xt), pof(c, xt), c->d_ll + 0, c->d_ll + 1, tm, T_LL, gate));
   HIPC(c, launch_accept(N, c->d_ll, xn2, xt, x, c->d_iter, c->s, gate));
   if (ctl) HIPC(c, launch_fit_control(c->d_ll, ctl, trace, gate, c->s));
   return HK_OK;
@@ -1361,7 +1395,10 @@ extern "C" int hk_time_pass(hk_ctx* c, int kind, int reps, double* ms) {
       break;
     }
     (void)hipEventRecord(a, c->s);
-    if (kind == 0) e = launch_qside(c->K, c->prec, MODE_LL, c->g, qq, pp, c->part_q, c->s);
+    if (kind == 0)
+      e = c->prec == PREC_FP64 ? launch_qside(c->K, c->prec, MODE_LL, c->g, qq, pp, c->part_q, c->s)
+                               : launch_ll_mm(c->K, c->g, (const float*)qq, (const float*)pp, nullptr, nullptr,
+                                              c->part_q, c->s);
     else if (kind == 1) e = launch_qside(c->K, c->prec, MODE_CURV, c->g, qq, pp, c->part_q, c->s);
     else if (kind == 2)
       e = launch_pside(c->K, c->prec, MODE_CURV, c->g, qq, pp, c->g.pchunks == 1 ? c->res_p : c->part_p, c->s);

@@ -42,6 +42,10 @@ hipError_t launch_curv_tables(int K, int prec, int side, const PassGeom& g, cons
                               hipStream_t s);
 hipError_t launch_qside(int K, int prec, int mode, const PassGeom& g, const void* q, const void* p, double* part,
                         hipStream_t s);
+// Log-likelihood pass of the MIXED engine on the matrix cores: one parameter set (q2 = p2 = nullptr) or two in
+// one genotype sweep; per-chunk row partials of set s at part + s * qchunks * Ip (launch_reduce_ll on each).
+hipError_t launch_ll_mm(int K, const PassGeom& g, const float* q1, const float* p1, const float* q2, const float* p2,
+                        double* part, hipStream_t s);
 hipError_t launch_pside(int K, int prec, int mode, const PassGeom& g, const void* q, const void* p, double* out,
                         hipStream_t s);
 hipError_t launch_reduce_q(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s);
@@ -146,6 +150,7 @@ struct hk_ctx {
   double* xb[4] = {nullptr, nullptr, nullptr, nullptr};  // parameter buffers [q | p | pad]
   int64_t xlen = 0;                                      // K * (I + 4 Jp)
   float* x32 = nullptr;                                  // FP32 mirror of one parameter buffer
+  float* x32b = nullptr;                                 // second mirror (two-set log-likelihood sweep)
   double* res_q = nullptr;   // [I][NT]
   double* res_p = nullptr;   // [4Jp][NT]
   double* part_q = nullptr;  // [qchunks][NT][Ip]

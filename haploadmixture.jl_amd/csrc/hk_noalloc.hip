@@ -112,7 +112,9 @@ extern "C" int hk_loglik_noalloc(hk_ctx* c, const double* q, const double* p, do
     hipLaunchKernelGGL(k_noalloc_params<double>, dim3(grid), dim3(256), 0, c->s, q, nq, p, np, np_pad,
                        (double*)nullptr, (double*)pd);
   e = hipGetLastError();
-  if (e == hipSuccess) e = launch_qside(c->K, c->prec, MODE_LL, g, f32 ? (const void*)qd : (const void*)q, pd, part, c->s);
+  if (e == hipSuccess)
+    e = f32 ? launch_ll_mm(c->K, g, (const float*)qd, (const float*)pd, nullptr, nullptr, part, c->s)
+            : launch_qside(c->K, c->prec, MODE_LL, g, (const void*)q, pd, part, c->s);
   if (e == hipSuccess) e = launch_reduce_ll(part, g.qchunks, g, rows_d, ll, c->s);
   if (e != hipSuccess) return nfail(c, HK_EHIP, "hk_loglik_noalloc: launch failed: %s", hipGetErrorString(e));
   return HK_OK;

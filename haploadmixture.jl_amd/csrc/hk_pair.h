@@ -1,0 +1,97 @@
+// hk_pair.h — per-(individual, SNP block) algebra shared by the genotype-pass kernels (hk_pass.hip, hk_ll.hip).
+#pragma once
+#include "hk_device.h"
+
+namespace hk {
+
+// -----------------------------------------------------------------------------------------------
+// Per-pair algebra (shared by every pass)
+//
+// Per-nibble compatibility coefficients, 128 entries in LDS (each workgroup fills them with threads
+// 0..127): lut[4 nib + f] = (h s1_00, h s1_01, h s1_11, e)[f] and lut[64 + 4 nib + f] =
+// (s2_00, s2_01, s2_11, 2 h s1_01)[f], with S1, S2 the compatibility matrices of SNPs 2j and 2j+1
+// (snp_s), h = 1/2 when either SNP is heterozygous (else 1), and e = 1 for the double-missing nibble,
+// whose S1 is 0.  With Y = S1' X S2 (S1' = h S1) and L' = <X, Y> + e:
+//   likelihood pass:  L' = L / 2^[a SNP is het]   (1 for a double-missing pair)
+//   gradient passes:  w_h / 2 = Y_h / L'          (0 for a double-missing pair)
+//   curvature passes: v_h = Y_h / (L' X_h) = c_h / (2 qp_h^2)
+// (the weights do not depend on the scale h), so no per-pair test or branch is needed for any class.
+// -----------------------------------------------------------------------------------------------
+
+template <typename TP>
+__device__ __forceinline__ void fill_pair_lut(TP* lut) {
+  const int t = threadIdx.x;
+  if (t < 128) {
+    const int tab = t >> 6, nib = (t >> 2) & 15, f = t & 3;
+    TP s00, s01, s11, h = ((nib & 3) == 2 || (nib >> 2) == 2) ? (TP)0.5 : (TP)1;
+    snp_s<TP>(tab ? (unsigned)nib >> 2 : (unsigned)nib & 3u, s00, s01, s11);
+    TP v;
+    if (tab == 0) {
+      v = f == 0 ? h * s00 : (f == 1 ? h * s01 : (f == 2 ? h * s11 : (TP)0));
+      if (nib == (int)HK_NIB_DM) v = f == 3 ? (TP)1 : (TP)0;
+    } else {
+      if (f < 3) {
+        v = f == 0 ? s00 : (f == 1 ? s01 : s11);
+      } else {
+        TP a00, a01, a11;  // 2 h s1_01 of SNP 2j (0 for the double-missing nibble)
+        snp_s<TP>((unsigned)nib & 3u, a00, a01, a11);
+        v = nib == (int)HK_NIB_DM ? (TP)0 : (TP)2 * h * a01;
+      }
+    }
+    lut[t] = v;
+  }
+}
+
+// Y = S1' X S2 and L' = <X, Y> + e from the two table entries of a nibble.
+template <typename TP>
+struct PairForm {
+  typedef TP T4 __attribute__((ext_vector_type(4)));
+  TP Y[4], L;
+  __device__ __forceinline__ PairForm(const TP (&X)[4], const TP* __restrict__ lut, unsigned nib) {
+    const T4 a = *(const T4*)(lut + 4 * nib);
+    const T4 b = *(const T4*)(lut + 64 + 4 * nib);
+    const TP z00 = hk_fma(X[1], b[1], X[0] * b[0]), z01 = hk_fma(X[1], b[2], X[0] * b[1]);
+    const TP z10 = hk_fma(X[3], b[1], X[2] * b[0]), z11 = hk_fma(X[3], b[2], X[2] * b[1]);
+    Y[0] = hk_fma(a[1], z10, a[0] * z00);
+    Y[1] = hk_fma(a[1], z11, a[0] * z01);
+    Y[2] = hk_fma(a[2], z10, a[1] * z00);
+    Y[3] = hk_fma(a[2], z11, a[1] * z01);
+    L = hk_fma(X[0], Y[0], a[3]);
+    L = hk_fma(X[1], Y[1], L);
+    L = hk_fma(X[2], Y[2], L);
+    L = hk_fma(X[3], Y[3], L);
+  }
+};
+
+// Pair likelihood L' without Y: with Z = X S2 and G_aa' = sum_b X_ab Z_a'b (G_01 = G_10, S2 symmetric),
+// L' = h s1_00 G_00 + 2 h s1_01 G_01 + h s1_11 G_11 + e.
+template <typename TP>
+__device__ __forceinline__ TP pair_lik(const TP (&X)[4], const TP* __restrict__ lut, unsigned nib) {
+  typedef TP T4 __attribute__((ext_vector_type(4)));
+  const T4 a = *(const T4*)(lut + 4 * nib);
+  const T4 b = *(const T4*)(lut + 64 + 4 * nib);
+  const TP z00 = hk_fma(X[1], b[1], X[0] * b[0]), z01 = hk_fma(X[1], b[2], X[0] * b[1]);
+  const TP z10 = hk_fma(X[3], b[1], X[2] * b[0]), z11 = hk_fma(X[3], b[2], X[2] * b[1]);
+  const TP g00 = hk_fma(X[1], z01, X[0] * z00);
+  const TP g01 = hk_fma(X[1], z11, X[0] * z10);
+  const TP g11 = hk_fma(X[3], z11, X[2] * z10);
+  return hk_fma(b[3], g01, hk_fma(a[2], g11, hk_fma(a[0], g00, a[3])));
+}
+
+// Curvature weights v_h = Y_h / (L X_h) (the factor 2 of c_h / qp_h^2 is in the outer-product tables).
+// Two reciprocals per pair: v_0, v_1 share 1 / (L' X_0 X_1) and v_2, v_3 share 1 / (L' X_2 X_3) (in FP32
+// range for every X_h >= ~3e-10, far below the 1e-5 floor the projections keep).
+template <typename TP>
+__device__ __forceinline__ void pair_v(const TP (&X)[4], const TP* __restrict__ lut, unsigned nib, TP (&v)[4]) {
+  const PairForm<TP> f(X, lut, nib);
+  const TP r01 = hk_rcp(f.L * X[0] * X[1]), r23 = hk_rcp(f.L * X[2] * X[3]);
+  v[0] = f.Y[0] * X[1] * r01;
+  v[1] = f.Y[1] * X[0] * r01;
+  v[2] = f.Y[2] * X[3] * r23;
+  v[3] = f.Y[3] * X[2] * r23;
+}
+
+// Genotype nibble at bit position sh of a word (one bit-field extract).
+__device__ __forceinline__ unsigned nib_at(uint32_t word, unsigned sh) { return __builtin_amdgcn_ubfe(word, sh, 4); }
+
+}  // namespace hk
