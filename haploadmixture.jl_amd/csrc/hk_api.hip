@@ -1462,7 +1462,30 @@ extern "C" int hk_skfr(hk_ctx* c, int k, int sparsity, int mode, int max_iter, i
                        double* prof) {
   if (!c || !clusters || !aims || !n_aims) return fail(c, HK_EINVAL, "null pointer");
   HIPC(c, hipSetDevice(c->dev));
-  const int rc = skfr_run(c->g, k, sparsity, mode, max_iter, tries, seed, init, clusters, aims, n_aims, wss, iters,
-                          prof, c->s, c->err, sizeof c->err);
+  const int rc = skfr_run(c->g, k, sparsity, mode, max_iter, tries, seed, init, nullptr, clusters, aims, n_aims, wss,
+                          iters, prof, c->s, c->err, sizeof c->err);
   return rc;
+}
+
+extern "C" int hk_skfr_path(hk_ctx* c, int k, int n_s, const int32_t* sparsities, int mode, int max_iter, int tries,
+                            uint64_t seed, int32_t* clusters, int32_t* aims, int* n_aims, double* wss, int* iters) {
+  if (!c || !sparsities || !clusters || !aims || !n_aims || n_s < 1) return fail(c, HK_EINVAL, "invalid arguments");
+  for (int t = 1; t < n_s; ++t)
+    if (sparsities[t] > sparsities[t - 1]) return fail(c, HK_EINVAL, "sparsities should be decreasing");
+  HIPC(c, hipSetDevice(c->dev));
+  int64_t off = 0;
+  for (int t = 0; t < n_s; ++t) {
+    const int s = sparsities[t];
+    const int64_t room = 2 * (c->J < (int64_t)k * s ? c->J : (int64_t)k * s);
+    int na = 0;
+    // the first sparsity: the best of `tries` seeded starts; each next one from the previous labels
+    const int rc = skfr_run(c->g, k, s, mode, max_iter, t == 0 ? tries : 1, seed, nullptr,
+                            t == 0 ? nullptr : clusters + (int64_t)(t - 1) * c->I, clusters + (int64_t)t * c->I,
+                            aims + off, &na, wss ? wss + t : nullptr, iters ? iters + t : nullptr, nullptr, c->s,
+                            c->err, sizeof c->err);
+    if (rc != 0) return rc;
+    n_aims[t] = na;
+    off += room;
+  }
+  return HK_OK;
 }

@@ -130,9 +130,10 @@ int mlp_predict_from(hk_mlp* m, const PassGeom& g, int n_blocks, uint64_t seed, 
 
 // SKFR marker pre-selection (hk_skfr.hip) on the genotype streams of g; returns 0, -1 (bad argument), -2 (HIP
 // error) or -3 (out of device memory) with err filled.
+// init_labels (I labels, or NULL): the first try starts from these cluster labels (a path step).
 int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, int tries, uint64_t seed,
-             const int32_t* init, int32_t* clusters, int32_t* aims, int* n_aims, double* wss, int* iters,
-             double* prof, hipStream_t s, char* err, size_t errlen);
+             const int32_t* init, const int32_t* init_labels, int32_t* clusters, int32_t* aims, int* n_aims,
+             double* wss, int* iters, double* prof, hipStream_t s, char* err, size_t errlen);
 
 }  // namespace hk
 

@@ -642,7 +642,8 @@ struct LaunchTimer {
   } while (0)
 
 int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, int tries, uint64_t seed,
-             const int32_t* init, int32_t* clusters_out, int32_t* aims_out, int* n_aims, double* wss_out,
+             const int32_t* init, const int32_t* init_labels, int32_t* clusters_out, int32_t* aims_out, int* n_aims,
+             double* wss_out,
              int* iters_out, double* prof, hipStream_t st, char* err, size_t errlen) {
   const int64_t I = g.I, J = g.J;
   if (k < 1 || k > SK_MAXC || k > I || sparsity < 1 || sparsity > J || max_iter < 1 || tries < 1 ||
@@ -652,6 +653,12 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
              SK_MAXC, sparsity, max_iter, tries, local);
     return -1;
   }
+  if (init_labels)  // a label out of 0..k-1 would index past the per-cluster tables on the device
+    for (int64_t i = 0; i < I; ++i)
+      if (init_labels[i] < 0 || init_labels[i] >= k) {
+        snprintf(err, errlen, "init_labels[%lld] = %d is not a cluster of 0..%d", (long long)i, init_labels[i], k - 1);
+        return -1;
+      }
   if (init)
     for (int c = 0; c < k; ++c)
       if (init[c] < 0 || init[c] >= I) {
@@ -737,9 +744,13 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
   const dim3 gdist(nblk(I), (unsigned)nch);
   for (int tr = 0; tr < tries; ++tr) {
     const bool given = tr == 0 && init != nullptr;
+    // a warm start (sparsekmeans_path): the first try begins from given labels -- the first center update
+    // reads them -- instead of starting centers and a nearest-center assignment
+    const bool from_labels = tr == 0 && init_labels != nullptr;
+    if (from_labels) SKC(hipMemcpyAsync(labels, init_labels, sizeof(int) * I, hipMemcpyHostToDevice, st));
     // starting centers: the given individuals, or k-means++ (first uniformly, then proportional to the
     // squared distance to the nearest center chosen so far)
-    for (int c = 0; c < k; ++c) {
+    for (int c = 0; c < k && !from_labels; ++c) {
       int64_t r;
       if (given) {
         r = init[c];
@@ -774,12 +785,14 @@ int skfr_run(const PassGeom& g, int k, int sparsity, int local, int max_iter, in
       }
       SKC(hipGetLastError());
     }
-    t_dist.start(st);
-    SK_BY_K(k, hipLaunchKernelGGL((k_sk_dist<KC_>), gdist, dim3(256), SK_DIST_LDS, st, g.gq, g.Ip, I, J,
-                                  (const double*)ztab, (const double*)rows, part));
-    t_dist.stop(st);
-    hipLaunchKernelGGL(k_sk_dist_finish, dim3(nblk(I)), dim3(256), 0, st, (const double*)part, nch, k, I, 1, 0, d2,
-                       labels);
+    if (!from_labels) {
+      t_dist.start(st);
+      SK_BY_K(k, hipLaunchKernelGGL((k_sk_dist<KC_>), gdist, dim3(256), SK_DIST_LDS, st, g.gq, g.Ip, I, J,
+                                    (const double*)ztab, (const double*)rows, part));
+      t_dist.stop(st);
+      hipLaunchKernelGGL(k_sk_dist_finish, dim3(nblk(I)), dim3(256), 0, st, (const double*)part, nch, k, I, 1, 0, d2,
+                         labels);
+    }
     SKC(hipGetLastError());
     int it = 0, moved = 1;
     double wsum = 0.0;

@@ -122,6 +122,7 @@ def hip():
     L.hk_features.argtypes = [_vp, _int, ctypes.c_uint64, _vp]
     L.hk_skfr.argtypes = [_vp, _int, _int, _int, _int, _int, ctypes.c_uint64, _vp, _vp, _vp, ctypes.POINTER(_int),
                           ctypes.POINTER(_dbl), ctypes.POINTER(_int), _vp]
+    L.hk_skfr_path.argtypes = [_vp, _int, _int, _vp, _int, _int, _int, ctypes.c_uint64, _vp, _vp, _vp, _vp, _vp]
     L.hk_mlp_create.argtypes = [ctypes.POINTER(_vp), _int, _int, _int, ctypes.c_uint64]
     L.hk_mlp_destroy.argtypes = [_vp]
     L.hk_mlp_last_error.argtypes = [_vp]

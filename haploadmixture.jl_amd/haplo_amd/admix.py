@@ -4,8 +4,7 @@ Mirrors src/driver.jl:38-201 (keyword names, defaults and control flow) with the
 the MI355X.  Differences, each forced by the build:
   * ``rng`` is an integer seed of the build's counter RNG (the reference takes a Julia AbstractRNG);
   * ``use_gpu=False`` is rejected: the CPU path of the reference is not part of this library;
-  * SKFR labels and markers are 0-based (``skfr`` module), and a vector of sparsities runs each
-    sparsity as its own seeded search (the reference's sparsekmeans_path warm-starts them);
+  * SKFR labels and markers are 0-based (``skfr`` module);
   * ``T2=np.float32`` selects the MIXED engine (FP32 per-pair products, FP64 accumulation), the
     analogue of the reference's ``T2=Float32`` GPU arrays (driver.jl:149, cuda/structs.jl:19-30).
 Addition: ``warm_start=True`` fills the ``init_q`` hook (driver.jl:56,184-185) with the learned
@@ -129,12 +128,17 @@ def _filter_SKFR(filename, K, sparsity, *, rng=1, prefix=None, tries=10, max_inn
     fs = plink.PlinkFileset(str(filename))
     outs, clusters, aims = [], [], []
     with skfr.ImputedSnpMatrix(fs, K, rng=rng, blocksize=2) as ism:
-        for s in sparsities:
-            if tries == 1:
-                cl, _, am, _, _ = ftn(ism, s, max_iter=max_inner_iter, squares=False)
-            else:
-                cl, _, am, _, _, _ = skfr.sparsekmeans_repeat(ism, s, iter=tries, max_inner_iter=max_inner_iter,
-                                                              ftn=ftn)
+        if many:  # driver.jl:108-131: sparsekmeans_path, each sparsity started from the previous labels
+            cls, ams = skfr.sparsekmeans_path(ism, sparsities, iter=tries, max_inner_iter=max_inner_iter)
+            runs = list(zip(sparsities, cls, ams))
+        elif tries == 1:
+            cl, _, am, _, _ = ftn(ism, sparsities[0], max_iter=max_inner_iter, squares=False)
+            runs = [(sparsities[0], cl, am)]
+        else:
+            cl, _, am, _, _, _ = skfr.sparsekmeans_repeat(ism, sparsities[0], iter=tries,
+                                                          max_inner_iter=max_inner_iter, ftn=ftn)
+            runs = [(sparsities[0], cl, am)]
+        for s, cl, am in runs:
             des = f"{prefix}_{K}_{s if many else 2 * s}aims"
             print(des)
             outs.append(plink.filter(str(filename), np.ones(fs.n_individuals, dtype=bool), np.sort(am), des))

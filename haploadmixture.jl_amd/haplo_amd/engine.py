@@ -148,6 +148,30 @@ class HipAdmixData:
         self._check(self._L.hk_count_double_missing_file(self._h, ptr(bed), bps, int(n_snp)))
         return self.double_missing()
 
+    def skfr_path(self, k: int, sparsities, mode: str = "global", max_iter: int = 50, tries: int = 1,
+                  seed: int = 1) -> list:
+        """hk_skfr_path (sparsekmeans_path): one result dict per sparsity (decreasing), each run after the
+        first started from the previous one's labels."""
+        if mode not in ("global", "local"):
+            raise ValueError("mode must be 'global' or 'local'")
+        sp = np.ascontiguousarray(sparsities, dtype=np.int32)
+        n = sp.size
+        clusters = np.zeros((n, self.I), dtype=np.int32)
+        rooms = [2 * min(self.J, int(k) * int(s)) for s in sp]
+        aims = np.zeros(max(1, sum(rooms)), dtype=np.int32)
+        n_aims = np.zeros(n, dtype=np.int32)
+        wss = np.zeros(n)
+        iters = np.zeros(n, dtype=np.int32)
+        self._check(self._L.hk_skfr_path(self._h, int(k), n, ptr(sp), int(mode == "local"), int(max_iter),
+                                         int(tries), int(seed) & ((1 << 64) - 1), ptr(clusters), ptr(aims),
+                                         ptr(n_aims), ptr(wss), ptr(iters)))
+        out, off = [], 0
+        for t in range(n):
+            out.append(dict(clusters=clusters[t].copy(), aims=aims[off: off + n_aims[t]].copy(), wss=float(wss[t]),
+                            iters=int(iters[t])))
+            off += rooms[t]
+        return out
+
     # -- runner level: the device arrays of CuAdmixData by name (cuda/structs.jl:1-15) --------------
     BUFFERS = {"q": 0, "p": 1, "q_next": 2, "p_next": 3, "XtX_q": 4, "Xtz_q": 5, "XtX_p": 6, "Xtz_p": 7}
 

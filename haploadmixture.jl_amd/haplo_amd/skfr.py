@@ -74,3 +74,12 @@ def sparsekmeans_repeat(X: ImputedSnpMatrix, sparsity: int, iter: int = 20, max_
     mode = "local" if ftn is sparsekmeans2 else "global"
     r = X.run(sparsity, mode, max_inner_iter, iter)
     return r["clusters"], None, r["aims"], r["wss"], r["iters"], iter
+
+
+def sparsekmeans_path(X: ImputedSnpMatrix, sparsities, iter: int = 20, max_inner_iter: int = 50, ftn=sparsekmeans1):
+    """SKFR along a decreasing list of sparsities (the call at src/driver.jl:122): the first sparsity is the
+    best of `iter` seeded runs, each next one starts from the previous one's labels.  Returns (clusters,
+    aims): one label vector and one marker list per sparsity."""
+    mode = "local" if ftn is sparsekmeans2 else "global"
+    rs = X.eng.skfr_path(X.k, sparsities, mode=mode, max_iter=max_inner_iter, tries=iter, seed=X.seed)
+    return [r["clusters"] for r in rs], [r["aims"][r["aims"] < X.n_snps] for r in rs]

@@ -271,6 +271,14 @@ int hk_skfr(hk_ctx* ctx, int k, int sparsity, int mode, int max_iter, int tries,
             const int32_t* init, int32_t* clusters, int32_t* aims, int* n_aims, double* wss, int* iters,
             double* prof);
 
+/* sparsekmeans_path (SparseKmeansFeatureRanking, called by _filter_SKFR for a vector of sparsities, src/driver.jl:
+ * 108-131): SKFR for each of n_s DECREASING sparsities, the first as hk_skfr with `tries` seeded starts, each next
+ * one started from the previous one's cluster labels.  clusters: n_s x I labels (row t for sparsities[t]);
+ * aims: the markers of sparsity t at offset sum_{u<t} 2 min(J, k sparsities[u]) (room for 2 min(J, k s) each),
+ * n_aims[t] of them, most important block first; wss and iters (n_s each) may be NULL. */
+int hk_skfr_path(hk_ctx* ctx, int k, int n_s, const int32_t* sparsities, int mode, int max_iter, int tries,
+                 uint64_t seed, int32_t* clusters, int32_t* aims, int* n_aims, double* wss, int* iters);
+
 /* Calibration: copy `bytes` device-to-device `reps` times with a plain streaming kernel (k_copy16, 16 B per
  * lane) on the context stream; gbps = (read + write bytes) / time.  Used to scale HBM byte counters. */
 int hk_copy_bench(hk_ctx* ctx, int64_t bytes, int reps, double* gbps);

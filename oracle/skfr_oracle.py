@@ -197,9 +197,10 @@ def assign(Z: np.ndarray, theta: np.ndarray, labels: np.ndarray, blocks: np.ndar
 
 
 def skfr(D: np.ndarray, k: int, sparsity: int, *, mode: str = "global", max_iter: int = 50, tries: int = 1,
-         seed: int = 1, init=None, return_history: bool = False) -> dict:
+         seed: int = 1, init=None, init_labels=None, return_history: bool = False) -> dict:
     """SKFR on the dosage matrix D (I x 2J, int8, -1 missing).  Returns clusters (I,), aims (0-based SNP
-    indices, most important block first), wss, iters (and the per-try results when return_history)."""
+    indices, most important block first), wss, iters (and the per-try results when return_history).
+    init_labels: the first try starts from these cluster labels (no starting centers, no first assignment)."""
     I, S = D.shape
     if S % 2:
         raise ValueError("D must hold whole SNP blocks (an even number of columns)")
@@ -216,8 +217,12 @@ def skfr(D: np.ndarray, k: int, sparsity: int, *, mode: str = "global", max_iter
     best = None
     history = []
     for tr in range(tries):
-        starts = list(init) if (tr == 0 and init is not None) else kmeanspp(Z, k, rng)
-        labels = np.argmin(_chunked_dist(Z0, Z0[starts], J), axis=1).astype(np.int32)
+        if tr == 0 and init_labels is not None:
+            starts = None
+            labels = np.asarray(init_labels, dtype=np.int32).copy()
+        else:
+            starts = list(init) if (tr == 0 and init is not None) else kmeanspp(Z, k, rng)
+            labels = np.argmin(_chunked_dist(Z0, Z0[starts], J), axis=1).astype(np.int32)
         it = 0
         for it in range(1, max_iter + 1):
             csize = np.bincount(labels, minlength=k)
@@ -249,4 +254,17 @@ def skfr(D: np.ndarray, k: int, sparsity: int, *, mode: str = "global", max_iter
     out = dict(clusters=best["clusters"], aims=aims, wss=best["wss"], iters=best["iters"])
     if return_history:
         out["history"] = history
+    return out
+
+
+def skfr_path(D: np.ndarray, k: int, sparsities, *, mode: str = "global", max_iter: int = 50, tries: int = 1,
+              seed: int = 1) -> list:
+    """sparsekmeans_path restated: SKFR for each of a decreasing list of sparsities, the first with `tries`
+    seeded starts, each next one started from the previous result's cluster labels."""
+    out, prev = [], None
+    for t, sp in enumerate(sparsities):
+        r = skfr(D, k, int(sp), mode=mode, max_iter=max_iter, tries=tries if t == 0 else 1, seed=seed,
+                 init_labels=prev)
+        out.append(r)
+        prev = r["clusters"]
     return out

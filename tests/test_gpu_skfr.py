@@ -139,3 +139,41 @@ def test_run_admixture_with_sparsity(tmp_path, odd):
     assert out.n_snps == aims.size
     assert np.array_equal(out.codes(), codes[np.sort(aims)])
     assert np.isfinite(d.ll_new) and d.q.shape == (K, I)
+
+
+@pytest.mark.parametrize("mode,tries", [("global", 2), ("local", 1)])
+def test_skfr_path_matches_oracle(mode, tries):
+    """hk_skfr_path (sparsekmeans_path): decreasing sparsities, each started on the device from the previous
+    labels; labels, markers, iterations and WSS equal the CPU restatement's path at every sparsity."""
+    I, J, k = 400, 600, 4
+    D, _ = planted(I, J, k, informative=range(0, J, 5), miss=0.02, seed=23)
+    sp = [120, 60, 25, 8]
+    print(f"skfr path {mode} tries={tries}", flush=True)
+    ref = so.skfr_path(D, k, sp, mode=mode, tries=tries, seed=3, max_iter=50)
+    with engine_for(D, K=k) as eng:
+        got = eng.skfr_path(k, sp, mode=mode, max_iter=50, tries=tries, seed=3)
+        with pytest.raises(HipError):
+            eng.skfr_path(k, [10, 20])
+    for g, r in zip(got, ref):
+        assert np.array_equal(g["clusters"], r["clusters"])
+        assert np.array_equal(g["aims"], r["aims"])
+        assert g["iters"] == r["iters"]
+        assert g["wss"] == pytest.approx(r["wss"], rel=1e-12, abs=0.0)
+
+
+def test_filter_skfr_path_writes_one_fileset_per_sparsity(tmp_path):
+    """_filter_SKFR with a decreasing vector of sparsities (driver.jl:108-131): one filtered fileset per
+    sparsity, named {prefix}_{K}_{s}aims, with the path's markers."""
+    from haplo_amd.admix import _filter_SKFR
+
+    I, J, K = 300, 200, 3
+    sim = simulate(I, J, K, fst=0.1, miss=0.01, seed=6)
+    path = str(tmp_path / "d.bed")
+    plink.write_bed(path, sim.bed, I, 2 * J)
+    outs, clusters, aims = _filter_SKFR(path, K, [40, 20], rng=9, tries=2)
+    D = so.dosages(plink.PlinkFileset(path).codes())
+    ref = so.skfr_path(D, K, [40, 20], tries=2, seed=9, max_iter=50)
+    for o, cl, am, r, s in zip(outs, clusters, aims, ref, (40, 20)):
+        assert o.endswith(f"d_{K}_{s}aims.bed") and os.path.exists(o)
+        assert np.array_equal(cl, r["clusters"]) and np.array_equal(am, r["aims"])
+        assert plink.PlinkFileset(o).n_snps == am.size

@@ -166,3 +166,19 @@ def test_invalid_arguments():
             so.skfr(D, kw["k"], kw["sparsity"])
     with pytest.raises(ValueError):
         so.skfr(D, 2, 2, mode="other")
+
+
+def test_path_warm_starts_from_previous_labels():
+    """skfr_path: the first sparsity is skfr with its tries; each next one is skfr started from the previous
+    result's labels; a start from a fixed point at the same sparsity returns it after one iteration."""
+    I, J, k = 300, 120, 3
+    D, _ = planted(I, J, k, informative=range(0, 40), miss=0.02, seed=21)
+    path = so.skfr_path(D, k, [40, 20, 10], tries=2, seed=5)
+    first = so.skfr(D, k, 40, tries=2, seed=5)
+    assert np.array_equal(path[0]["clusters"], first["clusters"]) and np.array_equal(path[0]["aims"], first["aims"])
+    for prev, cur, s in zip(path, path[1:], (20, 10)):
+        ref = so.skfr(D, k, s, init_labels=prev["clusters"])
+        assert np.array_equal(cur["clusters"], ref["clusters"]) and np.array_equal(cur["aims"], ref["aims"])
+        assert cur["aims"].size == 2 * s
+    again = so.skfr(D, k, 10, init_labels=path[2]["clusters"])
+    assert again["iters"] == 1 and np.array_equal(again["clusters"], path[2]["clusters"])
