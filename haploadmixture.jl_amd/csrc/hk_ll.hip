@@ -13,8 +13,12 @@ namespace hk {
 //
 // Per genotype word (8 blocks) a wave of 32 individuals forms X = P^T Q with v_mfma_f32_32x32x2f32:
 // A = p rows (32 = 8 blocks x 4 haplotypes, row 4b + h) x 2 populations, B = q (2 populations x 32
-// individuals), ceil(K/2) instructions (K = 6: 3, no padded half).  Operands: lane l holds A[l % 32][l / 32]
-// and B[l / 32][l % 32]; result register v of lane l holds D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32], i.e.
+// individuals), ceil(K/2) = NK2 instructions (K = 6: 3, no padded half).  Operands: lane l holds A[l % 32][l / 32]
+// and B[l / 32][l % 32]; instruction t contracts populations t (lanes 0-31) and NK2 + t (lanes 32-63, zero past
+// K), so a lane's A operands over t are NK2 consecutive floats of one p row: one 16-byte load per set at K = 8
+// and an 8-byte one at K = 4 (populations 2t and 2t + 1 per instruction put 32-byte strides between the lanes'
+// single-float loads at K = 8, which held the texture addresser busy 2.6 times as long as at K = 7).
+// Result register v of lane l holds D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32], i.e.
 // X_h (h = v % 4) of individual l % 32 at block 2 (v / 4) + l / 32 -- four pairs per lane, all four
 // haplotypes of each in the lane.  The per-pair likelihood (pair_lik) then runs on the VALU as in k_qside;
 // the FP64 running product is renormalised once per word.  NSET = 2 evaluates two parameter sets (the
@@ -46,7 +50,7 @@ __global__ __launch_bounds__(256) void k_ll_mm(const uint32_t* __restrict__ gq, 
   for (int st = 0; st < NSET; ++st)
 #pragma unroll
     for (int t = 0; t < NK2; ++t) {
-      const int k = 2 * t + half;
+      const int k = half * NK2 + t;
       bq[st][t] = k < K ? qs[st][k + (int64_t)K * ic] : 0.f;
     }
   double llm[NSET];
@@ -64,11 +68,17 @@ __global__ __launch_bounds__(256) void k_ll_mm(const uint32_t* __restrict__ gq, 
     wn = gq[(int64_t)jw * Ip + i];
 #pragma unroll
     for (int st = 0; st < NSET; ++st) {
-      const float* pb = ps[st] + (int64_t)K * (32 * (int64_t)jw + col);  // p[k + K (4 (8 jw + b) + h)], col = 4b + h
+      // p[k + K (4 (8 jw + b) + h)], col = 4b + h; this lane's populations half * NK2 + t
+      const float* pb = ps[st] + (int64_t)K * (32 * (int64_t)jw + col) + half * NK2;
+      if constexpr (K == 8) {
+        const float4 v = *reinterpret_cast<const float4*>(pb);  // 16-byte aligned: checked by the launcher
+        an[st][0] = v.x, an[st][1] = v.y, an[st][2] = v.z, an[st][3] = v.w;
+      } else if constexpr (K == 4) {
+        const float2 v = *reinterpret_cast<const float2*>(pb);
+        an[st][0] = v.x, an[st][1] = v.y;
+      } else {
 #pragma unroll
-      for (int t = 0; t < NK2; ++t) {
-        const int k = 2 * t + half;
-        an[st][t] = k < K ? pb[k] : 0.f;
+        for (int t = 0; t < NK2; ++t) an[st][t] = half * NK2 + t < K ? pb[t] : 0.f;
       }
     }
   };
@@ -118,6 +128,10 @@ static hipError_t ll_mm_k(const PassGeom& g, const float* q1, const float* p1, c
                           double* part, hipStream_t s) {
   const dim3 grid((unsigned)(g.Ip / 128), (unsigned)g.qchunks), block(256);
   const int64_t sstride = (int64_t)g.qchunks * g.Ip;
+  if (K == 8 || K == 4) {  // the kernel's vector loads of p rows
+    const uintptr_t al = K == 8 ? 16 : 8;
+    if ((uintptr_t)p1 % al || (q2 && (uintptr_t)p2 % al)) return hipErrorInvalidValue;
+  }
   if (q2)
     hipLaunchKernelGGL((k_ll_mm<K, 2>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, q1, p1, q2, p2, part,
                        sstride, g.gate);

@@ -53,7 +53,7 @@ def test_fp64_passes_match_oracle(I, J, K, oracle):
         assert qn2 is None and rel(pn2, op) <= 1e-12
 
 
-@pytest.mark.parametrize("I,J,K", [(301, 130, 5), (257, 65, 6), (77, 90, 8)])
+@pytest.mark.parametrize("I,J,K", [(301, 130, 5), (257, 65, 6), (77, 90, 8), (190, 70, 4), (130, 45, 7), (64, 33, 2)])
 def test_mixed_passes_match_oracle(I, J, K, oracle):
     print(f"mixed passes I={I} J={J} K={K}", flush=True)
     s, q, p = case(I, J, K)
