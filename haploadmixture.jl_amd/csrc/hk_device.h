@@ -12,6 +12,20 @@
 // replayed until all of them stopped.  nullptr = always run.
 __device__ __forceinline__ bool gate_off(const int* gate) { return gate != nullptr && *gate == 0; }
 
+// XCD-grouped workgroup order.  The dispatcher hands workgroup L (linear id in a 2-D grid) to XCD L % 8, and each
+// XCD has an L2 of its own.  For a pass whose grid is (row groups, chunks), with every workgroup of a chunk reading
+// the same staging-table slice, this bijection gives XCD x the contiguous range of logical ids
+// [x q + min(x, r), ...) (q = N / 8, r = N % 8): the workgroups of a chunk run back to back on one XCD, and the
+// slice is fetched into one L2 instead of eight.  Returns the logical (x, y) of this workgroup.
+#define HK_XCDS 8
+__device__ __forceinline__ void xcd_grouped_block(unsigned& bx, unsigned& by) {
+  const unsigned N = gridDim.x * gridDim.y, L = blockIdx.x + gridDim.x * blockIdx.y;
+  const unsigned q = N / HK_XCDS, r = N % HK_XCDS, x = L % HK_XCDS;
+  const unsigned lg = x * q + (x < r ? x : r) + L / HK_XCDS;
+  bx = lg % gridDim.x;
+  by = lg / gridDim.x;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Genotype nibble: bits 0-1 = BED code of SNP 2j, bits 2-3 = BED code of SNP 2j+1.
 // BED codes: 0 hom allele-1, 1 missing, 2 het, 3 hom allele-2 (src/loops.jl:63 maps them to

@@ -39,9 +39,11 @@ __global__ __launch_bounds__(256) void k_ll_mm(const uint32_t* __restrict__ gq, 
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
-  const int64_t i = (int64_t)blockIdx.x * 128 + wv * 32 + col;  // < Ip: the grid covers Ip / 128 workgroups
-  const int64_t ic = i < I ? i : I - 1;                           // padding individuals: double-missing nibbles
-  const int chunk = blockIdx.y;
+  unsigned bx, by;
+  xcd_grouped_block(bx, by);                               // a chunk's workgroups share one L2 for its p rows
+  const int64_t i = (int64_t)bx * 128 + wv * 32 + col;   // < Ip: the grid covers Ip / 128 workgroups
+  const int64_t ic = i < I ? i : I - 1;                  // padding individuals: double-missing nibbles
+  const int chunk = by;
   const int jw0 = chunk * wpc, jw1 = min(J8, jw0 + wpc);
   const float* qs[2] = {q1, q2};
   const float* ps[2] = {p1, p2};

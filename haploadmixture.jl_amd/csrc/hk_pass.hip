@@ -159,8 +159,10 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
   __shared__ __attribute__((aligned(16))) TP lut[128];
   fill_pair_lut<TP>(lut);
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int chunk = blockIdx.y;
+  unsigned bx, by;
+  xcd_grouped_block(bx, by);  // the workgroups of a chunk read its p rows through one L2
+  const int64_t i = (int64_t)bx * blockDim.x + threadIdx.x;
+  const int chunk = by;
   const int jw0 = chunk * wpc;
   const int jw1 = min(J8, jw0 + wpc);
   const int64_t ic = i < I ? i : I - 1;
@@ -648,8 +650,10 @@ __device__ __forceinline__ void qside_mm_body(const uint32_t* __restrict__ gq, i
   __shared__ __attribute__((aligned(16))) TP lut[128];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4;
-  const int64_t ibase = ((int64_t)blockIdx.x * 4 + wv) * 16 * R;
-  const int chunk = blockIdx.y;
+  unsigned bx, by;
+  xcd_grouped_block(bx, by);
+  const int64_t ibase = ((int64_t)bx * 4 + wv) * 16 * R;
+  const int chunk = by;
   const int jw0 = chunk * wpc, jw1 = min(J8, jw0 + wpc);
   fill_pair_lut<TP>(lut);
   TP qi[R][K];
