@@ -3,8 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define HK_MAXK 8          // largest K with compiled pass kernels
-#define HK_QP_NMAX 32      // 4 * HK_MAXK variables in a P-side QP
+#define HK_MAXK 16         // largest K with compiled pass and step kernels
+#define HK_MAXK_REG 8      // largest K whose kernels keep a pair's K x K products in one lane's registers
 #define HK_WAVE 64
 
 // Gate of a fit's iteration kernels: a fit whose device-side stopping rule has fired (*gate == 0) skips every

@@ -17,7 +17,8 @@ namespace hk {
 // and B[l / 32][l % 32]; instruction t contracts populations t (lanes 0-31) and NK2 + t (lanes 32-63, zero past
 // K), so a lane's A operands over t are NK2 consecutive floats of one p row: one 16-byte load per set at K = 8
 // and an 8-byte one at K = 4 (populations 2t and 2t + 1 per instruction put 32-byte strides between the lanes'
-// single-float loads at K = 8, which held the texture addresser busy 2.6 times as long as at K = 7).
+// single-float loads at K = 8, which held the texture addresser busy 2.6 times as long as at K = 7); K = 12: 8-byte
+// loads, K = 16: two 16-byte ones.
 // Result register v of lane l holds D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32], i.e.
 // X_h (h = v % 4) of individual l % 32 at block 2 (v / 4) + l / 32 -- four pairs per lane, all four
 // haplotypes of each in the lane.  The per-pair likelihood (pair_lik) then runs on the VALU as in k_qside;
@@ -72,12 +73,18 @@ __global__ __launch_bounds__(256) void k_ll_mm(const uint32_t* __restrict__ gq, 
     for (int st = 0; st < NSET; ++st) {
       // p[k + K (4 (8 jw + b) + h)], col = 4b + h; this lane's populations half * NK2 + t
       const float* pb = ps[st] + (int64_t)K * (32 * (int64_t)jw + col) + half * NK2;
-      if constexpr (K == 8) {
-        const float4 v = *reinterpret_cast<const float4*>(pb);  // 16-byte aligned: checked by the launcher
-        an[st][0] = v.x, an[st][1] = v.y, an[st][2] = v.z, an[st][3] = v.w;
-      } else if constexpr (K == 4) {
-        const float2 v = *reinterpret_cast<const float2*>(pb);
-        an[st][0] = v.x, an[st][1] = v.y;
+      if constexpr (K % 4 == 0 && NK2 % 4 == 0) {  // 16-byte aligned: checked by the launcher
+#pragma unroll
+        for (int u = 0; u < NK2 / 4; ++u) {
+          const float4 v = reinterpret_cast<const float4*>(pb)[u];
+          an[st][4 * u] = v.x, an[st][4 * u + 1] = v.y, an[st][4 * u + 2] = v.z, an[st][4 * u + 3] = v.w;
+        }
+      } else if constexpr (K % 2 == 0 && NK2 % 2 == 0) {  // 8-byte aligned
+#pragma unroll
+        for (int u = 0; u < NK2 / 2; ++u) {
+          const float2 v = reinterpret_cast<const float2*>(pb)[u];
+          an[st][2 * u] = v.x, an[st][2 * u + 1] = v.y;
+        }
       } else {
 #pragma unroll
         for (int t = 0; t < NK2; ++t) an[st][t] = half * NK2 + t < K ? pb[t] : 0.f;
@@ -130,10 +137,9 @@ static hipError_t ll_mm_k(const PassGeom& g, const float* q1, const float* p1, c
                           double* part, hipStream_t s) {
   const dim3 grid((unsigned)(g.Ip / 128), (unsigned)g.qchunks), block(256);
   const int64_t sstride = (int64_t)g.qchunks * g.Ip;
-  if (K == 8 || K == 4) {  // the kernel's vector loads of p rows
-    const uintptr_t al = K == 8 ? 16 : 8;
-    if ((uintptr_t)p1 % al || (q2 && (uintptr_t)p2 % al)) return hipErrorInvalidValue;
-  }
+  constexpr int NK2 = (K + 1) / 2;
+  constexpr uintptr_t al = (K % 4 == 0 && NK2 % 4 == 0) ? 16 : ((K % 2 == 0 && NK2 % 2 == 0) ? 8 : 4);
+  if ((uintptr_t)p1 % al || (q2 && (uintptr_t)p2 % al)) return hipErrorInvalidValue;  // the kernel's p-row loads
   if (q2)
     hipLaunchKernelGGL((k_ll_mm<K, 2>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, q1, p1, q2, p2, part,
                        sstride, g.gate);
@@ -154,6 +160,14 @@ hipError_t launch_ll_mm(int K, const PassGeom& g, const float* q1, const float* 
     case 6: return ll_mm_k<6>(g, q1, p1, q2, p2, part, s);
     case 7: return ll_mm_k<7>(g, q1, p1, q2, p2, part, s);
     case 8: return ll_mm_k<8>(g, q1, p1, q2, p2, part, s);
+    case 9: return ll_mm_k<9>(g, q1, p1, q2, p2, part, s);
+    case 10: return ll_mm_k<10>(g, q1, p1, q2, p2, part, s);
+    case 11: return ll_mm_k<11>(g, q1, p1, q2, p2, part, s);
+    case 12: return ll_mm_k<12>(g, q1, p1, q2, p2, part, s);
+    case 13: return ll_mm_k<13>(g, q1, p1, q2, p2, part, s);
+    case 14: return ll_mm_k<14>(g, q1, p1, q2, p2, part, s);
+    case 15: return ll_mm_k<15>(g, q1, p1, q2, p2, part, s);
+    case 16: return ll_mm_k<16>(g, q1, p1, q2, p2, part, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -94,4 +94,57 @@ __device__ __forceinline__ void pair_v(const TP (&X)[4], const TP* __restrict__ 
 // Genotype nibble at bit position sh of a word (one bit-field extract).
 __device__ __forceinline__ unsigned nib_at(uint32_t word, unsigned sh) { return __builtin_amdgcn_ubfe(word, sh, 4); }
 
+
+// Matrix-core step of the curvature passes for the parameter type (16 x 16 x 4, FP32 or FP64).
+template <typename TP>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int v) { return 4 * g + v; }
+};
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int v) { return g + 4 * v; }
+};
+
+__host__ __device__ constexpr int pow2_at_least(int v) { return v <= 1 ? 1 : 2 * pow2_at_least((v + 1) / 2); }
+
+// Staging table rows of the curvature passes: row c < nrow of out (RG entries) holds, for s < sub, the QS packed
+// products 2 x[k,m] x[k',m] of column m = sub c + s, then the sub K values x[:, sub c + s], then zeros.
+// Rows c >= ncol / sub repeat the last column group.
+template <int K, typename TP>
+__global__ void k_stage_rows(const TP* __restrict__ x, int64_t ncol, int64_t nrow, int sub, int QS, int RG,
+                             TP* __restrict__ out, const int* __restrict__ gate) {
+  if (gate_off(gate)) return;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nrow * RG) return;
+  const int64_t c = idx / RG;
+  const int e = (int)(idx - c * RG);
+  const int64_t last = ncol / sub - 1;
+  const int64_t cc = c < last ? c : last;
+  TP v = (TP)0;
+  if (e < sub * QS) {
+    const int sgrp = e / QS, n = e - sgrp * QS;
+    if (n < K * (K + 1) / 2) {
+      int k = 0;
+      while ((k + 1) * (k + 2) / 2 <= n) ++k;
+      const int k2 = n - k * (k + 1) / 2;
+      const TP* xc = x + (int64_t)K * (sub * cc + sgrp);
+      v = (TP)2 * (xc[k] * xc[k2]);
+    }
+  } else if (e < sub * (QS + K)) {
+    const int t = e - sub * QS, sgrp = t / K, k = t - sgrp * K;
+    v = x[k + (int64_t)K * (sub * cc + sgrp)];
+  }
+  out[idx] = v;
+}
+
 }  // namespace hk

@@ -341,28 +341,6 @@ __global__ __launch_bounds__(256) void k_pside_grad(const uint32_t* __restrict__
 // added to FP64 accumulators.  FP64: FP64 throughout.
 // -----------------------------------------------------------------------------------------------
 
-template <typename TP>
-struct Mfma;
-template <>
-struct Mfma<float> {
-  typedef float v4 __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int g, int v) { return 4 * g + v; }
-};
-template <>
-struct Mfma<double> {
-  typedef double v4 __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int g, int v) { return g + 4 * v; }
-};
-
-// Column split of the packed curvature (NKK = K(K+1)/2 entries): NTILE MFMA tiles of 16, then NR
-// VALU tail columns starting at TAIL0; the outer-product tables have rows of QS entries.
-__host__ __device__ constexpr int pow2_at_least(int v) { return v <= 1 ? 1 : 2 * pow2_at_least((v + 1) / 2); }
 
 // Column split of the packed curvature (NKK = K(K+1)/2 entries): NTILE MFMA tiles of 16, then NR VALU
 // tail columns starting at TAIL0; QS = padded length of one packed row.  Staging tables (k_stage_rows):
@@ -383,36 +361,6 @@ struct CurvShape {
   static constexpr int RLP = NTILE > 0 ? RGP + 16 : RGP, RLQ = NTILE > 0 ? RGQ + 16 : RGQ;
   static constexpr int PAD_ROWS = 32;  // rows past the last column (the last staging step reads up to 32)
 };
-
-// Staging table rows (see CurvShape): row c < nrow of out (RG entries) holds, for s < sub, the QS packed
-// products 2 x[k,m] x[k',m] of column m = sub c + s, then the sub K values x[:, sub c + s], then zeros.
-// Rows c >= ncol / sub repeat the last column group.
-template <int K, typename TP>
-__global__ void k_stage_rows(const TP* __restrict__ x, int64_t ncol, int64_t nrow, int sub, int QS, int RG,
-                             TP* __restrict__ out, const int* __restrict__ gate) {
-  if (gate_off(gate)) return;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nrow * RG) return;
-  const int64_t c = idx / RG;
-  const int e = (int)(idx - c * RG);
-  const int64_t last = ncol / sub - 1;
-  const int64_t cc = c < last ? c : last;
-  TP v = (TP)0;
-  if (e < sub * QS) {
-    const int sgrp = e / QS, n = e - sgrp * QS;
-    if (n < K * (K + 1) / 2) {
-      int k = 0;
-      while ((k + 1) * (k + 2) / 2 <= n) ++k;
-      const int k2 = n - k * (k + 1) / 2;
-      const TP* xc = x + (int64_t)K * (sub * cc + sgrp);
-      v = (TP)2 * (xc[k] * xc[k2]);
-    }
-  } else if (e < sub * (QS + K)) {
-    const int t = e - sub * QS, sgrp = t / K, k = t - sgrp * K;
-    v = x[k + (int64_t)K * (sub * cc + sgrp)];
-  }
-  out[idx] = v;
-}
 
 // Sum of a value over the four lanes l, l ^ 16, l ^ 32, l ^ 48 (same l % 16); bitwise equal in all four.
 __device__ __forceinline__ double sum_groups(double x) {
@@ -1038,7 +986,9 @@ static hipError_t qside_k(int mode, const PassGeom& g, const void* q, const void
   else if (mode == MODE_GRAD)
     hipLaunchKernelGGL((k_qside<K, TP, MODE_GRAD>), grid, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp, part,
                        g.flush_grad, g.gate);
-  else {
+  else if constexpr (K > HK_MAXK_REG) {
+    return launch_qside_curv_wide(K, sizeof(TP) == 8 ? PREC_FP64 : PREC_MIXED, g, q, part, s);
+  } else {
     const dim3 gmm((unsigned)(g.Ip / 128), (unsigned)g.qchunks);
     if constexpr (K <= 6 && sizeof(TP) == 4)
       hipLaunchKernelGGL((k_qside_mm_pf<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
@@ -1053,39 +1003,47 @@ static hipError_t qside_k(int mode, const PassGeom& g, const void* q, const void
 template <int K, typename TP>
 static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void* p, double* out,
                           hipStream_t s) {
-  dim3 grid((unsigned)(4 * g.Jp / 256), (unsigned)g.pchunks), block(256);
-  const TP* qq = (const TP*)q;
-  const TP* pp = (const TP*)p;
-  if (mode == MODE_GRAD)
-    hipLaunchKernelGGL((k_pside_grad<K, TP>), dim3((unsigned)(g.Jp / 256 + (g.Jp % 256 ? 1 : 0)), (unsigned)g.pchunks),
-                       block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out, g.flush_grad, g.gate);
-  else {
-    const dim3 gmm((unsigned)(g.Jp / 64), (unsigned)g.pchunks);
-    if constexpr (K <= 6 && sizeof(TP) == 4)
-      hipLaunchKernelGGL((k_pside_mm_pf<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
-                         (const TP*)g.qq, out, g.flush_curv, g.gate);
-    else
-      hipLaunchKernelGGL((k_pside_mm<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
-                         (const TP*)g.qq, out, g.flush_curv, g.gate);
+  if constexpr (K > HK_MAXK_REG) {
+    return launch_pside_wide(K, sizeof(TP) == 8 ? PREC_FP64 : PREC_MIXED, mode, g, q, p, out, s);
+  } else {
+    const dim3 block(256);
+    const TP* qq = (const TP*)q;
+    const TP* pp = (const TP*)p;
+    if (mode == MODE_GRAD) {
+      hipLaunchKernelGGL((k_pside_grad<K, TP>), dim3((unsigned)(g.Jp / 256 + (g.Jp % 256 ? 1 : 0)), (unsigned)g.pchunks),
+                         block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out, g.flush_grad, g.gate);
+    } else {
+      const dim3 gmm((unsigned)(g.Jp / 64), (unsigned)g.pchunks);
+      if constexpr (K <= 6 && sizeof(TP) == 4)
+        hipLaunchKernelGGL((k_pside_mm_pf<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
+                           (const TP*)g.qq, out, g.flush_curv, g.gate);
+      else
+        hipLaunchKernelGGL((k_pside_mm<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
+                           (const TP*)g.qq, out, g.flush_curv, g.gate);
+    }
+    return hipGetLastError();
   }
-  return hipGetLastError();
 }
 
 // Staging tables of the curvature passes from the pass parameters: the Q-side pass reads the p table
 // (rows per block), the P-side pass the q table (rows per individual).
 template <int K, typename TP>
 static hipError_t tables_k(int side, const PassGeom& g, const void* x, hipStream_t s) {
-  using S = CurvShape<K>;
-  if (side == 0) {
-    const int64_t nrow = g.Jp + S::PAD_ROWS;
-    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGQ + 255) / 256)), dim3(256), 0, s,
-                       (const TP*)x, 4 * g.Jp, nrow, 4, S::QS, S::RGQ, (TP*)g.pp, g.gate);
+  if constexpr (K > HK_MAXK_REG) {
+    return launch_curv_tables_wide(K, sizeof(TP) == 8 ? PREC_FP64 : PREC_MIXED, side, g, x, s);
   } else {
-    const int64_t nrow = g.Ip + S::PAD_ROWS;
-    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGP + 255) / 256)), dim3(256), 0, s,
-                       (const TP*)x, g.I, nrow, 1, S::QS, S::RGP, (TP*)g.qq, g.gate);
+    using S = CurvShape<K>;
+    if (side == 0) {
+      const int64_t nrow = g.Jp + S::PAD_ROWS;
+      hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGQ + 255) / 256)), dim3(256), 0, s,
+                         (const TP*)x, 4 * g.Jp, nrow, 4, S::QS, S::RGQ, (TP*)g.pp, g.gate);
+    } else {
+      const int64_t nrow = g.Ip + S::PAD_ROWS;
+      hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGP + 255) / 256)), dim3(256), 0, s,
+                         (const TP*)x, g.I, nrow, 1, S::QS, S::RGP, (TP*)g.qq, g.gate);
+    }
+    return hipGetLastError();
   }
-  return hipGetLastError();
 }
 
 #define HK_K_SWITCH(FN, TP, ...)                 \
@@ -1098,6 +1056,14 @@ static hipError_t tables_k(int side, const PassGeom& g, const void* x, hipStream
     case 6: return FN<6, TP>(__VA_ARGS__);       \
     case 7: return FN<7, TP>(__VA_ARGS__);       \
     case 8: return FN<8, TP>(__VA_ARGS__);       \
+    case 9: return FN<9, TP>(__VA_ARGS__);       \
+    case 10: return FN<10, TP>(__VA_ARGS__);     \
+    case 11: return FN<11, TP>(__VA_ARGS__);     \
+    case 12: return FN<12, TP>(__VA_ARGS__);     \
+    case 13: return FN<13, TP>(__VA_ARGS__);     \
+    case 14: return FN<14, TP>(__VA_ARGS__);     \
+    case 15: return FN<15, TP>(__VA_ARGS__);     \
+    case 16: return FN<16, TP>(__VA_ARGS__);     \
     default: return hipErrorInvalidValue;        \
   }
 
@@ -1137,7 +1103,7 @@ void stage_sizes(int K, int64_t Ip, int64_t Jp, int64_t* nq, int64_t* np) {
     case 6: return stage_sizes_k<6>(Ip, Jp, nq, np);
     case 7: return stage_sizes_k<7>(Ip, Jp, nq, np);
     case 8: return stage_sizes_k<8>(Ip, Jp, nq, np);
-    default: *nq = *np = 0;
+    default: return stage_sizes_wide(K, Ip, Jp, nq, np);
   }
 }
 
@@ -1200,6 +1166,7 @@ hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* re
     const hipError_t e = launch_set_int(dense, 0, s, g.gate);
     if (e != hipSuccess) return e;
   }
+  if (K > HK_MAXK_REG) return launch_finish_p_wide(K, g, p, res, s);  // no early dense list: every P QP is dense
   const dim3 grid((unsigned)((4 * g.J + 255) / 256));
   switch (K) {
     case 1: hipLaunchKernelGGL(k_finish_p<1>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;

@@ -46,7 +46,7 @@ namespace hk {
 
 namespace {
 
-constexpr int SK_MAXC = HK_MAXK;  // largest number of clusters
+constexpr int SK_MAXC = HK_MAXK_REG;  // largest number of clusters (per-lane cluster arrays)
 constexpr int SK_CHUNK = 1024;    // blocks per partial sum of the all-SNP distance kernel
 constexpr int SK_SUB = 256;       // blocks (or kept-list entries) per LDS stage
 constexpr int SK_CW = 128;        // genotype words (1024 individuals) per partial of the center counts

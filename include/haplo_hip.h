@@ -45,7 +45,10 @@ enum hk_precision {
                         sum, QP and QN in FP64.  A comparison point for HK_PREC_MIXED. */
 };
 
-#define HK_MAX_K 8
+/* Largest K.  K <= 8 runs the register-resident pass and four-lane P-step kernels; 9 <= K <= 16 runs passes over
+ * windows of the packed K x K columns and the dense one-wave solver for every P row QP (slower per pair). */
+#define HK_MAX_K 16
+#define HK_SKFR_MAX_K 8 /* largest SKFR cluster count */
 
 /* _cu_admixture_base (src/cuda/structs.jl:34-44, CuAdmixData :17-32): upload the genotypes of the first
  * I individuals and first 2J SNPs of a BED payload with bytes_per_snp = ceil(I_file/4) bytes per SNP and
@@ -257,7 +260,7 @@ int hk_mlp_adam(hk_mlp* m, float lr, float beta1, float beta2, float eps, int st
 /* ---- SKFR marker pre-selection: _filter_SKFR (src/driver.jl:84-131) over SparseKmeansFeatureRanking's
  * sparsekmeans1 (mode 0, skfr_mode = :global) / sparsekmeans2 (mode 1, :local) and sparsekmeans_repeat
  * (tries > 1) on the context's J SNP blocks (blocksize = 2: SNPs 2b and 2b+1 form block b).
- * Sparse k-means with feature ranking on standardised allele-2 dosages: k clusters (1..HK_MAX_K, <= I),
+ * Sparse k-means with feature ranking on standardised allele-2 dosages: k clusters (1..HK_SKFR_MAX_K, <= I),
  * `sparsity` blocks kept (1..J; :local keeps that many per cluster, markers = their union), at most
  * max_iter (skfr_max_inner_iter) reassignments per try; the try with the smallest within-cluster sum of
  * squares over the kept SNPs wins.  seed drives the k-means++ starts; init (k individual indices, or NULL)

@@ -34,8 +34,8 @@ def test_create_rejects_bad_arguments_without_gpu_work():
     L = _native.hip()
     h = ctypes.c_void_p()
     bed = np.zeros(16, dtype=np.uint8)
-    assert L.hk_ctx_create(ctypes.byref(h), 4, 2, 9, _native.ptr(bed), 1, 4, 0) == -1  # K > 8
-    assert b"K in 1..8" in L.hk_create_error()
+    assert L.hk_ctx_create(ctypes.byref(h), 4, 2, 17, _native.ptr(bed), 1, 4, 0) == -1  # K > 16
+    assert b"K in 1..16" in L.hk_create_error()
     assert L.hk_ctx_create(ctypes.byref(h), 4, 3, 2, _native.ptr(bed), 1, 4, 0) == -1  # n_snp < 2J
     assert L.hk_ctx_create(ctypes.byref(h), 4, 2, 2, _native.ptr(bed), 1, 4, 7) == -1  # precision
 
