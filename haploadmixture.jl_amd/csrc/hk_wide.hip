@@ -31,7 +31,14 @@ struct WideShape {
 // of four different rows, which then fall on distinct banks.
 __host__ __device__ constexpr int wide_pitch(int n) { return n <= 16 ? 16 : ((n - 16 + 63) / 64) * 64 + 16; }
 
-constexpr int kWideTiles = 2;  // packed-column tiles per window
+// Packed-column tiles per window.  Fewer windows mean less repeated per-pair algebra, more tiles per window
+// more accumulators: three keep two waves per SIMD on the Q side and, for FP32 up to K = 12, on the P side
+// (whose lanes also hold a block's 4K p values), two otherwise; with four tiles in all (K = 10) two windows of
+// two beat two windows of three, one of them half empty.
+template <int K, typename TP, int SIDE>
+constexpr int wide_tiles() {
+  return (SIDE == 1 && (sizeof(TP) == 8 || K > 12)) || WideShape<K>::NTILE == 4 ? 2 : 3;
+}
 
 // ---------------------------------------------------------------------------------------------------------
 // Q side: a wave owns 2 x 16 individuals; each MFMA step takes 4 blocks.  A stage holds 16 blocks (2 words):
@@ -313,9 +320,9 @@ __global__ void k_finish_pw(int K, int64_t J, const double* __restrict__ p, doub
 // ---------------------------------------------------------------------------------------------------------
 // Launchers (9 <= K <= 16)
 // ---------------------------------------------------------------------------------------------------------
-template <int K>
+template <int K, typename TP, int SIDE>
 static int wide_windows() {
-  return (WideShape<K>::NTILE + kWideTiles - 1) / kWideTiles;
+  return (WideShape<K>::NTILE + wide_tiles<K, TP, SIDE>() - 1) / wide_tiles<K, TP, SIDE>();
 }
 
 template <int K, typename TP>
@@ -335,8 +342,8 @@ static hipError_t wide_tables_k(int side, const PassGeom& g, const void* x, hipS
 
 template <int K, typename TP>
 static hipError_t wide_qcurv_k(const PassGeom& g, const void* q, double* part, hipStream_t s) {
-  const dim3 grid((unsigned)(g.Ip / 128), (unsigned)g.qchunks, (unsigned)wide_windows<K>());
-  hipLaunchKernelGGL((k_qside_mmw<K, TP, kWideTiles>), grid, dim3(256), 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc,
+  const dim3 grid((unsigned)(g.Ip / 128), (unsigned)g.qchunks, (unsigned)wide_windows<K, TP, 0>());
+  hipLaunchKernelGGL((k_qside_mmw<K, TP, wide_tiles<K, TP, 0>()>), grid, dim3(256), 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc,
                      (const TP*)q, (const TP*)g.pp, part, g.flush_curv, g.gate);
   return hipGetLastError();
 }
@@ -345,8 +352,8 @@ template <int K, typename TP>
 static hipError_t wide_pside_k(int mode, const PassGeom& g, const void* q, const void* p, double* out,
                                hipStream_t s) {
   if (mode == MODE_CURV) {
-    const dim3 grid((unsigned)(g.Jp / 64), (unsigned)g.pchunks, (unsigned)wide_windows<K>());
-    hipLaunchKernelGGL((k_pside_mmw<K, TP, kWideTiles, MODE_CURV>), grid, dim3(256), 0, s, g.gp, g.Jp, g.I,
+    const dim3 grid((unsigned)(g.Jp / 64), (unsigned)g.pchunks, (unsigned)wide_windows<K, TP, 1>());
+    hipLaunchKernelGGL((k_pside_mmw<K, TP, wide_tiles<K, TP, 1>(), MODE_CURV>), grid, dim3(256), 0, s, g.gp, g.Jp, g.I,
                        (int)g.I8, g.pwpc, (const TP*)q, (const TP*)p, (const TP*)g.qq, out, g.flush_curv, g.gate);
   } else if (mode == MODE_GRAD) {
     const dim3 grid((unsigned)(g.Jp / 64), (unsigned)g.pchunks, 1);
