@@ -167,6 +167,33 @@ def test_wide_graph_batch_and_single_fits_agree():
         assert x["iters"] == y["iters"] and x["ll"] == y["ll"]
 
 
+def test_wide_fp32_accumulation_mode(oracle):
+    """HK_PREC_FP32 (FP32 products and FP32 sums over the whole i or j range) at K = 10: within FP32-accumulation
+    error of the FP64 oracle."""
+    I, J, K = 2000, 48, 10
+    print(f"wide fp32 accumulation I={I} J={J} K={K}", flush=True)
+    s, q, p = case(I, J, K, miss=0.02)
+    with _eng(s, q, p, "fp32acc") as e:
+        ll = e.loglikelihood()
+        assert abs(ll - oracle.loglik(s.bed, I, J, q, p)) <= 2e-6 * abs(ll)
+        XtX, Xtz = e.update_p()
+        oX, oz = oracle.update_p(s.bed, I, J, q, p)
+        assert rel(XtX, oX) <= 1e-3 and rel(Xtz, oz) <= 1e-3
+        XtX, Xtz = e.update_q()
+        oX, oz = oracle.update_q(s.bed, I, J, q, p)
+        assert rel(XtX, oX) <= 1e-3 and rel(Xtz, oz) <= 1e-3
+
+
+def test_skfr_cluster_limit_is_checked():
+    """SKFR keeps per-lane cluster arrays of 8: k = 9 is rejected with a message, the context stays usable."""
+    s = simulate(64, 40, 3, seed=2)
+    with HipAdmixData(s.bed, 64, 40, 12) as e:
+        with pytest.raises(HipError):
+            e.skfr(9, 10)
+        r = e.skfr(3, 10, max_iter=5, seed=1)
+        assert len(r["clusters"]) == 64
+
+
 def test_k_above_limit_is_rejected():
     s = simulate(20, 10, 2, seed=1)
     with pytest.raises(HipError):

@@ -79,7 +79,7 @@ def _noalloc(e, q, p, rows=True):
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_noalloc_matches_loglik_two_calls_second_smaller(prec):
-    for (I, J, K, seed) in ((3000, 5000, 6, 3), (257, 130, 4, 8)):
+    for (I, J, K, seed) in ((3000, 5000, 6, 3), (257, 130, 4, 8), (200, 90, 12, 5)):  # K = 12: the wide kernels
         print(f"noalloc {prec} I={I} J={J} K={K}", flush=True)
         s, q, p = case(I, J, K, seed=seed, miss=0.05)
         with HipAdmixData(s.bed, I, J, K, precision=prec) as e:
