@@ -204,6 +204,60 @@ def test_qp_kkt_random(oracle):
         assert np.all(kk[(x >= hi - 1e-10) & ~free] <= scale)
 
 
+@pytest.mark.parametrize("K", [12, 16])
+def test_wide_k_gradient_identities_and_p_qp_kkt(oracle, K):
+    """K above 8 (the GPU's windowed passes and dense P row QPs are checked against these): the gradients are
+    finite differences of the log-likelihood, sum_k q_ki Xtz_q[k,i] = q' XtX_q q = 2 (J - dm_i), and the
+    4K-variable P row QP (penalty and K group equalities) satisfies its KKT conditions."""
+    I, J = 23, 9
+    s, q, p = case(I, J, K, miss=0.1)
+    XtXq, Xtzq = oracle.update_q(s.bed, I, J, q, p)
+    XtXp, Xtzp = oracle.update_p(s.bed, I, J, q, p)
+    h = 1e-6
+    for (k, i) in [(0, 0), (K - 1, 7), (K // 2, 22)]:
+        qa, qb = q.copy(order="F"), q.copy(order="F")
+        qa[k, i] += h
+        qb[k, i] -= h
+        fd = (oracle.loglik(s.bed, I, J, qa, p) - oracle.loglik(s.bed, I, J, qb, p)) / (2 * h)
+        assert abs(fd - Xtzq[k, i]) <= 1e-6 * max(1.0, abs(fd))
+    for (k, c) in [(0, 0), (K - 1, 13), (K // 2, 35)]:
+        pa, pb = p.copy(order="F"), p.copy(order="F")
+        pa[k, c] += h
+        pb[k, c] -= h
+        fd = (oracle.loglik(s.bed, I, J, q, pa) - oracle.loglik(s.bed, I, J, q, pb)) / (2 * h)
+        assert abs(fd - Xtzp[k, c]) <= 1e-6 * max(1.0, abs(fd))
+    dm = oracle.count_double_missing(s.bed, I, J)
+    assert np.allclose(np.einsum("ki,ki->i", q, Xtzq), 2 * (J - dm), rtol=1e-12)
+    assert np.allclose(np.einsum("ki,kli,li->i", q, XtXq, q), 2 * (J - dm), rtol=1e-12)
+    rng = np.random.default_rng(K)
+    n = 4 * K
+    for t in range(6):
+        A = rng.standard_normal((n, 2 * n))
+        pen = t % 2 == 0
+        H = A @ A.T / n + (1e3 * np.kron(np.ones((4, 4)), np.eye(K)) if pen else 0)
+        g = rng.standard_normal(n) * 5
+        x0 = rng.uniform(1e-5, 1, n)
+        lo, hi = -x0, 1 - x0
+        grp = None if pen else (np.arange(n) % K).astype(np.int32)
+        x, it = oracle.qp_solve(H, g, lo, hi, grp)
+        assert it >= 0
+        assert np.all(x >= lo - 1e-12) and np.all(x <= hi + 1e-12)
+        r = H @ x + g
+        free = (x > lo + 1e-10) & (x < hi - 1e-10)
+        lam = np.zeros(n)
+        if grp is not None:
+            for c in range(K):
+                m = grp == c
+                assert abs(x[m].sum()) <= 1e-10
+                f = m & free
+                lam[m] = -r[f].mean() if f.any() else 0.0
+        kk = r + lam
+        scale = 1e-7 * (np.abs(g).max() + np.abs(H).max())
+        assert np.all(np.abs(kk[free]) <= scale)
+        assert np.all(kk[(x <= lo + 1e-10) & ~free] >= -scale)
+        assert np.all(kk[(x >= hi - 1e-10) & ~free] <= scale)
+
+
 def test_projection_matches_bruteforce_fixture(oracle):
     z = np.load(os.path.join(HERE, "golden", "projection_bruteforce.npz"))
     for t in range(len(z["n"])):
