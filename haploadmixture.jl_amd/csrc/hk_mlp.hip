@@ -11,6 +11,7 @@
 // converged Q of the plain method, with the clusters of each batch relabelled by the permutation that
 // minimises it (cluster labels of an admixture fit are arbitrary).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <math.h>
 #include <stdarg.h>
 #include <stdint.h>
@@ -223,37 +224,101 @@ struct ProjBuffers {
 // ---------------------------------------------------------------------------------------------------
 
 // C[M x N] (row stride ldc) = A(M x Kd) B(Kd x N) (+ C if accumulate): A FP32 with element strides
-// (sam, sak), B FP32 or BF16 with strides (sbk, sbn).  One wavefront per 16 x 16 tile, one MFMA per 4 of Kd.
+// (sam, sak), B BF16 or FP32 with strides (sbk, sbn), FP32 accumulation.
+//
+// On v_mfma_f32_16x16x32_bf16.  An FP32 operand enters as three BF16 terms x = x_hi + x_mid + x_lo (each the
+// rounded remainder of the previous ones, so the three carry the 24 bits of x), and every BF16 x BF16 product
+// is exact in the FP32 accumulator: A x BF16 weights takes three matrix instructions per tile and K-slice of 32,
+// FP32 x FP32 six (the term pairs down to 2^-16 of the product; the dropped ones are below FP32's rounding).
+// Workgroup tile 64 x 64, K-slices of 32 staged in LDS (any strides: each operand is read along its unit stride);
+// wave w owns rows 16w .. 16w + 15 and the four 16-column tiles.  Operand layout of the instruction: lane l holds
+// A[l % 16][8 (l / 16) .. + 8] and B[8 (l / 16) .. + 8][l % 16]; register r of the result is D[4 (l / 16) + r][l % 16].
+constexpr int GT = 64, GK = 32, GP = GK + 8;  // tile edge, K-slice, LDS row pitch (16-byte aligned rows)
+
+__device__ __forceinline__ void split3(float x, uint16_t& h, uint16_t& m, uint16_t& l) {
+  h = f32_to_bf16(x);
+  const float r1 = x - bf16_to_f32(h);  // exact: the remainder has at most 16 significant bits
+  m = f32_to_bf16(r1);
+  l = f32_to_bf16(r1 - bf16_to_f32(m));
+}
+
 template <typename TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int Kd, const float* __restrict__ A, int64_t sam,
                                               int64_t sak, const TB* __restrict__ B, int64_t sbk, int64_t sbn,
                                               float* __restrict__ C, int64_t ldc, int accumulate) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr bool BF = sizeof(TB) == 2;
+  constexpr int NB = BF ? 1 : 3;  // BF16 terms of a B element
+  __shared__ __attribute__((aligned(16))) uint16_t As[3][GT * GP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[NB][GT * GP];  // [column][k]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m = lane & 15, g = lane >> 4;
-  const int tiles_n = (N + 15) / 16;
-  const int tile = blockIdx.x * 4 + wv;
-  const int tm = tile / tiles_n, tn = tile - tm * tiles_n;
-  if (tm * 16 >= M) return;  // whole wavefront exits together
-  const int row = tm * 16 + m, col = tn * 16 + m;
-  v4f acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < Kd; k0 += 4) {
-    const int k = k0 + g;
-    float a = 0.f, b = 0.f;
-    if (row < M && k < Kd) a = A[row * sam + k * sak];
-    if (col < N && k < Kd) {
-      if constexpr (sizeof(TB) == 2) b = bf16_to_f32(B[k * sbk + col * sbn]);
-      else b = B[k * sbk + col * sbn];
+  const int row0 = blockIdx.x * GT, col0 = blockIdx.y * GT;
+  v4f acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = (v4f){0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < Kd; k0 += GK) {
+    __syncthreads();  // the previous slice has been consumed
+#pragma unroll
+    for (int u = 0; u < GT * GK / 256; ++u) {
+      const int e = tid + 256 * u;
+      const int r = sak == 1 ? e / GK : e % GT, k = sak == 1 ? e % GK : e / GT;  // read along A's unit stride
+      const int gr = row0 + r, gk = k0 + k;
+      const float x = (gr < M && gk < Kd) ? A[(int64_t)gr * sam + (int64_t)gk * sak] : 0.f;
+      uint16_t h, md, lo;
+      split3(x, h, md, lo);
+      As[0][r * GP + k] = h;
+      As[1][r * GP + k] = md;
+      As[2][r * GP + k] = lo;
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < GT * GK / 256; ++u) {
+      const int e = tid + 256 * u;
+      const int c = sbn == 1 ? e % GT : e / GK, k = sbn == 1 ? e / GT : e % GK;  // read along B's unit stride
+      const int gc = col0 + c, gk = k0 + k;
+      const bool in = gc < N && gk < Kd;
+      if constexpr (BF) {
+        Bs[0][c * GP + k] = in ? B[(int64_t)gk * sbk + (int64_t)gc * sbn] : (uint16_t)0;
+      } else {
+        uint16_t h, md, lo;
+        split3(in ? B[(int64_t)gk * sbk + (int64_t)gc * sbn] : 0.f, h, md, lo);
+        Bs[0][c * GP + k] = h;
+        Bs[1][c * GP + k] = md;
+        Bs[2][c * GP + k] = lo;
+      }
+    }
+    __syncthreads();
+    bf16x8 a[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) a[q] = *reinterpret_cast<const bf16x8*>(&As[q][(16 * wv + m) * GP + 8 * g]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      bf16x8 b[NB];
+#pragma unroll
+      for (int q = 0; q < NB; ++q) b[q] = *reinterpret_cast<const bf16x8*>(&Bs[q][(16 * t + m) * GP + 8 * g]);
+      if constexpr (BF) {
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc[t], 0, 0, 0);  // small terms first
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc[t], 0, 0, 0);
+      } else {
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc[t], 0, 0, 0);
+      }
+    }
   }
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int r = tm * 16 + 4 * g + v, c = tn * 16 + m;
-    if (r < M && c < N) {
-      float* o = C + (int64_t)r * ldc + c;
-      *o = accumulate ? *o + acc[v] : acc[v];
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = row0 + 16 * wv + 4 * g + v, c = col0 + 16 * t + m;
+      if (r < M && c < N) {
+        float* o = C + (int64_t)r * ldc + c;
+        *o = accumulate ? *o + acc[t][v] : acc[t][v];
+      }
     }
-  }
 }
 
 // Z[n x N] += bias; optionally H = relu(Z) in place.
@@ -276,15 +341,29 @@ __global__ void k_softmax(const float* __restrict__ Z, int64_t n, int K, float* 
   for (int k = 0; k < K; ++k) P[i * K + k] = expf(z[k] - mx) / s;
 }
 
-// Mx[k * K + k2] = sum_i T[i, k2] log P[i, k]  (thread per (k, k2); fixed order)
-__global__ void k_match(const float* __restrict__ P, const float* __restrict__ T, int64_t n, int K,
-                        double* __restrict__ Mx) {
-  const int t = threadIdx.x;
-  if (t >= K * K) return;
-  const int k = t / K, k2 = t - k * K;
+// Mx[k * K + k2] = sum_i T[i, k2] log P[i, k], in two fixed-order stages: workgroup b sums rows b, b + nb, ...
+// (four row groups of 64 threads, thread = entry (k, k2)) into part[b][64]; k_match_sum adds the nb partials.
+constexpr int MATCH_BLOCKS = 64;
+__global__ __launch_bounds__(256) void k_match(const float* __restrict__ P, const float* __restrict__ T, int64_t n,
+                                               int K, double* __restrict__ part) {
+  __shared__ double sh[4][64];
+  const int e = threadIdx.x & 63, rg = threadIdx.x >> 6;
   double s = 0.0;
-  for (int64_t i = 0; i < n; ++i) s += (double)T[i * K + k2] * log(fmaxf(P[i * K + k], 1e-30f));
-  Mx[t] = s;
+  if (e < K * K) {
+    const int k = e / K, k2 = e - k * K;
+    for (int64_t i = (int64_t)blockIdx.x * 4 + rg; i < n; i += (int64_t)gridDim.x * 4)
+      s += (double)T[i * K + k2] * log(fmaxf(P[i * K + k], 1e-30f));
+  }
+  sh[rg][e] = s;
+  __syncthreads();
+  if (rg == 0) part[blockIdx.x * 64 + e] = ((sh[0][e] + sh[1][e]) + sh[2][e]) + sh[3][e];
+}
+
+__global__ void k_match_sum(const double* __restrict__ part, int nb, double* __restrict__ Mx) {
+  const int e = threadIdx.x;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[b * 64 + e];
+  Mx[e] = s;
 }
 
 // dZ[i, k] = (P[i, k] - T[i, perm[k]]) / n
@@ -303,13 +382,23 @@ __global__ void k_relu_back(float* __restrict__ dZ, const float* __restrict__ H,
   if (t < n && !(H[t] > 0.0f)) dZ[t] = 0.0f;
 }
 
-// db[c] = sum_i dZ[i, c]
-__global__ void k_colsum(const float* __restrict__ dZ, int64_t n, int N, float* __restrict__ db) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
+// db[c] = sum_i dZ[i, c]: workgroup = 16 columns x 16 row groups (rows i = rg, rg + 16, ...), the 16 partials of a
+// column added in a fixed order.
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ dZ, int64_t n, int N, float* __restrict__ db) {
+  __shared__ float sh[16][17];
+  const int cl = threadIdx.x & 15, rg = threadIdx.x >> 4;
+  const int c = blockIdx.x * 16 + cl;
   float s = 0.f;
-  for (int64_t i = 0; i < n; ++i) s += dZ[i * N + c];
-  db[c] = s;
+  if (c < N)
+    for (int64_t i = rg; i < n; i += 16) s += dZ[i * N + c];
+  sh[rg][cl] = s;
+  __syncthreads();
+  if (rg == 0 && c < N) {
+    float t = 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) t += sh[r][cl];
+    db[c] = t;
+  }
 }
 
 // Adam on the FP32 masters, then the BF16 copy of the weight matrices (mask[t] = 1 for matrix entries).
@@ -380,9 +469,9 @@ static unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b
 template <typename TB>
 static hipError_t gemm(hipStream_t s, int M, int N, int Kd, const float* A, int64_t sam, int64_t sak, const TB* B,
                        int64_t sbk, int64_t sbn, float* C, int64_t ldc, int acc) {
-  const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
-  hipLaunchKernelGGL((k_gemm<TB>), dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, M, N, Kd, A, sam, sak, B, sbk,
-                     sbn, C, ldc, acc);
+  if (M < 1 || N < 1) return hipSuccess;
+  hipLaunchKernelGGL((k_gemm<TB>), dim3((unsigned)((M + GT - 1) / GT), (unsigned)((N + GT - 1) / GT)), dim3(256), 0, s,
+                     M, N, Kd, A, sam, sak, B, sbk, sbn, C, ldc, acc);
   return hipGetLastError();
 }
 
@@ -469,7 +558,7 @@ int hk_mlp_create(hk_mlp** out, int din, int width, int dout, uint64_t seed) {
   H(hipMalloc(&m->m, sizeof(float) * o));
   H(hipMalloc(&m->v, sizeof(float) * o));
   H(hipMalloc(&m->wb, sizeof(uint16_t) * o));
-  H(hipMalloc(&m->Mx, sizeof(double) * 64));
+  H(hipMalloc(&m->Mx, sizeof(double) * 64 * (1 + MATCH_BLOCKS)));  // result, then the stage-1 partials
   H(hipMalloc(&m->perm, sizeof(int) * 8));
   if (rc == HK_OK) {
     // He-uniform initialisation on the host (seeded), biases zero.
@@ -578,7 +667,9 @@ int hk_mlp_grad(hk_mlp* m, int64_t row0, int64_t n, int match, float* loss, int*
   if (rc != HK_OK) return rc;
   rc = mlp_forward_dev(m, X, n);
   if (rc != HK_OK) return rc;
-  hipLaunchKernelGGL(k_match, dim3(1), dim3(64), 0, m->s, m->P, T, n, K, m->Mx);
+  const int nbm = (int)std::min<int64_t>(MATCH_BLOCKS, (n + 255) / 256);
+  hipLaunchKernelGGL(k_match, dim3(nbm), dim3(256), 0, m->s, m->P, T, n, K, m->Mx + 64);
+  hipLaunchKernelGGL(k_match_sum, dim3(1), dim3(64), 0, m->s, m->Mx + 64, nbm, m->Mx);
   double Mx[64];
   MH(hipMemcpyAsync(Mx, m->Mx, sizeof(double) * K * K, hipMemcpyDeviceToHost, m->s));
   MH(hipStreamSynchronize(m->s));
@@ -620,17 +711,17 @@ int hk_mlp_grad(hk_mlp* m, int64_t row0, int64_t n, int match, float* loss, int*
   hipLaunchKernelGGL(k_dlogits, dim3(nblk(n * K)), dim3(256), 0, m->s, m->P, T, n, K, m->perm, m->dZ3);
   // layer 3: gW3 (H x K) = H2^T dZ3, gb3, dH2 = dZ3 W3^T
   MH(gemm<float>(m->s, H, K, (int)n, m->H2, 1, H, m->dZ3, K, 1, gW3, K, 0));
-  hipLaunchKernelGGL(k_colsum, dim3(nblk(K, 64)), dim3(64), 0, m->s, m->dZ3, n, K, gb3);
+  hipLaunchKernelGGL(k_colsum, dim3(nblk(K, 16)), dim3(256), 0, m->s, m->dZ3, n, K, gb3);
   MH(gemm<uint16_t>(m->s, (int)n, H, K, m->dZ3, K, 1, W3, 1, K, m->dH2, H, 0));
   hipLaunchKernelGGL(k_relu_back, dim3(nblk(n * H)), dim3(256), 0, m->s, m->dH2, m->H2, n * H);
   // layer 2
   MH(gemm<float>(m->s, H, H, (int)n, m->H1, 1, H, m->dH2, H, 1, gW2, H, 0));
-  hipLaunchKernelGGL(k_colsum, dim3(nblk(H)), dim3(256), 0, m->s, m->dH2, n, H, gb2);
+  hipLaunchKernelGGL(k_colsum, dim3(nblk(H, 16)), dim3(256), 0, m->s, m->dH2, n, H, gb2);
   MH(gemm<uint16_t>(m->s, (int)n, H, H, m->dH2, H, 1, W2, 1, H, m->dH1, H, 0));
   hipLaunchKernelGGL(k_relu_back, dim3(nblk(n * H)), dim3(256), 0, m->s, m->dH1, m->H1, n * H);
   // layer 1
   MH(gemm<float>(m->s, D, H, (int)n, X, 1, D, m->dH1, H, 1, gW1, H, 0));
-  hipLaunchKernelGGL(k_colsum, dim3(nblk(H)), dim3(256), 0, m->s, m->dH1, n, H, gb1);
+  hipLaunchKernelGGL(k_colsum, dim3(nblk(H, 16)), dim3(256), 0, m->s, m->dH1, n, H, gb1);
   MH(hipGetLastError());
   MH(hipStreamSynchronize(m->s));
   return HK_OK;
