@@ -144,7 +144,8 @@ def select_bench(a, w, prec):
         "aic_by_k": {str(K): round(b.best_by_k[K]["aic"], 1) for K in Ks},
         "ll_by_k": {str(K): b.best_by_k[K]["ll"] for K in Ks},
         "best_k": b.best_k,
-        "autotuned_chunks_by_k": {str(K): list(v) for K, v in b.tuning.items()},
+        "autotuned_by_k": {str(K): dict(zip(("q_chunks", "p_chunks", "q_shape", "p_shape"), v))
+                           for K, v in b.tuning.items()},
         "iters_by_fit": [t["iters"] for t in b.table],
     }
     print(json.dumps(out), flush=True)

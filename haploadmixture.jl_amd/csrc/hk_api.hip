@@ -435,6 +435,23 @@ int hk_get_tuning(hk_ctx* c, int* qc, int* pc) {
   return HK_OK;
 }
 
+int hk_set_pass_shape(hk_ctx* c, int q_shape, int p_shape) {
+  if (!c || q_shape < 0 || q_shape > 1 || p_shape < 0 || p_shape > 1) return fail(c, HK_EINVAL, "invalid pass shape");
+  if ((q_shape && !has_alt_shape(c->K, c->prec, 0)) || (p_shape && !has_alt_shape(c->K, c->prec, 1)))
+    return fail(c, HK_EINVAL, "no second curvature-pass variant for K=%d in this precision", c->K);
+  drop_graph(c);  // captured iterations hold the kernel choice
+  c->g.qshape = q_shape;
+  c->g.pshape = p_shape;
+  return HK_OK;
+}
+
+int hk_get_pass_shape(hk_ctx* c, int* q_shape, int* p_shape) {
+  if (!c) return HK_EINVAL;
+  if (q_shape) *q_shape = c->g.qshape;
+  if (p_shape) *p_shape = c->g.pshape;
+  return HK_OK;
+}
+
 int hk_sync(hk_ctx* c) {
   if (!c) return HK_EINVAL;
   HIPC(c, hipStreamSynchronize(c->s));
@@ -1354,23 +1371,34 @@ extern "C" int hk_autotune(hk_ctx* c, int reps, int* q_chunks, int* p_chunks) {
   for (int m : mult) qc.push_back(q0 * m);
   for (int d : div) if (p0 / d >= 1) pc.push_back(p0 / d);
   for (int m : mult) pc.push_back(p0 * m);
+  // chunk counts x curvature-pass variants (the second variant only where one exists for K)
+  const int nqs = has_alt_shape(c->K, c->prec, 0) ? 2 : 1, nps = has_alt_shape(c->K, c->prec, 1) ? 2 : 1;
+  int qsbest = 0, psbest = 0;
   for (int qv : qc) {
     if (qv > c->g.J8) continue;
     CHK(hk_set_tuning(c, qv, preq0));
-    double tl = 0, tu = 0;
+    double tl = 0;
     CHK(time_full_pass(c, 0, reps, &tl));
-    CHK(time_full_pass(c, 1, reps, &tu));
-    if (tl + tu < tq_best) { tq_best = tl + tu; qbest = c->g.qchunks; }
+    for (int sh = 0; sh < nqs; ++sh) {
+      CHK(hk_set_pass_shape(c, sh, 0));
+      double tu = 0;
+      CHK(time_full_pass(c, 1, reps, &tu));
+      if (tl + tu < tq_best) { tq_best = tl + tu; qbest = c->g.qchunks; qsbest = sh; }
+    }
   }
   for (int pv : pc) {
     if (pv > c->g.I8) continue;
     CHK(hk_set_tuning(c, qbest, pv));
-    double tp = 0;
-    CHK(time_full_pass(c, 2, reps, &tp));
-    if (tp < tp_best) { tp_best = tp; pbest = c->g.pchunks; }
+    for (int sh = 0; sh < nps; ++sh) {
+      CHK(hk_set_pass_shape(c, qsbest, sh));
+      double tp = 0;
+      CHK(time_full_pass(c, 2, reps, &tp));
+      if (tp < tp_best) { tp_best = tp; pbest = c->g.pchunks; psbest = sh; }
+    }
   }
   (void)qreq0;
   CHK(hk_set_tuning(c, qbest, pbest));
+  CHK(hk_set_pass_shape(c, qsbest, psbest));
   if (q_chunks) *q_chunks = qbest;
   if (p_chunks) *p_chunks = pbest;
   return HK_OK;

@@ -25,7 +25,15 @@ struct PassGeom {
   void* qq;            // staging table of the P-side curvature pass (TP; rows per individual)
   void* pp;            // staging table of the Q-side curvature pass (TP; rows per block)
   const int* gate;     // device gate of the fit's iteration kernels (nullptr: always run)
+  // Curvature-pass variant per side (FP32 passes, K <= 8): 0 = the default for K, 1 = the other staging form
+  // where one compiles without spills (K <= 6: without the register prefetch; Q side at K = 7: with it).  Both
+  // forms do the same arithmetic in the same order; the autotuner times them.
+  int qshape = 0, pshape = 0;
 };
+// Whether the FP32 curvature pass of side (0 = Q, 1 = P) has a second variant at K.
+inline bool has_alt_shape(int K, int prec, int side) {
+  return prec != 0 && (K <= 6 || (K == 7 && side == 0));
+}
 
 int nacc(int K, int mode);
 // Element counts of the curvature passes' staging tables (P side: nq, Q side: np).

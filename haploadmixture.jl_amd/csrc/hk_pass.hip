@@ -990,10 +990,15 @@ static hipError_t qside_k(int mode, const PassGeom& g, const void* q, const void
     return launch_qside_curv_wide(K, sizeof(TP) == 8 ? PREC_FP64 : PREC_MIXED, g, q, part, s);
   } else {
     const dim3 gmm((unsigned)(g.Ip / 128), (unsigned)g.qchunks);
-    if constexpr (K <= 6 && sizeof(TP) == 4)
-      hipLaunchKernelGGL((k_qside_mm_pf<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
-                         (const TP*)g.pp, part, g.flush_curv, g.gate);
-    else
+    // FP32: prefetching staging by default up to K = 6; g.qshape picks the other form (K <= 7)
+    bool pf = false;
+    if constexpr (sizeof(TP) == 4 && K <= 7) pf = (K <= 6) != (g.qshape != 0);
+    if constexpr (sizeof(TP) == 4 && K <= 7) {
+      if (pf)
+        hipLaunchKernelGGL((k_qside_mm_pf<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
+                           (const TP*)g.pp, part, g.flush_curv, g.gate);
+    }
+    if (!pf)
       hipLaunchKernelGGL((k_qside_mm<K, TP>), gmm, block, 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, qq, pp,
                          (const TP*)g.pp, part, g.flush_curv, g.gate);
   }
@@ -1014,10 +1019,15 @@ static hipError_t pside_k(int mode, const PassGeom& g, const void* q, const void
                          block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp, out, g.flush_grad, g.gate);
     } else {
       const dim3 gmm((unsigned)(g.Jp / 64), (unsigned)g.pchunks);
-      if constexpr (K <= 6 && sizeof(TP) == 4)
-        hipLaunchKernelGGL((k_pside_mm_pf<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
-                           (const TP*)g.qq, out, g.flush_curv, g.gate);
-      else
+      // FP32 up to K = 6: prefetching staging by default, g.pshape picks the form without it
+      bool pf = false;
+      if constexpr (sizeof(TP) == 4 && K <= 6) {
+        pf = g.pshape == 0;
+        if (pf)
+          hipLaunchKernelGGL((k_pside_mm_pf<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
+                             (const TP*)g.qq, out, g.flush_curv, g.gate);
+      }
+      if (!pf)
         hipLaunchKernelGGL((k_pside_mm<K, TP>), gmm, block, 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, qq, pp,
                            (const TP*)g.qq, out, g.flush_curv, g.gate);
     }

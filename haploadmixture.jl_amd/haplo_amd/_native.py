@@ -116,6 +116,8 @@ def hip():
     L.hk_fit_ex.argtypes = [_vp, ctypes.POINTER(FitOpts), _dbl, _vp, _vp, ctypes.POINTER(FitResult)]
     L.hk_set_tuning.argtypes = [_vp, _int, _int]
     L.hk_get_tuning.argtypes = [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int)]
+    L.hk_set_pass_shape.argtypes = [_vp, _int, _int]
+    L.hk_get_pass_shape.argtypes = [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int)]
     L.hk_sync.argtypes = [_vp]
     L.hk_time_pass.argtypes = [_vp, _int, _int, ctypes.POINTER(_dbl)]
     L.hk_copy_bench.argtypes = [_vp, _i64, _int, ctypes.POINTER(_dbl)]

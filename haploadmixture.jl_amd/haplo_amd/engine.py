@@ -257,6 +257,15 @@ class HipAdmixData:
         self._check(self._L.hk_get_tuning(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def set_pass_shape(self, q_shape=0, p_shape=0):
+        """Curvature-pass variant per side (hk_set_pass_shape): 0 = default for K, 1 = the other staging form."""
+        self._check(self._L.hk_set_pass_shape(self._h, int(q_shape), int(p_shape)))
+
+    def get_pass_shape(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.hk_get_pass_shape(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def time_pass(self, kind: int, reps: int = 5) -> float:
         ms = ctypes.c_double()
         self._check(self._L.hk_time_pass(self._h, int(kind), int(reps), ctypes.byref(ms)))

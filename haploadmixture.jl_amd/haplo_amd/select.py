@@ -64,10 +64,12 @@ def select_k(bed, I, J, Ks=(2, 3, 4, 5, 6, 7, 8), restarts=4, seed=1, precision=
             for K in Ks:
                 first = next(e for e, (k, _) in zip(engines, specs) if k == K)
                 qc, pc = first.autotune(reps=2)
-                out.tuning[K] = (qc, pc)
+                qs, ps = first.get_pass_shape()
+                out.tuning[K] = (qc, pc, qs, ps)
                 for e, (k, _) in zip(engines, specs):
                     if k == K and e is not first:
                         e.set_tuning(qc, pc)
+                        e.set_pass_shape(qs, ps)
         opts = [HipAdmixData.default_opts(em_iters=em_iters, n_iter=n_iter, rtol=rtol, Qn=Q,
                                           use_graph=int(use_graph), batch_rounds=int(batch_rounds)) for _ in engines]
         t0 = time.perf_counter()

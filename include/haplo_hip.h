@@ -200,10 +200,19 @@ int hk_fit_ex(hk_ctx* ctx, const hk_fit_opts* opts, double ll_start, double* ll_
 int hk_fit_batch(hk_ctx** ctxs, int n, const hk_fit_opts* opts, const double* ll_start, hk_fit_result* res,
                  double* seconds_at_finish);
 
-/* Autotuner: time candidate chunk shapes of the passes on the context's data (reps launches each) and keep
- * the fastest (each pass timed with the reduction over its chunks); the chosen q/p chunk counts are
- * returned. */
+/* Autotuner: time candidate chunk counts of the passes on the context's data (reps launches each; each pass
+ * timed with the reduction over its chunks) and, where K and the precision have two curvature-pass variants
+ * (hk_set_pass_shape), both variants at each count; keep the fastest.  The chosen q/p chunk counts are returned,
+ * the variants are read with hk_get_pass_shape.  No counterpart in the reference (its kernels take fixed launch
+ * shapes). */
 int hk_autotune(hk_ctx* ctx, int reps, int* q_chunks, int* p_chunks);
+
+/* Curvature-pass variant per side (q: the individual-parallel update_q pass, p: the block-parallel update_p
+ * pass): 0 = the default for K, 1 = the other staging form (FP32 passes: K <= 6 on both sides, register
+ * prefetch of the next stage off; K = 7 on the Q side, prefetch on).  Same arithmetic in the same order, so
+ * results are bitwise the same; HK_EINVAL where K and the precision have no second variant. */
+int hk_set_pass_shape(hk_ctx* ctx, int q_shape, int p_shape);
+int hk_get_pass_shape(hk_ctx* ctx, int* q_shape, int* p_shape);
 
 /* Pass geometry tuning: number of j chunks of the individual-parallel passes and i chunks of the
  * block-parallel passes (0 = automatic). */

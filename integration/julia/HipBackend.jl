@@ -240,7 +240,7 @@ function hip_select(dh::HipAdmixData, Ks, restarts; rtol=1e-7, seed=1, precision
     ctxs, specs = Ptr{Cvoid}[], Tuple{Int,Int}[]
     try
         for K in Ks
-            qc, pc = Ref{Cint}(0), Ref{Cint}(0)
+            qc, pc, qs, ps = Ref{Cint}(0), Ref{Cint}(0), Ref{Cint}(0), Ref{Cint}(0)
             for r in 1:restarts
                 h = Ref{Ptr{Cvoid}}(C_NULL)
                 hk_check(dh.h, ccall((:hk_ctx_create_shared, libhk), Cint, (Ref{Ptr{Cvoid}}, Ptr{Cvoid}, Cint, Cint),
@@ -254,8 +254,10 @@ function hip_select(dh::HipAdmixData, Ks, restarts; rtol=1e-7, seed=1, precision
                 hk_check(h[], ccall((:hk_set_params, libhk), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), h[], q, p))
                 if r == 1
                     hk_check(h[], ccall((:hk_autotune, libhk), Cint, (Ptr{Cvoid}, Cint, Ref{Cint}, Ref{Cint}), h[], 2, qc, pc))
+                    hk_check(h[], ccall((:hk_get_pass_shape, libhk), Cint, (Ptr{Cvoid}, Ref{Cint}, Ref{Cint}), h[], qs, ps))
                 else
                     hk_check(h[], ccall((:hk_set_tuning, libhk), Cint, (Ptr{Cvoid}, Cint, Cint), h[], qc[], pc[]))
+                    hk_check(h[], ccall((:hk_set_pass_shape, libhk), Cint, (Ptr{Cvoid}, Cint, Cint), h[], qs[], ps[]))
                 end
             end
         end

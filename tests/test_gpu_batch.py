@@ -64,6 +64,23 @@ def test_batch_matches_sequential_fits(rounds):
     assert np.all(fin > 0)
 
 
+@pytest.mark.parametrize("K", [4, 6, 7])
+def test_pass_shapes_give_identical_curvature(K):
+    """Both variants of the FP32 curvature passes (register prefetch of the next staging step or not) do the same
+    arithmetic in the same order: bitwise the same XtX / Xtz; sides without a second variant are rejected."""
+    s, q, p = case(700, 600, K, miss=0.02)
+    with HipAdmixData(s.bed, 700, 600, K, precision="mixed") as e:
+        e.set_params(q, p)
+        a = (e.update_q(), e.update_p())
+        e.set_pass_shape(1, 1 if K <= 6 else 0)
+        b = (e.update_q(), e.update_p())
+        if K == 7:
+            with pytest.raises(Exception):
+                e.set_pass_shape(0, 1)
+    for x, y in zip(a, b):
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+
+
 def test_autotune_keeps_results():
     s, q, p = case(600, 900, 5, miss=0.02)
     with HipAdmixData(s.bed, 600, 900, 5, precision="fp64") as e:

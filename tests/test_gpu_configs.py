@@ -71,8 +71,10 @@ def test_c4_batch_equals_single_fits_and_aic_picks_true_k():
         for K in C4_KS:  # per-K chunk shapes from the autotuner, as select_k does
             group = [e for e, (k, _) in zip(engines, specs) if k == K]
             qc, pc = group[0].autotune(reps=2)
+            qs, ps = group[0].get_pass_shape()
             for e in group[1:]:
                 e.set_tuning(qc, pc)
+                e.set_pass_shape(qs, ps)
             group[0].set_params(*inits[specs.index((K, 0))])
         opts = [HipAdmixData.default_opts(rtol=1e-7, n_iter=1000) for _ in engines]
         res, fin = fit_batch(engines, opts)
