@@ -46,7 +46,7 @@ namespace hk {
 
 namespace {
 
-constexpr int SK_MAXC = HK_MAXK_REG;  // largest number of clusters (per-lane cluster arrays)
+constexpr int SK_MAXC = HK_MAXK;  // largest number of clusters (per-lane cluster arrays)
 constexpr int SK_CHUNK = 1024;    // blocks per partial sum of the all-SNP distance kernel
 constexpr int SK_SUB = 256;       // blocks (or kept-list entries) per LDS stage
 constexpr int SK_CW = 128;        // genotype words (1024 individuals) per partial of the center counts
@@ -114,7 +114,10 @@ __global__ void k_sk_row(const uint32_t* __restrict__ gq, int64_t Ip, int64_t J,
 // differences e2[c][snp][code] = (z(code) - row_c,snp)^2 (code 0b01 = missing: z = 0), computed with the
 // same operations as the element-wise form, so each element costs one LDS gather and one add per center.
 constexpr size_t SK_DIST_LDS = 16384;  // bytes: 2048 doubles of e2 table per stage
-__host__ __device__ constexpr int dist_sub(int nc) { return nc <= 1 ? 256 : (nc <= 2 ? 128 : (nc <= 4 ? 64 : 32)); }
+// (NC x 4 x 2 dist_sub doubles = SK_DIST_LDS bytes at most: 2048 doubles for every NC up to 16)
+__host__ __device__ constexpr int dist_sub(int nc) {
+  return nc <= 1 ? 256 : (nc <= 2 ? 128 : (nc <= 4 ? 64 : (nc <= 8 ? 32 : 16)));
+}
 
 template <int NC>
 __global__ void __launch_bounds__(256) k_sk_dist(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J,
@@ -124,6 +127,7 @@ __global__ void __launch_bounds__(256) k_sk_dist(const uint32_t* __restrict__ gq
   double* e2 = sk_lds;  // [NC][2 sub][4]
   constexpr int nc = NC;
   constexpr int sub = dist_sub(NC), snps = 2 * sub, NWQ = sub / 8;
+  static_assert(NC * 4 * snps * sizeof(double) <= SK_DIST_LDS, "the e2 table of a stage exceeds SK_DIST_LDS");
   const int tid = threadIdx.x;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
   const bool act = i < I;
@@ -366,19 +370,25 @@ __global__ void k_sk_zero(int64_t J, int k, int local, const unsigned* __restric
   flags[b] = m != 0u ? 1 : 0;
 }
 
-// LDS bytes of the kept-list staging of k_sk_cost / k_sk_wss: value table (8 SK_SUB) + theta (2 SK_SUB k).
-__host__ __device__ constexpr size_t kept_lds_bytes(int k) { return (size_t)(8 * SK_SUB + 2 * SK_SUB * k) * 8; }
+// Kept-list entries per LDS stage of k_sk_wss: SK_SUB up to 8 clusters, half of it above, so that the staging
+// (value table 8 sub + theta 2 sub k doubles) stays within 48 KB: kept_lds_bytes(k).  The stage size does not
+// change the order of the sums.
+__host__ __device__ constexpr int kept_sub(int k) { return k <= 8 ? SK_SUB : SK_SUB / 2; }
+__host__ __device__ constexpr size_t kept_lds_bytes(int k) {
+  return (size_t)(8 * kept_sub(k) + 2 * kept_sub(k) * k) * 8;
+}
+static_assert(kept_lds_bytes(SK_MAXC) <= 48 * 1024 && kept_lds_bytes(8) <= 48 * 1024, "kept-list staging exceeds 48 KB");
 
-// Stage SK_SUB kept-list entries [u0, u0 + nu) in LDS: zt[8 t + 4 a + d] = value table of SNP 2 b_t + a,
-// th[c * 2 SK_SUB + 2 t + a] = theta of cluster c there.  Called by all threads between barriers.
-__device__ __forceinline__ void stage_kept(const int* __restrict__ blocks, int u0, int nu, int64_t J, int k,
+// Stage kept-list entries [u0, u0 + nu) (nu <= sub) in LDS: zt[8 t + 4 a + d] = value table of SNP 2 b_t + a,
+// th[c * 2 sub + 2 t + a] = theta of cluster c there.  Called by all threads between barriers.
+__device__ __forceinline__ void stage_kept(const int* __restrict__ blocks, int u0, int nu, int sub, int64_t J, int k,
                                            const double* __restrict__ ztab, const double* __restrict__ theta,
                                            double* zt, double* th) {
   const int tid = threadIdx.x;
   for (int x = tid; x < 8 * nu; x += 256) zt[x] = ztab[8 * (int64_t)blocks[u0 + (x >> 3)] + (x & 7)];
   for (int x = tid; x < 2 * nu * k; x += 256) {
     const int c = x / (2 * nu), r = x - c * 2 * nu;
-    th[c * 2 * SK_SUB + r] = theta[(int64_t)c * 2 * J + 2 * (int64_t)blocks[u0 + (r >> 1)] + (r & 1)];
+    th[c * 2 * sub + r] = theta[(int64_t)c * 2 * J + 2 * (int64_t)blocks[u0 + (r >> 1)] + (r & 1)];
   }
 }
 
@@ -495,8 +505,9 @@ __global__ void __launch_bounds__(256) k_sk_wss(const uint32_t* __restrict__ gq,
                                                 const double* __restrict__ theta, const int* __restrict__ labels,
                                                 double* __restrict__ pw) {
   extern __shared__ double sk_lds[];
+  const int sub = kept_sub(k);
   double* zt = sk_lds;
-  double* th = sk_lds + 8 * SK_SUB;
+  double* th = sk_lds + 8 * sub;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = i < I;
   const int nb = *nblocks;
@@ -505,10 +516,10 @@ __global__ void __launch_bounds__(256) k_sk_wss(const uint32_t* __restrict__ gq,
   const int t1 = t0 + ach < nb ? t0 + ach : nb;
   const int lab = act ? labels[i] : 0;
   double w = 0.0;
-  for (int u0 = t0; u0 < t1; u0 += SK_SUB) {
-    const int nu = t1 - u0 < SK_SUB ? t1 - u0 : SK_SUB;
+  for (int u0 = t0; u0 < t1; u0 += sub) {
+    const int nu = t1 - u0 < sub ? t1 - u0 : sub;
     __syncthreads();
-    stage_kept(blocks, u0, nu, J, k, ztab, theta, zt, th);
+    stage_kept(blocks, u0, nu, sub, J, k, ztab, theta, zt, th);
     __syncthreads();
     if (!act) continue;
     for (int t = 0; t < nu; t += 4) {
@@ -523,7 +534,7 @@ __global__ void __launch_bounds__(256) k_sk_wss(const uint32_t* __restrict__ gq,
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int r = 2 * t + (u < 2 * tn ? u : 0);
-        const double tb = th[lab * 2 * SK_SUB + r];
+        const double tb = th[lab * 2 * sub + r];
         e[u] = zval(zt, r, (nib[u >> 1] >> (2 * (u & 1))) & 3u, tb) - tb;
       }
 #pragma unroll
@@ -628,7 +639,15 @@ struct LaunchTimer {
       case 5: { constexpr int KC_ = 5; LAUNCH; } break; \
       case 6: { constexpr int KC_ = 6; LAUNCH; } break; \
       case 7: { constexpr int KC_ = 7; LAUNCH; } break; \
-      default: { constexpr int KC_ = 8; LAUNCH; } break; \
+      case 8: { constexpr int KC_ = 8; LAUNCH; } break; \
+      case 9: { constexpr int KC_ = 9; LAUNCH; } break; \
+      case 10: { constexpr int KC_ = 10; LAUNCH; } break; \
+      case 11: { constexpr int KC_ = 11; LAUNCH; } break; \
+      case 12: { constexpr int KC_ = 12; LAUNCH; } break; \
+      case 13: { constexpr int KC_ = 13; LAUNCH; } break; \
+      case 14: { constexpr int KC_ = 14; LAUNCH; } break; \
+      case 15: { constexpr int KC_ = 15; LAUNCH; } break; \
+      default: { constexpr int KC_ = 16; LAUNCH; } break; \
     }                                                \
   } while (0)
 

@@ -48,7 +48,7 @@ enum hk_precision {
 /* Largest K.  K <= 8 runs the register-resident pass and four-lane P-step kernels; 9 <= K <= 16 runs passes over
  * windows of the packed K x K columns and the dense one-wave solver for every P row QP (slower per pair). */
 #define HK_MAX_K 16
-#define HK_SKFR_MAX_K 8 /* largest SKFR cluster count */
+#define HK_SKFR_MAX_K 16 /* largest SKFR cluster count */
 
 /* _cu_admixture_base (src/cuda/structs.jl:34-44, CuAdmixData :17-32): upload the genotypes of the first
  * I individuals and first 2J SNPs of a BED payload with bytes_per_snp = ceil(I_file/4) bytes per SNP and

@@ -36,7 +36,7 @@ import numpy as np
 
 CHUNK = 1024          # blocks per partial sum of the all-SNP distances (device: SK_CHUNK)
 ACH_MIN = 256         # kept blocks per partial of the reassignment sums, at least (device: SK_ACH_MIN)
-MAX_CLUSTERS = 8
+MAX_CLUSTERS = 16    # the device's per-lane cluster arrays (the published method has no limit)
 _MASK = (1 << 64) - 1
 
 

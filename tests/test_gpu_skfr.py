@@ -48,6 +48,8 @@ CASES = [
     (9, 5, 3, 2, "global", 1, 0.1, 15, None),            # fewer individuals / blocks than a genotype word
     (16, 8, 2, 8, "local", 2, 0.3, 16, [3, 11]),         # heavy missingness, given starts, then a k-means++ try
     (7, 3, 7, 3, "global", 1, 0.0, 17, None),            # k = I
+    (600, 200, 12, 40, "global", 1, 0.02, 21, None),     # k above 8
+    (400, 300, 16, 60, "local", 1, 0.02, 22, None),      # the largest k
 ]
 
 
@@ -101,7 +103,7 @@ def test_skfr_fixed_point_at_scale():
 def test_skfr_rejects_bad_arguments():
     D, _ = planted(40, 20, 2, informative=[0, 5], seed=1)
     with engine_for(D) as eng:
-        for kw in (dict(k=9, sparsity=2), dict(k=2, sparsity=21), dict(k=2, sparsity=0), dict(k=0, sparsity=2)):
+        for kw in (dict(k=17, sparsity=2), dict(k=2, sparsity=21), dict(k=2, sparsity=0), dict(k=0, sparsity=2)):
             with pytest.raises(HipError):
                 eng.skfr(kw["k"], kw["sparsity"])
         with pytest.raises(HipError):

@@ -185,11 +185,11 @@ def test_wide_fp32_accumulation_mode(oracle):
 
 
 def test_skfr_cluster_limit_is_checked():
-    """SKFR keeps per-lane cluster arrays of 8: k = 9 is rejected with a message, the context stays usable."""
+    """SKFR keeps per-lane cluster arrays of 16: k = 17 is rejected with a message, the context stays usable."""
     s = simulate(64, 40, 3, seed=2)
     with HipAdmixData(s.bed, 64, 40, 12) as e:
         with pytest.raises(HipError):
-            e.skfr(9, 10)
+            e.skfr(17, 10)
         r = e.skfr(3, 10, max_iter=5, seed=1)
         assert len(r["clusters"]) == 64
 

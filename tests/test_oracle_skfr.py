@@ -161,7 +161,7 @@ def test_chunked_distances_match_direct_sums():
 
 def test_invalid_arguments():
     D, _ = planted(20, 10, 2, informative=[0], seed=1)
-    for kw in (dict(k=0, sparsity=2), dict(k=9, sparsity=2), dict(k=2, sparsity=11), dict(k=2, sparsity=0)):
+    for kw in (dict(k=0, sparsity=2), dict(k=17, sparsity=2), dict(k=2, sparsity=11), dict(k=2, sparsity=0)):
         with pytest.raises(ValueError):
             so.skfr(D, kw["k"], kw["sparsity"])
     with pytest.raises(ValueError):
