@@ -173,7 +173,8 @@ static int run_qpass(hk_ctx* c, int mode, const double* q, const double* p, Time
   if (mode == MODE_CURV) HIPC(c, launch_curv_tables(c->K, c->prec, 0, g, qq, pp, c->s));
   if (tm) tm->begin(cat);
   if (mode == MODE_LL && c->prec != PREC_FP64)
-    HIPC(c, launch_ll_mm(c->K, g, (const float*)qq, (const float*)pp, nullptr, nullptr, c->part_q, c->s));
+    HIPC(c, launch_ll_mm(c->K, g, (const float*)qq, (const float*)pp, nullptr, nullptr, c->part_q, c->ll_stage,
+                         c->s));
   else
     HIPC(c, launch_qside(c->K, c->prec, mode, g, qq, pp, c->part_q, c->s));
   if (tm) tm->end();
@@ -222,7 +223,8 @@ static int run_ll_pair(hk_ctx* c, const double* q1, const double* p1, const doub
   CHK(pass_params(c, q1, p1, &qa, &pa, gate, c->x32));
   CHK(pass_params(c, q2, p2, &qb, &pb, gate, c->x32b));
   if (tm) tm->begin(cat);
-  HIPC(c, launch_ll_mm(c->K, g, (const float*)qa, (const float*)pa, (const float*)qb, (const float*)pb, c->part_q, c->s));
+  HIPC(c, launch_ll_mm(c->K, g, (const float*)qa, (const float*)pa, (const float*)qb, (const float*)pb, c->part_q,
+                       c->ll_stage, c->s));
   if (tm) tm->end();
   // set 2 first: set 1's workgroup sums go where set 2's row partials were
   HIPC(c, launch_reduce_ll(c->part_q + (int64_t)g.qchunks * g.Ip, g.qchunks, g, c->ll_rows, ll2, c->s));
@@ -252,8 +254,8 @@ int hk_ctx_destroy(hk_ctx* c) {
   if (c->s) (void)hipStreamSynchronize(c->s);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
-  void* owned[] = {c->qq, c->pp, c->xb[0], c->xb[1], c->xb[2], c->xb[3], c->x32, c->x32b, c->res_q, c->res_p,
-                   c->part_q, c->part_p, c->ll_dev, c->ll_rows, c->U, c->V, c->zal_part, c->zal_coef, c->fails,
+  void* owned[] = {c->qq, c->pp, c->xb[0], c->xb[1], c->xb[2], c->xb[3], c->x32, c->x32b, c->ll_stage,
+                   c->res_q, c->res_p, c->part_q, c->part_p, c->ll_dev, c->ll_rows, c->U, c->V, c->zal_part, c->zal_coef, c->fails,
                    c->redo, c->redo_late, c->pflag, c->full_a, c->full_b, c->d_ll, c->d_iter,
                    c->named[0], c->named[1], c->named[2], c->named[3]};
   for (void* p : owned)
@@ -292,6 +294,7 @@ static int init_fit_buffers(hk_ctx* c) {
   if (c->prec == HK_PREC_MIXED) {
     HIPC(c, hipMalloc(&c->x32, sizeof(float) * c->xlen));
     HIPC(c, hipMalloc(&c->x32b, sizeof(float) * c->xlen));
+    HIPC(c, hipMalloc(&c->ll_stage, 2 * ll_stage_bytes(K, g.J8)));
   }
   const int NT = ntq(K);
   HIPC(c, hipMalloc(&c->res_q, sizeof(double) * I * NT));
@@ -1426,7 +1429,7 @@ extern "C" int hk_time_pass(hk_ctx* c, int kind, int reps, double* ms) {
     if (kind == 0)
       e = c->prec == PREC_FP64 ? launch_qside(c->K, c->prec, MODE_LL, c->g, qq, pp, c->part_q, c->s)
                                : launch_ll_mm(c->K, c->g, (const float*)qq, (const float*)pp, nullptr, nullptr,
-                                              c->part_q, c->s);
+                                              c->part_q, c->ll_stage, c->s);
     else if (kind == 1) e = launch_qside(c->K, c->prec, MODE_CURV, c->g, qq, pp, c->part_q, c->s);
     else if (kind == 2)
       e = launch_pside(c->K, c->prec, MODE_CURV, c->g, qq, pp, c->g.pchunks == 1 ? c->res_p : c->part_p, c->s);

@@ -49,6 +49,24 @@ __device__ __forceinline__ float hk_fma(float a, float b, float c) { return __bu
 __device__ __forceinline__ double hk_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // ---------------------------------------------------------------------------------------------
+// BF16 terms of FP32 values for the BF16 matrix instructions.  split3: x = h + m + l, each term the rounded
+// remainder of the previous ones, so the three carry the 24 bits of x (each remainder is exact in FP32: it has at
+// most 16, then 8 significant bits), and every BF16 x BF16 product is exact in an FP32 accumulator.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ void split3(float x, uint16_t& h, uint16_t& m, uint16_t& l) {
+  h = f32_to_bf16(x);
+  const float r1 = x - bf16_to_f32(h);
+  m = f32_to_bf16(r1);
+  l = f32_to_bf16(r1 - bf16_to_f32(m));
+}
+
+// ---------------------------------------------------------------------------------------------
 // Reciprocals.  FP32: the hardware v_rcp_f32 (1 ulp).  FP64: v_rcp_f64 seed + two Newton steps.
 // ---------------------------------------------------------------------------------------------
 __device__ __forceinline__ float hk_rcp(float x) { return __builtin_amdgcn_rcpf(x); }

@@ -52,8 +52,10 @@ hipError_t launch_qside(int K, int prec, int mode, const PassGeom& g, const void
                         hipStream_t s);
 // Log-likelihood pass of the MIXED engine on the matrix cores: one parameter set (q2 = p2 = nullptr) or two in
 // one genotype sweep; per-chunk row partials of set s at part + s * qchunks * Ip (launch_reduce_ll on each).
+// stage: ll_stage_bytes(K, g.J8) per set (16-byte aligned), the BF16 terms of p that the pass reads.
 hipError_t launch_ll_mm(int K, const PassGeom& g, const float* q1, const float* p1, const float* q2, const float* p2,
-                        double* part, hipStream_t s);
+                        double* part, void* stage, hipStream_t s);
+size_t ll_stage_bytes(int K, int64_t J8);
 hipError_t launch_pside(int K, int prec, int mode, const PassGeom& g, const void* q, const void* p, double* out,
                         hipStream_t s);
 // 9 <= K <= 16 (hk_wide.hip): curvature passes over windows of packed columns, the P-side EM pass on the
@@ -171,6 +173,7 @@ struct hk_ctx {
   int64_t xlen = 0;                                      // K * (I + 4 Jp)
   float* x32 = nullptr;                                  // FP32 mirror of one parameter buffer
   float* x32b = nullptr;                                 // second mirror (two-set log-likelihood sweep)
+  void* ll_stage = nullptr;  // BF16 terms of p for the MIXED log-likelihood pass (two sets)
   double* res_q = nullptr;   // [I][NT]
   double* res_p = nullptr;   // [4Jp][NT]
   double* part_q = nullptr;  // [qchunks][NT][Ip]

@@ -35,13 +35,6 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
-  return (uint16_t)(u >> 16);
-}
-
 // ---------------------------------------------------------------------------------------------------
 // Random projection
 // ---------------------------------------------------------------------------------------------------
@@ -234,13 +227,6 @@ struct ProjBuffers {
 // wave w owns rows 16w .. 16w + 15 and the four 16-column tiles.  Operand layout of the instruction: lane l holds
 // A[l % 16][8 (l / 16) .. + 8] and B[8 (l / 16) .. + 8][l % 16]; register r of the result is D[4 (l / 16) + r][l % 16].
 constexpr int GT = 64, GK = 32, GP = GK + 8;  // tile edge, K-slice, LDS row pitch (16-byte aligned rows)
-
-__device__ __forceinline__ void split3(float x, uint16_t& h, uint16_t& m, uint16_t& l) {
-  h = f32_to_bf16(x);
-  const float r1 = x - bf16_to_f32(h);  // exact: the remainder has at most 16 significant bits
-  m = f32_to_bf16(r1);
-  l = f32_to_bf16(r1 - bf16_to_f32(m));
-}
 
 template <typename TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int Kd, const float* __restrict__ A, int64_t sam,

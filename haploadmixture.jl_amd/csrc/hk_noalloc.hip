@@ -78,11 +78,13 @@ extern "C" int hk_loglik_noalloc(hk_ctx* c, const double* q, const double* p, do
   const bool f32 = c->prec != PREC_FP64;
   const size_t el = f32 ? sizeof(float) : sizeof(double);
   const int64_t nq = (int64_t)c->K * c->I, np = (int64_t)c->K * 4 * c->J, np_pad = (int64_t)c->K * 4 * g0.Jp;
-  // layout: [p padded | q (MIXED) | rows (unless the caller passes them) | chunk partials, workgroup sums]
+  // layout: [p padded | q (MIXED) | BF16 terms of p (MIXED) | rows (unless the caller passes them) |
+  //          chunk partials, workgroup sums]
   const size_t b_p = up256(el * (size_t)np_pad), b_q = f32 ? up256(el * (size_t)nq) : 0;
+  const size_t b_st = f32 ? up256(ll_stage_bytes(c->K, g0.J8)) : 0;
   const size_t b_rows = rows ? 0 : up256(sizeof(double) * (size_t)c->I);
   const size_t nb = (size_t)((c->I + 255) / 256);
-  const size_t fixed = b_p + b_q + b_rows + sizeof(double) * nb;
+  const size_t fixed = b_p + b_q + b_st + b_rows + sizeof(double) * nb;
   const size_t per_chunk = sizeof(double) * (size_t)g0.Ip;
   if (fixed + per_chunk > kScratchBytes)
     return nfail(c, HK_EINVAL,
@@ -94,8 +96,9 @@ extern "C" int hk_loglik_noalloc(hk_ctx* c, const double* q, const double* p, do
   unsigned char* sp = (unsigned char*)base;
   void* pd = sp;
   void* qd = f32 ? (void*)(sp + b_p) : nullptr;
-  double* rows_d = rows ? rows : (double*)(sp + b_p + b_q);
-  double* part = (double*)(sp + b_p + b_q + b_rows);
+  void* stage = f32 ? (void*)(sp + b_p + b_q) : nullptr;
+  double* rows_d = rows ? rows : (double*)(sp + b_p + b_q + b_st);
+  double* part = (double*)(sp + b_p + b_q + b_st + b_rows);
   // The context's j chunks when they fit (the sums are then those of hk_loglik, bit for bit), fewer otherwise.
   PassGeom g = g0;
   int64_t nch = g0.qchunks;
@@ -113,7 +116,7 @@ extern "C" int hk_loglik_noalloc(hk_ctx* c, const double* q, const double* p, do
                        (double*)nullptr, (double*)pd);
   e = hipGetLastError();
   if (e == hipSuccess)
-    e = f32 ? launch_ll_mm(c->K, g, (const float*)qd, (const float*)pd, nullptr, nullptr, part, c->s)
+    e = f32 ? launch_ll_mm(c->K, g, (const float*)qd, (const float*)pd, nullptr, nullptr, part, stage, c->s)
             : launch_qside(c->K, c->prec, MODE_LL, g, (const void*)q, pd, part, c->s);
   if (e == hipSuccess) e = launch_reduce_ll(part, g.qchunks, g, rows_d, ll, c->s);
   if (e != hipSuccess) return nfail(c, HK_EHIP, "hk_loglik_noalloc: launch failed: %s", hipGetErrorString(e));

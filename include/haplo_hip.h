@@ -228,8 +228,8 @@ int hk_sync(hk_ctx* ctx);
  * stream and not waited for (hk_sync).  It calls no hipMalloc / hipFree: the staged parameters and partial
  * sums live in a fixed 64 MiB device array that each GPU holds for the life of the process, shared by every
  * call on that GPU -- calls that overlap on one GPU race.  A problem that needs more than that (about
- * K * 4J * 4 bytes for the MIXED engine, 8 for FP64, plus one partial row of I doubles) is rejected with
- * HK_EINVAL. */
+ * (16 K + 128 ceil(6K / 16)) J bytes for the MIXED engine -- FP32 p and its BF16 terms, 480 J at K = 6 --,
+ * 32 K J for FP64, plus one partial row of I doubles) is rejected with HK_EINVAL. */
 int hk_loglik_noalloc(hk_ctx* ctx, const double* q, const double* p, double* ll, double* rows);
 
 /* Time one pass kind on the current parameters: kind 0 = log-likelihood, 1 = update_q pass,
