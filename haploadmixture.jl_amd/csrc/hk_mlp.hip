@@ -6,10 +6,10 @@
 // and R a seeded Gaussian S x D matrix (Box-Muller over a counter hash, D = 256).
 //
 // MLP: x (D) -> relu(W1 x + b1) (H) -> relu(W2 . + b2) (H) -> W3 . + b3 (K) -> softmax.  Weights are kept
-// as FP32 masters (Adam state) and used in the matrix products as BF16 (round-to-nearest-even); every
-// product accumulates in FP32 on v_mfma_f32_16x16x4f32.  The loss is the cross-entropy against the
-// converged Q of the plain method, with the clusters of each batch relabelled by the permutation that
-// minimises it (cluster labels of an admixture fit are arbitrary).
+// as FP32 masters (Adam state) and used in the matrix products as BF16 (round-to-nearest-even); the products
+// run on v_mfma_f32_16x16x32_bf16 with FP32 accumulation (FP32 operands as three BF16 terms, k_gemm).  The loss
+// is the cross-entropy against the converged Q of the plain method, with the clusters of each batch relabelled
+// by the permutation that minimises it (cluster labels of an admixture fit are arbitrary).
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <math.h>

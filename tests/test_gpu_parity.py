@@ -68,6 +68,21 @@ def test_mixed_passes_match_oracle(I, J, K, oracle):
         assert rel(XtX, oX) <= 2e-5 and rel(Xtz, oz) <= 2e-5
 
 
+@pytest.mark.parametrize("K", list(range(1, 17)))
+def test_mixed_loglik_rows_every_k(K, oracle):
+    """The MIXED log-likelihood pass (X = P^T Q from three BF16 terms of each FP32 parameter on the matrix
+    cores, ceil(6K / 16) instructions per set) per individual against the oracle at every compiled K: FP32
+    per-pair products with an FP64 product and sum keep each row within 1e-6 relative.  J = 37 leaves a
+    partial last genotype word, I = 131 padding individuals in the last workgroup."""
+    I, J = 131, 37
+    s, q, p = case(I, J, K)
+    with _eng(s, q, p, "mixed") as e:
+        ll, rows = e.loglikelihood(rows=True)
+    orows = oracle.loglik_rows(s.bed, I, J, q, p)
+    assert rel(rows, orows) <= 1e-6
+    assert abs(ll - orows.sum()) <= 1e-6 * abs(ll)
+
+
 def test_fp32_accumulation_mode(oracle):
     """The reference's T2 = Float32 semantics (HK_PREC_FP32): FP32 products and FP32 gradient / curvature
     sums, each over the whole i or j range.  Its passes stay within FP32-accumulation error of the FP64
