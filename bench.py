@@ -443,8 +443,9 @@ def main():
                      "kernel": name, "avg_launch_ms": round(avg_ms, 3), "launches": nl,
                      "flops_per_launch": fl,
                      "timing": f"HIP events on the engine stream, {nprof} profiled fits after the timed ones",
-                     "note": ("compute-bound: v_mfma_f32_16x16x4f32 plus FP32 VALU (they do not overlap on "
-                              "gfx950); MI355X's FP32 peak is 157.3 TFLOP/s for matrix and vector alike"
+                     "note": ("compute-bound (instruction issue): v_mfma_f32_16x16x4f32 plus FP32 VALU, which "
+                              "gfx950 overlaps little; MI355X's FP32 peak is 157.3 TFLOP/s for matrix and vector "
+                              "alike"
                               if prec == "mixed" else
                               "compute-bound: v_mfma_f64_16x16x4f64 plus FP64 VALU; FP64 peak 78.6 TFLOP/s")},
     }

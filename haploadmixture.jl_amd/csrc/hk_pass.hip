@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void k_pside_grad(const uint32_t* __restrict__
 // with v_h = Y_h / (L X_h) = c_h / (2 qp_h^2) (pair_v; the factor 2 is in the tables).  One lane
 // computes one (i, j) pair with all four haplotype weights (no redundant per-pair work);
 // v_mfma_{f32,f64}_16x16x4 takes A = v_h (16 rows x 4), B = QQ or PP (4 x 16 columns): one MFMA per
-// haplotype and column tile.  gfx950 does not overlap FP32 MFMA and VALU issue, and one
+// haplotype and column tile.  gfx950 overlaps FP32 MFMA and VALU issue little, and one
 // v_mfma_f32_16x16x4f32 holds the SIMD as long as ~12 FMAs, so a short tail of packed columns (<= 6,
 // e.g. 16..20 for K = 6) is accumulated with VALU FMAs instead of a mostly empty extra tile (CurvShape).
 //   A operand: lane l holds A[l % 16][l / 16];  B operand: lane l holds B[l / 16][l % 16];
