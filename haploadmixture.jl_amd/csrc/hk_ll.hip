@@ -22,8 +22,9 @@ namespace hk {
 // so NM = ceil(6K / 16) instructions per set (K = 6: 3, against ceil(K / 2) = 3 of the FP32 32 x 32 x 2
 // instruction at twice its cycles; K = 8: 3 against 4).  Operand layout: lane l holds A[l % 32][slots of half
 // l / 32] and B[slots of half l / 32][l % 32] (the same slots for both, whatever their order inside the
-// instruction).  The A operands (p terms, per word) come pre-split from k_ll_split, NM contiguous 16-byte
-// loads per lane and set, loaded one word ahead; the B operands (q terms) are split once per wave.
+// instruction).  The A operands (p terms, per word) come pre-split from k_ll_split, NM 16-byte loads per lane and
+// set (one contiguous kilobyte per wave and instruction), loaded one word ahead; the B operands (q terms) are
+// split once per wave.
 // Result register v of lane l holds D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32], i.e. X_h (h = v % 4) of
 // individual l % 32 at block 2 (v / 4) + l / 32 -- four pairs per lane, all four haplotypes of each in the
 // lane.  The per-pair likelihood (pair_lik) then runs on the VALU; the FP64 running product is renormalised
@@ -38,8 +39,9 @@ __host__ __device__ constexpr int ll_term_q(int u) { return u == 1 || u == 4 ? 1
 
 size_t ll_stage_bytes(int K, int64_t J8) { return (size_t)J8 * 64 * (size_t)ll_nm(K) * 16; }
 
-// A operands of k_ll_mm: for set blockIdx.y, stage[(((jw 32 + m) 2 + hf) NM + t) 8 + e] = BF16 term ll_term_p(u)
-// of p[k + K (32 jw + m)] with v = 16 t + 8 hf + e, k = v / 6, u = v % 6 (0 for v >= 6K).  One thread per 8 terms.
+// A operands of k_ll_mm: for set blockIdx.y, stage[((jw NM + t) 64 + 32 hf + m) 8 + e] = BF16 term ll_term_p(u)
+// of p[k + K (32 jw + m)] with v = 16 t + 8 hf + e, k = v / 6, u = v % 6 (0 for v >= 6K): the wave's 16-byte
+// operands of instruction t are one contiguous kilobyte.  One thread per 8 terms.
 template <int K>
 __global__ void k_ll_split(const float* __restrict__ p1, const float* __restrict__ p2, int64_t J8,
                            uint16_t* __restrict__ stage, int64_t sstride, const int* __restrict__ gate) {
@@ -49,10 +51,10 @@ __global__ void k_ll_split(const float* __restrict__ p1, const float* __restrict
   uint16_t* out = stage + blockIdx.y * sstride;
   const int64_t n = J8 * 64 * NM;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
-    const int t = (int)(x % NM);
-    const int64_t r = x / NM;  // (32 jw + m) 2 + hf
-    const int hf = (int)(r & 1);
-    const float* pr = p + (int64_t)K * (r >> 1);
+    const int ln = (int)(x & 63);  // lane 32 hf + m
+    const int64_t jt = x >> 6;     // jw NM + t
+    const int t = (int)(jt % NM), hf = ln >> 5;
+    const float* pr = p + (int64_t)K * (32 * (jt / NM) + (ln & 31));
     uint32_t w[4];
 #pragma unroll
     for (int e2 = 0; e2 < 4; ++e2) {
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(256) void k_ll_mm(const uint32_t* __restrict__ gq, 
   const uint4* sa[NSET];
 #pragma unroll
   for (int st = 0; st < NSET; ++st)
-    sa[st] = reinterpret_cast<const uint4*>(stage + st * sstride) + ((int64_t)(32 * jw0 + col) * 2 + half) * NM;
+    sa[st] = reinterpret_cast<const uint4*>(stage + st * sstride) + (int64_t)jw0 * NM * 64 + lane;
   const uint32_t* gw = gq + (int64_t)jw0 * Ip + i;
   auto fetch = [&](uint4 (&a)[NSET][NM], uint32_t& w) {
     w = *gw;
@@ -138,7 +140,7 @@ __global__ __launch_bounds__(256) void k_ll_mm(const uint32_t* __restrict__ gq, 
 #pragma unroll
     for (int st = 0; st < NSET; ++st) {
 #pragma unroll
-      for (int t = 0; t < NM; ++t) a[st][t] = sa[st][t];
+      for (int t = 0; t < NM; ++t) a[st][t] = sa[st][64 * t];
       sa[st] += 64 * NM;
     }
   };
