@@ -36,6 +36,12 @@ inline bool has_alt_shape(int K, int prec, int side) {
 }
 
 int nacc(int K, int mode);
+// Workgroups of 256 for a loop over n elements, capped at 2^22 (2^30 work-items): a dispatch packet holds the grid
+// size in work-items as a 32-bit count, so element loops that can pass 2^32 stride over a capped grid instead.
+inline unsigned grid_1d(int64_t n) {
+  const int64_t nb = (n + 255) / 256;
+  return (unsigned)(nb < 1 ? 1 : (nb < (1 << 22) ? nb : (1 << 22)));
+}
 // Element counts of the curvature passes' staging tables (P side: nq, Q side: np).
 void stage_sizes(int K, int64_t Ip, int64_t Jp, int64_t* nq, int64_t* np);
 

@@ -124,27 +124,28 @@ template <int K, typename TP>
 __global__ void k_stage_rows(const TP* __restrict__ x, int64_t ncol, int64_t nrow, int sub, int QS, int RG,
                              TP* __restrict__ out, const int* __restrict__ gate) {
   if (gate_off(gate)) return;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nrow * RG) return;
-  const int64_t c = idx / RG;
-  const int e = (int)(idx - c * RG);
-  const int64_t last = ncol / sub - 1;
-  const int64_t cc = c < last ? c : last;
-  TP v = (TP)0;
-  if (e < sub * QS) {
-    const int sgrp = e / QS, n = e - sgrp * QS;
-    if (n < K * (K + 1) / 2) {
-      int k = 0;
-      while ((k + 1) * (k + 2) / 2 <= n) ++k;
-      const int k2 = n - k * (k + 1) / 2;
-      const TP* xc = x + (int64_t)K * (sub * cc + sgrp);
-      v = (TP)2 * (xc[k] * xc[k2]);
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nrow * RG;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = idx / RG;
+    const int e = (int)(idx - c * RG);
+    const int64_t last = ncol / sub - 1;
+    const int64_t cc = c < last ? c : last;
+    TP v = (TP)0;
+    if (e < sub * QS) {
+      const int sgrp = e / QS, n = e - sgrp * QS;
+      if (n < K * (K + 1) / 2) {
+        int k = 0;
+        while ((k + 1) * (k + 2) / 2 <= n) ++k;
+        const int k2 = n - k * (k + 1) / 2;
+        const TP* xc = x + (int64_t)K * (sub * cc + sgrp);
+        v = (TP)2 * (xc[k] * xc[k2]);
+      }
+    } else if (e < sub * (QS + K)) {
+      const int t = e - sub * QS, sgrp = t / K, k = t - sgrp * K;
+      v = x[k + (int64_t)K * (sub * cc + sgrp)];
     }
-  } else if (e < sub * (QS + K)) {
-    const int t = e - sub * QS, sgrp = t / K, k = t - sgrp * K;
-    v = x[k + (int64_t)K * (sub * cc + sgrp)];
+    out[idx] = v;
   }
-  out[idx] = v;
 }
 
 }  // namespace hk

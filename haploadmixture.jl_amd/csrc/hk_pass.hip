@@ -45,24 +45,27 @@ namespace hk {
 // Repacking and counting
 // -----------------------------------------------------------------------------------------------
 
+// Element loops over sizes that can pass 2^32 (the genotype streams hold I J / 8 words: I = 80,000 x J = 540,000
+// blocks is 2^32) stride over a capped grid (grid_1d): a launch of more than 2^32 work-items would be truncated.
 __global__ void k_repack_q(const uint8_t* __restrict__ raw, int64_t bps, int64_t I, int64_t J, int64_t Ip,
                            int64_t J8, uint32_t* __restrict__ gq) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= J8 * Ip) return;
-  const int64_t jw = idx / Ip, i = idx - jw * Ip;
-  uint32_t w = 0;
-  for (int n = 0; n < 8; ++n) {
-    const int64_t j = jw * 8 + n;
-    uint32_t nib = HK_NIB_DM;
-    if (i < I && j < J) {
-      const unsigned sh = (unsigned)(i & 3) * 2u;
-      const unsigned c1 = (raw[(2 * j) * bps + (i >> 2)] >> sh) & 3u;
-      const unsigned c2 = (raw[(2 * j + 1) * bps + (i >> 2)] >> sh) & 3u;
-      nib = c1 | (c2 << 2);
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < J8 * Ip;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t jw = idx / Ip, i = idx - jw * Ip;
+    uint32_t w = 0;
+    for (int n = 0; n < 8; ++n) {
+      const int64_t j = jw * 8 + n;
+      uint32_t nib = HK_NIB_DM;
+      if (i < I && j < J) {
+        const unsigned sh = (unsigned)(i & 3) * 2u;
+        const unsigned c1 = (raw[(2 * j) * bps + (i >> 2)] >> sh) & 3u;
+        const unsigned c2 = (raw[(2 * j + 1) * bps + (i >> 2)] >> sh) & 3u;
+        nib = c1 | (c2 << 2);
+      }
+      w |= nib << (4 * n);
     }
-    w |= nib << (4 * n);
+    gq[idx] = w;
   }
-  gq[idx] = w;
 }
 
 // Block-major stream from the individual-major one: an 8 x 8 nibble transpose per thread.  Thread (iw, jw)
@@ -70,18 +73,19 @@ __global__ void k_repack_q(const uint8_t* __restrict__ raw, int64_t bps, int64_t
 // threads take consecutive jw, so the 32-byte writes are contiguous.  Padding nibbles carry over.
 __global__ void k_repack_p(const uint32_t* __restrict__ gq, int64_t Ip, int64_t J8, int64_t Jp, int64_t I8,
                            uint32_t* __restrict__ gp) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= I8 * J8) return;
-  const int64_t iw = idx / J8, jw = idx - iw * J8;
-  uint32_t in[8];
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < I8 * J8;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t iw = idx / J8, jw = idx - iw * J8;
+    uint32_t in[8];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) in[t] = gq[jw * Ip + 8 * iw + t];
+    for (int t = 0; t < 8; ++t) in[t] = gq[jw * Ip + 8 * iw + t];
 #pragma unroll
-  for (int n = 0; n < 8; ++n) {
-    uint32_t w = 0;
+    for (int n = 0; n < 8; ++n) {
+      uint32_t w = 0;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) w |= ((in[t] >> (4 * n)) & 15u) << (4 * t);
-    gp[iw * Jp + 8 * jw + n] = w;
+      for (int t = 0; t < 8; ++t) w |= ((in[t] >> (4 * n)) & 15u) << (4 * t);
+      gp[iw * Jp + 8 * jw + n] = w;
+    }
   }
 }
 
@@ -127,13 +131,14 @@ __global__ void k_count_dm_raw(const uint8_t* __restrict__ raw, int64_t bps, int
 // Decoded genotype readback (bit-exact decode check): g[s * I + i] = BED code of SNP s.
 __global__ void k_decode(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int64_t J,
                          uint8_t* __restrict__ codes) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= I * J) return;
-  const int64_t j = idx / I, i = idx - j * I;
-  const uint32_t w = gq[(j >> 3) * Ip + i];
-  const unsigned nib = (w >> (4 * (j & 7))) & 15u;
-  codes[(2 * j) * I + i] = (uint8_t)(nib & 3u);
-  codes[(2 * j + 1) * I + i] = (uint8_t)(nib >> 2);
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < I * J;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = idx / I, i = idx - j * I;
+    const uint32_t w = gq[(j >> 3) * Ip + i];
+    const unsigned nib = (w >> (4 * (j & 7))) & 15u;
+    codes[(2 * j) * I + i] = (uint8_t)(nib & 3u);
+    codes[(2 * j + 1) * I + i] = (uint8_t)(nib >> 2);
+  }
 }
 
 // -----------------------------------------------------------------------------------------------
@@ -903,16 +908,17 @@ __global__ void k_finish_p(int64_t J, const double* __restrict__ p, double* __re
 __global__ void k_reduce_p(const double* __restrict__ part, int nchunk, int NT, int64_t Jp, int64_t J,
                            double* __restrict__ res, int acc32, const int* __restrict__ gate) {
   if (gate_off(gate)) return;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= 4 * J * NT) return;
-  if (acc32) {
-    float s = 0.f;
-    for (int c = 0; c < nchunk; ++c) s += (float)part[(int64_t)c * 4 * Jp * NT + idx];
-    res[idx] = s;
-  } else {
-    double s = 0.0;
-    for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * 4 * Jp * NT + idx];
-    res[idx] = s;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < 4 * J * NT;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    if (acc32) {
+      float s = 0.f;
+      for (int c = 0; c < nchunk; ++c) s += (float)part[(int64_t)c * 4 * Jp * NT + idx];
+      res[idx] = s;
+    } else {
+      double s = 0.0;
+      for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * 4 * Jp * NT + idx];
+      res[idx] = s;
+    }
   }
 }
 
@@ -963,11 +969,12 @@ __global__ void k_expand(const double* __restrict__ res, int K, int NT, int64_t 
 // EM sums in the reference's form (em_loop!, loops.jl:286-291): out[k, c] = par[k, c] * grad[c][k].
 __global__ void k_em_expand(const double* __restrict__ res, int K, int NT, int64_t n,
                             const double* __restrict__ par, double* __restrict__ out) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)K * n) return;
-  const int64_t c = idx / K;
-  const int k = (int)(idx - c * K);
-  out[idx] = par[idx] * res[c * NT + k];
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)K * n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = idx / K;
+    const int k = (int)(idx - c * K);
+    out[idx] = par[idx] * res[c * NT + k];
+  }
 }
 
 // -----------------------------------------------------------------------------------------------
@@ -1045,11 +1052,11 @@ static hipError_t tables_k(int side, const PassGeom& g, const void* x, hipStream
     using S = CurvShape<K>;
     if (side == 0) {
       const int64_t nrow = g.Jp + S::PAD_ROWS;
-      hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGQ + 255) / 256)), dim3(256), 0, s,
+      hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3(grid_1d(nrow * S::RGQ)), dim3(256), 0, s,
                          (const TP*)x, 4 * g.Jp, nrow, 4, S::QS, S::RGQ, (TP*)g.pp, g.gate);
     } else {
       const int64_t nrow = g.Ip + S::PAD_ROWS;
-      hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGP + 255) / 256)), dim3(256), 0, s,
+      hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3(grid_1d(nrow * S::RGP)), dim3(256), 0, s,
                          (const TP*)x, g.I, nrow, 1, S::QS, S::RGP, (TP*)g.qq, g.gate);
     }
     return hipGetLastError();
@@ -1121,10 +1128,8 @@ int nacc(int K, int mode) { return mode == MODE_LL ? 1 : (mode == MODE_GRAD ? K 
 
 hipError_t launch_repack(const uint8_t* raw, int64_t bps, const PassGeom& g, hipStream_t s) {
   int64_t nq = g.J8 * g.Ip, nt = g.I8 * g.J8;
-  hipLaunchKernelGGL(k_repack_q, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, raw, bps, g.I, g.J, g.Ip,
-                     g.J8, (uint32_t*)g.gq);
-  hipLaunchKernelGGL(k_repack_p, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, g.gq, g.Ip, g.J8, g.Jp, g.I8,
-                     (uint32_t*)g.gp);
+  hipLaunchKernelGGL(k_repack_q, dim3(grid_1d(nq)), dim3(256), 0, s, raw, bps, g.I, g.J, g.Ip, g.J8, (uint32_t*)g.gq);
+  hipLaunchKernelGGL(k_repack_p, dim3(grid_1d(nt)), dim3(256), 0, s, g.gq, g.Ip, g.J8, g.Jp, g.I8, (uint32_t*)g.gp);
   return hipGetLastError();
 }
 
@@ -1150,7 +1155,7 @@ hipError_t launch_count_dm_raw(const uint8_t* raw, int64_t bps, int64_t I, int64
 
 hipError_t launch_decode(const PassGeom& g, uint8_t* codes, hipStream_t s) {
   int64_t n = g.I * g.J;
-  hipLaunchKernelGGL(k_decode, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g.gq, g.Ip, g.I, g.J, codes);
+  hipLaunchKernelGGL(k_decode, dim3(grid_1d(n)), dim3(256), 0, s, g.gq, g.Ip, g.I, g.J, codes);
   return hipGetLastError();
 }
 
@@ -1193,7 +1198,7 @@ hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* re
 
 hipError_t launch_reduce_p(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s) {
   int64_t n = 4 * g.J * NT;
-  hipLaunchKernelGGL(k_reduce_p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nchunk, NT, g.Jp, g.J, res,
+  hipLaunchKernelGGL(k_reduce_p, dim3(grid_1d(n)), dim3(256), 0, s, part, nchunk, NT, g.Jp, g.J, res,
                      g.acc32, g.gate);
   return hipGetLastError();
 }
@@ -1217,7 +1222,7 @@ hipError_t launch_expand(const double* res, int K, int NT, int64_t n, double* Xt
 hipError_t launch_em_expand(const double* res, int K, int NT, int64_t n, const double* par, double* out,
                             hipStream_t s) {
   int64_t m = (int64_t)K * n;
-  hipLaunchKernelGGL(k_em_expand, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, res, K, NT, n, par, out);
+  hipLaunchKernelGGL(k_em_expand, dim3(grid_1d(m)), dim3(256), 0, s, res, K, NT, n, par, out);
   return hipGetLastError();
 }
 

@@ -563,14 +563,13 @@ __global__ void k_em_p(int K, int64_t J, const double* __restrict__ res, int NT,
 __global__ void k_pair_to_f32(const double* __restrict__ q, int64_t nq, const double* __restrict__ p, int64_t np,
                               float* __restrict__ dst, const int* __restrict__ gate) {
   if (gate_off(gate)) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nq) dst[i] = (float)q[i];
-  else if (i < nq + np) dst[i] = (float)p[i - nq];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq + np; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = i < nq ? (float)q[i] : (float)p[i - nq];
 }
 
 __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 // Plain streaming copy (16 B per lane per iteration): HBM bandwidth and counter calibration.
@@ -933,13 +932,13 @@ hipError_t launch_em_p(int K, int64_t J, const double* res_p, int NT, const doub
 
 hipError_t launch_pair_to_f32(const double* q, int64_t nq, const double* p, int64_t np, float* dst, hipStream_t s,
                               const int* gate) {
-  hipLaunchKernelGGL(k_pair_to_f32, dim3((unsigned)((nq + np + 255) / 256)), dim3(256), 0, s, q, nq, p, np, dst, gate);
+  hipLaunchKernelGGL(k_pair_to_f32, dim3(grid_1d(nq + np)), dim3(256), 0, s, q, nq, p, np, dst, gate);
   return hipGetLastError();
 }
 
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  hipLaunchKernelGGL(k_fill, dim3(grid_1d(n)), dim3(256), 0, s, p, n, v);
   return hipGetLastError();
 }
 

@@ -303,19 +303,20 @@ __global__ __launch_bounds__(256) void k_pside_mmw(const uint32_t* __restrict__ 
 __global__ void k_finish_pw(int K, int64_t J, const double* __restrict__ p, double* __restrict__ res,
                             const int* __restrict__ gate) {
   if (gate_off(gate)) return;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= 4 * J * K) return;
-  const int64_t jh = idx / K;
-  const int k = (int)(idx - jh * K);
-  const int NT = K + K * (K + 1) / 2;
-  const double* r = res + jh * NT + K;
-  const double* pv = p + (int64_t)K * jh;
-  double s = 0.0;
-  for (int k2 = 0; k2 < K; ++k2) {
-    const int hi = k > k2 ? k : k2, lo = k > k2 ? k2 : k;
-    s += r[hi * (hi + 1) / 2 + lo] * pv[k2];
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < 4 * J * K;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t jh = idx / K;
+    const int k = (int)(idx - jh * K);
+    const int NT = K + K * (K + 1) / 2;
+    const double* r = res + jh * NT + K;
+    const double* pv = p + (int64_t)K * jh;
+    double s = 0.0;
+    for (int k2 = 0; k2 < K; ++k2) {
+      const int hi = k > k2 ? k : k2, lo = k > k2 ? k2 : k;
+      s += r[hi * (hi + 1) / 2 + lo] * pv[k2];
+    }
+    res[jh * NT + k] = s;  // the slots written here are read by no other thread
   }
-  res[jh * NT + k] = s;  // the slots written here are read by no other thread
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -331,11 +332,11 @@ static hipError_t wide_tables_k(int side, const PassGeom& g, const void* x, hipS
   using S = WideShape<K>;
   if (side == 0) {
     const int64_t nrow = g.Jp + S::PAD_ROWS;
-    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGQ + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3(grid_1d(nrow * S::RGQ)), dim3(256), 0, s,
                        (const TP*)x, 4 * g.Jp, nrow, 4, S::QS, S::RGQ, (TP*)g.pp, g.gate);
   } else {
     const int64_t nrow = g.Ip + S::PAD_ROWS;
-    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3((unsigned)((nrow * S::RGP + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL((k_stage_rows<K, TP>), dim3(grid_1d(nrow * S::RGP)), dim3(256), 0, s,
                        (const TP*)x, g.I, nrow, 1, S::QS, S::RGP, (TP*)g.qq, g.gate);
   }
   return hipGetLastError();
@@ -397,7 +398,7 @@ hipError_t launch_pside_wide(int K, int prec, int mode, const PassGeom& g, const
 
 hipError_t launch_finish_p_wide(int K, const PassGeom& g, const double* p, double* res, hipStream_t s) {
   if (K < 9 || K > HK_MAXK) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_finish_pw, dim3((unsigned)((4 * g.J * K + 255) / 256)), dim3(256), 0, s, K, g.J, p, res, g.gate);
+  hipLaunchKernelGGL(k_finish_pw, dim3(grid_1d(4 * g.J * K)), dim3(256), 0, s, K, g.J, p, res, g.gate);
   return hipGetLastError();
 }
 

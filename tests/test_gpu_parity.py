@@ -367,3 +367,57 @@ def test_full_size_invariants():
         assert np.max(np.abs(pn.reshape(K, J, 4).sum(axis=(0, 2)) - 2 * (I - dmb)) / (2 * I)) <= 1e-5
         ll = e.loglikelihood()
         assert math.isfinite(ll) and ll < 0
+
+
+@pytest.mark.slow
+def test_full_size_mixed_matches_fp64_engine():
+    """At the mixed-precision workload shape (I=20k, J=200k, K=6), where the oracle would take hours, the MIXED
+    passes against the FP64 engine (itself within 1e-12 of the oracle up to I=5k, J=50k) on the same
+    parameters, element by element: log-likelihood rows and total, update_q and update_p curvature and
+    gradients within 2e-5 relative (FP32 per-pair products, FP64 accumulation over 200,000 blocks or 20,000
+    individuals)."""
+    I, J, K = 20000, 200000, 6
+    print("full-size mixed vs fp64", flush=True)
+    s, q, p = case(I, J, K, fst=0.05, miss=0.02, seed=5)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        with _eng(s, q, p, prec) as e:
+            ll, rows = e.loglikelihood(rows=True)
+            uq = e.update_q()
+            up = e.update_p()
+        out[prec] = (ll, rows, uq, up)
+    (l64, r64, q64, p64), (l32, r32, q32, p32) = out["fp64"], out["mixed"]
+    assert abs(l32 - l64) <= 1e-8 * abs(l64)
+    assert rel(r32, r64) <= 2e-6
+    assert rel(q32[0], q64[0]) <= 2e-5 and rel(q32[1], q64[1]) <= 2e-5
+    assert rel(p32[0], p64[0]) <= 2e-5 and rel(p32[1], p64[1]) <= 2e-5
+
+
+@pytest.mark.slow
+def test_genotype_streams_beyond_2_32_words():
+    """I = 70,000 x J = 500,000 blocks: each packed stream holds J/8 x Ip = 4.38e9 words, past the 2^32
+    work-items a single dispatch can name, so the repack kernels must stride over a capped grid.  The BED is a
+    pattern with closed-form results (every genotype homozygous allele 1, the last 8 blocks missing for everyone),
+    so every word of both streams is checked through the passes: double-missing counts (Q-side stream) = 8;
+    with q = 1/K and p = 1/4 every observed pair has likelihood 1/16 and X_0 = 1/4, so the log-likelihood is
+    I (J - 8) log(1/16) and the P-side gradient (block-major stream) is Xtz_p[k, 4j] = 2 / X_0 / K * I for the
+    observed blocks, 0 for every other slot (FP64 engine, exact)."""
+    I, J, K = 70000, 500000, 2
+    bps = (I + 3) // 4
+    print(f"streams beyond 2^32 words: I={I} J={J}", flush=True)
+    bed = np.zeros(2 * J * bps, dtype=np.uint8)  # BED code 00: homozygous allele 1
+    bed[2 * (J - 8) * bps:] = 0x55               # code 01 (missing) for the last 16 SNPs
+    q = np.full((K, I), 1.0 / K)
+    p = np.full((K, 4 * J), 0.25)
+    with HipAdmixData(bed, I, J, K, precision="fp64") as e:
+        del bed
+        e.set_params(q, p)
+        dm = e.double_missing()
+        assert np.all(dm == 8)
+        ll = e.loglikelihood()
+        ref = I * (J - 8) * math.log(1.0 / 16.0)
+        assert abs(ll - ref) <= 1e-12 * abs(ref), (ll, ref)
+        _, Xtz = e.update_p()
+    want = np.zeros((K, J, 4))
+    want[:, : J - 8, 0] = 2.0 / 0.25 / K * I
+    assert np.array_equal(Xtz.reshape(K, J, 4), want)
