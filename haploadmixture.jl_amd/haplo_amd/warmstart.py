@@ -253,30 +253,30 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def train_default(out_path=None, n_datasets=200, epochs=10, log=_say):
-    """Regenerates the committed weights: 200 datasets (I=2000, J=20,000, K=5, Fst ~ U[0.02, 0.15]) fitted by
-    the plain method (batched, MIXED, rtol 1e-6), then `epochs` passes of Adam over their (features,
-    converged Q) rows.  Writes the weights and, next to them, a JSON record of the run (settings, the loss
-    per epoch and the device time of each stage)."""
+def train_default(out_path=None, K=5, n_datasets=200, epochs=10, log=_say):
+    """Regenerates the committed weights for K clusters: 200 datasets of the world-seed population structure
+    (I=2000, J=20,000, Fst ~ U[0.02, 0.15]) fitted by the plain method (batched, MIXED, rtol 1e-6), then
+    `epochs` passes of Adam over their (features, converged Q) rows.  Writes the weights and, next to them, a
+    JSON record of the run (settings, the loss per epoch and the device time of each stage)."""
     import json
 
     t0 = time.perf_counter()
-    ts = make_training_set(n_datasets=n_datasets, log=log)
+    ts = make_training_set(n_datasets=n_datasets, K=K, log=log)
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
-    mlp, hist = train(ts, 5, epochs=epochs, log=log)
+    mlp, hist = train(ts, K, epochs=epochs, log=log)
     t_train = time.perf_counter() - t0
-    out_path = out_path or default_weights_path()
+    out_path = out_path or default_weights_path(K)
     os.makedirs(os.path.dirname(os.path.abspath(out_path)), exist_ok=True)
     mlp.save(out_path)
     record = {
-        "weights": os.path.basename(out_path), "world_seed": WORLD_SEED, "K": 5,
+        "weights": os.path.basename(out_path), "world_seed": WORLD_SEED, "K": K,
         "datasets": {"n": n_datasets, "I": 2000, "J": 20000, "fst": [min(ts.fsts), max(ts.fsts)],
                      "fst_draw": "U[0.02, 0.15], numpy default_rng(1000)", "missing": 0.01,
                      "data_seeds": [1000, 1000 + n_datasets - 1]},
         "targets": "converged Q of the plain method (hk_fit_batch, MIXED, rtol 1e-6, at most 500 iterations)",
         "features": f"seeded Gaussian projection of the first {PANEL_BLOCKS} SNP pairs to {D_FEAT} (seed {FEATURE_SEED})",
-        "network": f"{D_FEAT}-{WIDTH}-{WIDTH}-5, ReLU, softmax; BF16 weights, FP32 accumulation",
+        "network": f"{D_FEAT}-{WIDTH}-{WIDTH}-{K}, ReLU, softmax; BF16 weights, FP32 accumulation",
         "training": {"epochs": epochs, "batch": 500, "optimizer": "Adam lr 1e-3 betas (0.9, 0.999) eps 1e-8",
                      "loss": "cross-entropy against the best label permutation per dataset batch", "seed": 3},
         "loss_per_epoch": [round(float(v), 6) for v in hist],
@@ -291,7 +291,14 @@ def train_default(out_path=None, n_datasets=200, epochs=10, log=_say):
 
 
 if __name__ == "__main__":
-    import sys
+    # python -m haplo_amd.warmstart [--out DIR] [K ...]: train the network of each K (default 5)
+    import argparse
 
-    path, hist, ts = train_default(out_path=sys.argv[1] if len(sys.argv) > 1 else None)
-    print(f"wrote {path}; final loss {hist[-1]:.5f}; fits {ts.fit_seconds:.1f}s")
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=None, help="directory for the weights and records (default: the package data)")
+    ap.add_argument("K", nargs="*", type=int, default=[5])
+    a = ap.parse_args()
+    for k in a.K:
+        dst = os.path.join(a.out, os.path.basename(default_weights_path(k))) if a.out else None
+        path, hist, ts = train_default(out_path=dst, K=k)
+        print(f"wrote {path}; final loss {hist[-1]:.5f}; fits {ts.fit_seconds:.1f}s", flush=True)

@@ -144,18 +144,21 @@ def test_adam_matches_torch_optimizer():
         assert np.max(np.abs(t.detach().numpy() - w)) <= 1e-15 * max(1.0, np.max(np.abs(w)))
 
 
-def test_shipped_network_has_its_training_record():
-    """The shipped K=5 weights come with the record of the run that produced them (haplo_amd.warmstart.
-    train_default): 200 plain-method fits, the loss per epoch, the device time of each stage."""
+@pytest.mark.parametrize("K", range(2, 9))
+def test_shipped_network_has_its_training_record(K):
+    """A network ships for every K from 2 to 8, each with the record of the run that produced it
+    (haplo_amd.warmstart.train_default): 200 plain-method fits, the loss per epoch, the device time of each
+    stage."""
     import json
     import os
 
     from haplo_amd import warmstart as ws
 
-    path = ws.default_weights_path(5)
+    path = ws.default_weights_path(K)
     rec = json.load(open(os.path.splitext(path)[0] + ".json"))
     z = np.load(path)
-    assert rec["weights"] == os.path.basename(path) and int(z["K"]) == rec["K"] == 5
+    assert rec["weights"] == os.path.basename(path) and int(z["K"]) == rec["K"] == K
+    assert rec["network"].startswith(f"256-512-512-{K},")
     assert rec["datasets"]["n"] == 200 and rec["rows"] == 200 * rec["datasets"]["I"]
     hist = rec["loss_per_epoch"]
     assert len(hist) == rec["training"]["epochs"] and hist[-1] < hist[0]
