@@ -155,7 +155,8 @@ def warmstart_bench(a, w, prec):
     """Plain fit (random start, init_em!, QN) against the warm-started fit (network Q0, EM on P with Q0
     held, QN) on held-out data: iterations and wall time until the log-likelihood reaches
     ll_plain - 1e-6 |ll_plain|, ll_plain = the plain method's converged value.  The main line is the
-    Fst=0.05 dataset; Fst=0.02 and 0.12 datasets are reported beside it."""
+    Fst=0.05 dataset; Fst=0.02 and 0.12 datasets are reported beside it, and each row also gives the
+    SKFR-label start (Q0 from sparse k-means labels, no network) for comparison."""
     import itertools
 
     from haplo_amd import warmstart as ws
@@ -179,12 +180,13 @@ def warmstart_bench(a, w, prec):
             # set-up, untimed like the context's: the network's projection scratch for this panel size
             # (~250 MB at I=10k) and the seeded projection matrix are made on its first prediction
             ws.predict_q(e, mlp)
-            plain_runs, warm_runs = [], []
+            plain_runs, warm_runs, skfr_runs = [], [], []
             for _ in range(a.steps):
                 e.set_params(q0, p0)
                 plain_runs.append(e.fit(rtol=a.rtol, n_iter=a.max_iter))
                 warm_runs.append(ws.fit_warm(e, mlp, rtol=a.rtol, n_iter=a.max_iter))
-            plain, warm = plain_runs[-1], warm_runs[-1]
+                skfr_runs.append(ws.fit_skfr_start(e, rtol=a.rtol, n_iter=a.max_iter))
+            plain, warm, sk = plain_runs[-1], warm_runs[-1], skfr_runs[-1]
             Q0 = ws.predict_q(e, mlp)
         ll = plain["ll"]
         ip, tp = ws.time_to_rel(plain["trace"], plain["t_trace"], ll)
@@ -192,15 +194,23 @@ def warmstart_bench(a, w, prec):
         tp_med = float(np.median([ws.time_to_rel(r["trace"], r["t_trace"], r["ll"])[1] for r in plain_runs]))
         tw_med = float(np.median([ws.time_to_rel(r["trace"], r["t_trace"], ll)[1] + r["predict_seconds"]
                                   for r in warm_runs]))
+        isk, _ = ws.time_to_rel(sk["trace"], sk["t_trace"], ll)
+        tsk = [ws.time_to_rel(r["trace"], r["t_trace"], ll)[1] for r in skfr_runs]
+        tsk_med = (None if any(t is None for t in tsk) else
+                   float(np.median([t + r["init_seconds"] for t, r in zip(tsk, skfr_runs)])))
         rmse = min(float(np.sqrt(np.mean((Q0[list(pm)] - sd.q_true) ** 2))) for pm in itertools.permutations(range(K)))
         row = {"fst": fst, "data_seed": seed, "plain_iters_to_target": ip, "plain_s_to_target": round(tp_med, 4),
                "plain_iters_total": plain["iters"], "plain_s_total": round(plain["seconds"], 4),
                "warm_iters_to_target": iw, "warm_s_to_target_incl_predict": round(tw_med, 4),
                "warm_iters_total": warm["iters"], "warm_s_total": round(warm["seconds"], 4),
                "predict_s": round(warm["predict_seconds"], 4), "em_p_iters": 10,
+               "skfr_start_iters_to_target": isk,
+               "skfr_start_s_to_target_incl_init": None if tsk_med is None else round(tsk_med, 4),
+               "skfr_start_ll_minus_plain": round(sk["ll"] - ll, 2),
                "ll_plain": ll, "ll_warm_minus_plain": round(warm["ll"] - ll, 2), "q0_rmse_vs_truth": round(rmse, 4)}
         log(f"Fst={fst}: plain {ip} it / {tp_med:.3f}s; warm {iw} it / {tw_med:.3f}s; "
-            f"ll warm-plain {warm['ll'] - ll:.1f}")
+            f"ll warm-plain {warm['ll'] - ll:.1f}; skfr start {isk} it / "
+            f"{'-' if tsk_med is None else format(tsk_med, '.3f')}s")
         rows.append(row)
     mlp.close()
     m = rows[0]

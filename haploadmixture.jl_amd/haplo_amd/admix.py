@@ -10,6 +10,7 @@ the MI355X.  Differences, each forced by the build:
 Addition: ``warm_start=True`` fills the ``init_q`` hook (driver.jl:56,184-185) with the learned
 initialiser (``warmstart``): Q0 from the shipped network for this K, then EM on P with Q0 held instead
 of ``init_em!``.  Without a network for K, or when ``init_q`` is given, the usual start runs.
+``warm_start="skfr"`` fills it from sparse k-means labels on the panel instead (``warmstart.skfr_q0``; any K).
 """
 from __future__ import annotations
 
@@ -76,14 +77,22 @@ def _admixture_base(filename, I, J, K, *, n_iter=1000, rtol=1e-7, rng=1, em_iter
         if engine_opts:
             base.update(engine_opts)
         ll_start = math.nan
-        mlp = warmstart.load_default(K) if (warm_start and init_q is None) else None
-        if warm_start and init_q is None and mlp is None:
+        if warm_start not in (False, True, "network", "skfr"):
+            raise ValueError("warm_start must be False, True / 'network' or 'skfr'")
+        use_net = warm_start in (True, "network") and init_q is None
+        mlp = warmstart.load_default(K) if use_net else None
+        if use_net and mlp is None:
             print(f"No warm-start network for K={K}; starting from random Q")
+        q_warm = None
         if mlp is not None:
-            # learned Q0, then 2 * em_iters EM updates of P with Q0 held (each costs one P pass and one
-            # log-likelihood pass, a third of a full EM update), in place of init_em!
-            eng.set_params(warmstart.predict_q(eng, mlp), d.p)
+            q_warm = warmstart.predict_q(eng, mlp)
             mlp.close()
+        elif warm_start == "skfr" and init_q is None:
+            q_warm = warmstart.skfr_q0(eng, seed=rng)
+        if q_warm is not None:
+            # learned or SKFR-label Q0, then 2 * em_iters EM updates of P with Q0 held (each costs one P pass
+            # and one log-likelihood pass, a third of a full EM update), in place of init_em!
+            eng.set_params(q_warm, d.p)
             r = eng.fit(eng.default_opts(**{**base, "em_iters": max(em_iters, 1) * 2, "em_fix_q": 1, "n_iter": 0,
                                             "fix_q": 0, "fix_p": 0}))
             ll_start = r["ll"]

@@ -7,6 +7,11 @@ its ancestry proportions.  It is trained with hand-written kernels (libhaplo_hip
 one population structure (``sim.simulate_world``), with a permutation-matched cross-entropy (fit labels
 are arbitrary).  The warm-started fit then runs EM on P with Q held at the prediction (``em_fix_q``)
 before the usual QN/SQP loop; random initialisation stays the fallback.
+
+A second, data-driven start needs no network and works for any K the library fits (``skfr_q0``,
+``fit_skfr_start``): Q0 from the cluster labels of sparse k-means with feature ranking on the panel's own
+genotypes (``hk_skfr``, the SKFR step of ``run_admixture(sparsity=...)``), weight ``SKFR_Q0_WEIGHT`` on an
+individual's cluster and the rest spread evenly over the others, then the same EM on P with Q0 held.
 """
 from __future__ import annotations
 
@@ -237,6 +242,38 @@ def fit_auto(engine: HipAdmixData, mlp: MLP | None = None, em_iters=10, **fit_kw
         return r
     r = fit_warm(engine, mlp, em_iters=em_iters, **fit_kw)
     r["warm_start"] = True
+    return r
+
+
+SKFR_Q0_WEIGHT = 0.8
+SKFR_MAX_BLOCKS = 5000
+
+
+def skfr_q0(engine: HipAdmixData, sparsity: int | None = None, weight: float = SKFR_Q0_WEIGHT, seed: int = 1,
+            max_iter: int = 20) -> np.ndarray:
+    """Q0 (K x I) from sparse k-means labels on the engine's genotypes (hk_skfr, K clusters, the top
+    ``sparsity`` blocks -- at most SKFR_MAX_BLOCKS by default): ``weight`` on an individual's cluster,
+    (1 - weight) / (K - 1) on each other one."""
+    K, I = engine.K, engine.I
+    if K == 1:
+        return np.ones((1, I), order="F")
+    s = min(engine.J, SKFR_MAX_BLOCKS) if sparsity is None else int(sparsity)
+    lab = engine.skfr(K, s, max_iter=max_iter, tries=1, seed=seed)["clusters"]
+    q0 = np.full((K, I), (1.0 - weight) / (K - 1))
+    q0[lab, np.arange(I)] = weight
+    return np.asfortranarray(q0)
+
+
+def fit_skfr_start(engine: HipAdmixData, p_seed=11, em_iters=10, **fit_kw):
+    """Fit from the SKFR-label start: Q0 = skfr_q0, P random, ``em_iters`` EM updates of P with Q0 held, then
+    QN/SQP on both.  ``init_seconds`` in the result is the sparse k-means time, on top of ``seconds``."""
+    t0 = time.perf_counter()
+    q0 = skfr_q0(engine)
+    _, p0 = init_params(engine.I, engine.J, engine.K, seed=p_seed)
+    engine.set_params(q0, p0)
+    t_init = time.perf_counter() - t0
+    r = engine.fit(em_iters=em_iters, em_fix_q=1, **fit_kw)
+    r["init_seconds"] = t_init
     return r
 
 

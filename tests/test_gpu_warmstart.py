@@ -240,6 +240,46 @@ def test_driver_warm_start_and_fallback(tmp_path, capsys):
     assert np.isfinite(fb.ll_new)
 
 
+@pytest.mark.parametrize("I,J,K,fst,alpha", [(3000, 30000, 6, 0.1, 0.1), (2000, 20000, 10, 0.05, 0.5)])
+def test_skfr_start_reaches_plain_optimum(I, J, K, fst, alpha):
+    """The SKFR-label start (no network; any K): Q0 weights 0.8 on each individual's sparse k-means cluster,
+    EM on P with Q0 held, then QN.  It ends at least as high as the plain random start (on panels with sharp
+    clusters the random start stops in a lower optimum) and Q0 is on the simplex."""
+    from haplo_amd.sim import simulate
+
+    s = simulate(I, J, K, fst=fst, miss=0.01, alpha_q=alpha, seed=21)
+    with HipAdmixData(s.bed, I, J, K, precision="mixed") as e:
+        q0, p0 = ws.init_params(I, J, K, seed=11)
+        e.set_params(q0, p0)
+        plain = e.fit(rtol=1e-7, n_iter=500)
+        Q0 = ws.skfr_q0(e)
+        assert Q0.shape == (K, I) and np.allclose(Q0.sum(axis=0), 1.0) and Q0.min() > 0
+        assert np.all(np.isclose(Q0.max(axis=0), ws.SKFR_Q0_WEIGHT))
+        sk = ws.fit_skfr_start(e, rtol=1e-7, n_iter=500)
+    print(f"K={K}: plain {plain['iters']} it ll={plain['ll']:.6f}; skfr start {sk['iters']} it ll={sk['ll']:.6f} "
+          f"(+{sk['init_seconds']:.3f}s)", flush=True)
+    assert sk["qp_failures"] == 0 and sk["qp_iter_limit"] == 0
+    assert sk["ll"] >= plain["ll"] - 1e-6 * abs(plain["ll"])
+
+
+def test_driver_skfr_start(tmp_path):
+    """run_admixture(warm_start="skfr") at K=9, where no network ships: the SKFR-label start fills init_q and
+    the fit ends near the plain driver's optimum."""
+    import haplo_amd
+    from haplo_amd import plink
+
+    I, J, K = 600, 2000, 9
+    sd = ws.simulate_world(I, J, K, 0.1, world_seed=ws.WORLD_SEED, data_seed=4711, truth=False)
+    f = str(tmp_path / "s.bed")
+    plink.write_bed(f, sd.bed, I, 2 * J)
+    plain, _, _ = haplo_amd.run_admixture(f, I, J, K, T2=np.float32)
+    sk, _, _ = haplo_amd.run_admixture(f, I, J, K, T2=np.float32, warm_start="skfr")
+    assert np.allclose(sk.q.sum(axis=0), 1.0) and np.isfinite(sk.ll_new)
+    assert sk.ll_new >= plain.ll_new - 1e-4 * abs(plain.ll_new)
+    with pytest.raises(ValueError):
+        haplo_amd.run_admixture(f, I, J, K, T2=np.float32, warm_start="sometimes")
+
+
 @pytest.mark.slow
 def test_training_from_scratch_learns_a_useful_start():
     """A 256-512-512-3 network trained from random weights by the library's own kernels (hk_mlp_grad /
