@@ -27,7 +27,7 @@ namespace hk {
 // split once per wave.
 // Result register v of lane l holds D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32], i.e. X_h (h = v % 4) of
 // individual l % 32 at block 2 (v / 4) + l / 32 -- four pairs per lane, all four haplotypes of each in the
-// lane.  The per-pair likelihood (pair_lik) then runs on the VALU; the FP64 running product is renormalised
+// lane.  The per-pair likelihood (lik_factored) then runs on the VALU; the FP64 running product is renormalised
 // once per word.  NSET = 2 evaluates two parameter sets (the QN iteration's ll_basic and ll_qn) over one
 // genotype load, nibble decode and class-table read per pair.
 // part: set s, chunk c, individual i at part[s * pstride + c * Ip + i] (lanes l and l + 32 combined).
@@ -87,8 +87,8 @@ __global__ __launch_bounds__(256) void k_ll_mm(const uint32_t* __restrict__ gq, 
   typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
   typedef unsigned short u8v __attribute__((ext_vector_type(8)));
   constexpr int NM = ll_nm(K);
-  __shared__ __attribute__((aligned(16))) float lut[128];
-  fill_pair_lut<float>(lut);
+  __shared__ __attribute__((aligned(16))) float lut[160];
+  fill_lik_lut<float>(lut);
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = lane & 31, half = lane >> 5;
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void k_ll_mm(const uint32_t* __restrict__ gq, 
 #pragma unroll
       for (int st = 0; st < NSET; ++st) {
         const float Xh[4] = {X[st][4 * m], X[st][4 * m + 1], X[st][4 * m + 2], X[st][4 * m + 3]};
-        llm[st] *= (double)pair_lik<float>(Xh, lut, nib);
+        llm[st] *= (double)lik_factored<float>(Xh, lut, nib);
       }
     }
 #pragma unroll

@@ -78,6 +78,50 @@ __device__ __forceinline__ TP pair_lik(const TP (&X)[4], const TP* __restrict__ 
   return hk_fma(b[3], g01, hk_fma(a[2], g11, hk_fma(a[0], g00, a[3])));
 }
 
+// Factored pair likelihood (the matrix-core likelihood pass): the L' of pair_lik in fewer operations.  With X
+// indexed 2a + b (a the allele of SNP 2j, b of SNP 2j+1), the compatibility matrix of a homozygous or missing SNP
+// code is u u' (u = (1,0), (0,1), or (1,1) for missing) and that of a heterozygous one e0 e1' + e1 e0', so
+// L' = h sum S1[a,a'] S2[b,b'] X_ab X_a'b' + e factors per nibble as
+//   L' = (alpha . X)(beta . X) + dh (X_00 X_11 + X_01 X_10) + e:
+// alpha = beta = u1 (x) u2 when neither SNP is heterozygous; alpha = e0 (x) u2, beta = e1 (x) u2 when SNP 2j is
+// (h = 1/2 cancels the 2 of the two cross terms), and alpha = u1 (x) e0, beta = u1 (x) e1 when SNP 2j+1 is;
+// dh = 1 (alpha = beta = 0) when both are; e = 1 (all else 0) for the double-missing nibble.  12 operations per
+// pair against 17 for the general form.  Table (160 entries): lut[4 nib + f] = alpha_f, lut[64 + 4 nib + f] =
+// beta_f, lut[128 + 2 nib] = (dh, e); threads 0..159 fill it.
+template <typename TP>
+__device__ __forceinline__ void fill_lik_lut(TP* lut) {
+  const int t = threadIdx.x;
+  if (t < 128) {
+    const int tab = t >> 6, nib = (t >> 2) & 15, f = t & 3;
+    const unsigned c1 = (unsigned)nib & 3u, c2 = (unsigned)nib >> 2;
+    const int a = f >> 1, b = f & 1;
+    // u_x of a homozygous or missing code (codes: 0 hom, 1 missing, 2 het, 3 hom alt)
+    auto u = [](unsigned c, int x) { return c == 1u ? 1 : (c == 0u ? (int)(x == 0) : (c == 3u ? (int)(x == 1) : 0)); };
+    int v;
+    if (nib == (int)HK_NIB_DM || (c1 == 2u && c2 == 2u)) v = 0;
+    else if (c1 == 2u) v = (int)(a == tab) * u(c2, b);
+    else if (c2 == 2u) v = u(c1, a) * (int)(b == tab);
+    else v = u(c1, a) * u(c2, b);
+    lut[t] = (TP)v;
+  } else if (t < 160) {
+    const int nib = (t - 128) >> 1;
+    lut[t] = (t & 1) ? (nib == (int)HK_NIB_DM ? (TP)1 : (TP)0) : (nib == 10 ? (TP)1 : (TP)0);
+  }
+}
+
+template <typename TP>
+__device__ __forceinline__ TP lik_factored(const TP (&X)[4], const TP* __restrict__ lut, unsigned nib) {
+  typedef TP T4 __attribute__((ext_vector_type(4)));
+  typedef TP T2 __attribute__((ext_vector_type(2)));
+  const T4 al = *(const T4*)(lut + 4 * nib);
+  const T4 be = *(const T4*)(lut + 64 + 4 * nib);
+  const T2 de = *(const T2*)(lut + 128 + 2 * nib);
+  const TP A = hk_fma(al[3], X[3], hk_fma(al[2], X[2], hk_fma(al[1], X[1], al[0] * X[0])));
+  const TP B = hk_fma(be[3], X[3], hk_fma(be[2], X[2], hk_fma(be[1], X[1], be[0] * X[0])));
+  const TP D = hk_fma(X[1], X[2], X[0] * X[3]);
+  return hk_fma(A, B, hk_fma(de[0], D, de[1]));
+}
+
 // Curvature weights v_h = Y_h / (L X_h) (the factor 2 of c_h / qp_h^2 is in the outer-product tables).
 // Two reciprocals per pair: v_0, v_1 share 1 / (L' X_0 X_1) and v_2, v_3 share 1 / (L' X_2 X_3) (in FP32
 // range for every X_h >= ~3e-10, far below the 1e-5 floor the projections keep).

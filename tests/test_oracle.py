@@ -65,7 +65,7 @@ def test_pair_loglik_all_genotype_classes(brute):
 
 
 def _pair_table():
-    """The kernels' per-nibble table (hk_pass.hip fill_pair_lut), restated for the checks below:
+    """The kernels' per-nibble table (hk_pair.h fill_pair_lut), restated for the checks below:
     nibble -> (h s1_00, h s1_01, h s1_11, e), (s2_00, s2_01, s2_11, 2 h s1_01)."""
     def snp_s(c):
         hi, lo = c >> 1, c & 1
@@ -84,7 +84,7 @@ def _pair_table():
 
 
 def _pair_form(X, nib, t1, t2):
-    """PairForm of hk_pass.hip: Y = S1' X S2 and L' = <X, Y> + e."""
+    """PairForm of hk_pair.h: Y = S1' X S2 and L' = <X, Y> + e."""
     a, b = t1[nib], t2[nib]
     z00, z01 = X[0] * b[0] + X[1] * b[1], X[0] * b[1] + X[1] * b[2]
     z10, z11 = X[2] * b[0] + X[3] * b[1], X[2] * b[1] + X[3] * b[2]
@@ -93,7 +93,7 @@ def _pair_form(X, nib, t1, t2):
 
 
 def _pair_lik(X, nib, t1, t2):
-    """pair_lik of hk_pass.hip: the quadratic form G without Y."""
+    """pair_lik of hk_pair.h: the quadratic form G without Y."""
     a, b = t1[nib], t2[nib]
     z00, z01 = X[0] * b[0] + X[1] * b[1], X[0] * b[1] + X[1] * b[2]
     z10, z11 = X[2] * b[0] + X[3] * b[1], X[2] * b[1] + X[3] * b[2]
@@ -101,14 +101,45 @@ def _pair_lik(X, nib, t1, t2):
     return a[0] * g00 + b[3] * g01 + a[2] * g11 + a[3]
 
 
+def _lik_table():
+    """The likelihood passes' factored table (hk_pair.h fill_lik_lut), restated: nibble -> alpha, beta, (dh, e)
+    with L' = (alpha . X)(beta . X) + dh (X_00 X_11 + X_01 X_10) + e."""
+    def u(c, x):  # rank-one factor of a homozygous or missing code
+        return 1.0 if c == 1 else float((c == 0 and x == 0) or (c == 3 and x == 1))
+    al, be, de = np.zeros((16, 4)), np.zeros((16, 4)), np.zeros((16, 2))
+    for nib in range(16):
+        c1, c2 = nib & 3, nib >> 2
+        for tab, out in ((0, al), (1, be)):
+            for f in range(4):
+                a, b = f >> 1, f & 1
+                if nib == 5 or (c1 == 2 and c2 == 2):
+                    v = 0.0
+                elif c1 == 2:
+                    v = float(a == tab) * u(c2, b)
+                elif c2 == 2:
+                    v = u(c1, a) * float(b == tab)
+                else:
+                    v = u(c1, a) * u(c2, b)
+                out[nib, f] = v
+        de[nib] = (1.0 if nib == 10 else 0.0, 1.0 if nib == 5 else 0.0)
+    return al, be, de
+
+
+def _lik_factored(X, nib, tab):
+    """lik_factored of hk_pair.h."""
+    al, be, de = tab
+    return (al[nib] @ X) * (be[nib] @ X) + de[nib, 0] * (X[0] * X[3] + X[1] * X[2]) + de[nib, 1]
+
+
 def test_pair_table_forms_match_bruteforce(brute):
     """Every nibble (both BED codes) and random mixtures: the table-driven per-pair forms of the kernels
     equal the ordered-haplotype-pair enumeration -- likelihood L' (with the reference's 1/2 per het SNP,
-    1 for a double-missing pair) in both of its forms, gradient weights 2 Y_h / L' = c_h / qp_h and
+    1 for a double-missing pair) in all three of its forms (with Y, without, and the factored table), gradient weights 2 Y_h / L' = c_h / qp_h and
     curvature weights Y_h / (L' qp_h) = c_h / (2 qp_h^2) (0 for a double-missing pair) -- so no genotype
     class can drop a term."""
     geno = {0: 0, 1: -1, 2: 1, 3: 2}  # BED code -> allele-2 count (-1 missing), loops.jl:1-2
     t1, t2 = _pair_table()
+    lik = _lik_table()
     rng = np.random.default_rng(11)
     for _ in range(25):
         X = rng.dirichlet(np.ones(4))
@@ -118,6 +149,7 @@ def test_pair_table_forms_match_bruteforce(brute):
             want = brute.pair_loglik(geno[c1], geno[c2], X)
             assert abs(np.log(L) - want) <= 1e-13 * max(1.0, abs(want)), (c1, c2)
             assert abs(np.log(_pair_lik(X, nib, t1, t2)) - want) <= 1e-13 * max(1.0, abs(want)), (c1, c2)
+            assert abs(np.log(_lik_factored(X, nib, lik)) - want) <= 1e-13 * max(1.0, abs(want)), (c1, c2)
             c = brute.pair_counts(geno[c1], geno[c2], X)
             assert np.allclose(2 * Y / L, c / X, rtol=1e-13, atol=1e-15), (c1, c2)
             assert np.allclose(Y / (L * X), c / (2 * X * X), rtol=1e-13, atol=1e-15), (c1, c2)
