@@ -83,6 +83,28 @@ def test_mixed_loglik_rows_every_k(K, oracle):
     assert abs(ll - orows.sum()) <= 1e-6 * abs(ll)
 
 
+@pytest.mark.parametrize("I,J,K", [(1, 1, 2), (1, 4, 3), (2, 1, 1), (5, 2, 4), (3, 9, 6), (1, 17, 8)])
+def test_degenerate_shapes_match_oracle(I, J, K, oracle):
+    """One individual, one SNP block, and half of all genotypes missing: every lane but a few of each pass
+    is padding (individuals past I, blocks past J), so the padded words must contribute exactly nothing.
+    Both engines against the oracle on every pass output."""
+    print(f"degenerate shape I={I} J={J} K={K}", flush=True)
+    s, q, p = case(I, J, K, miss=0.5)
+    olr = oracle.loglik_rows(s.bed, I, J, q, p)
+    oXq, ozq = oracle.update_q(s.bed, I, J, q, p)
+    oXp, ozp = oracle.update_p(s.bed, I, J, q, p)
+    for prec, tol, ltol in (("fp64", 1e-12, 1e-12), ("mixed", 2e-5, 1e-6)):
+        with _eng(s, q, p, prec) as e:
+            ll, rows = e.loglikelihood(rows=True)
+            assert rel(rows, olr) <= ltol, prec
+            assert abs(ll - olr.sum()) <= ltol * abs(olr.sum()), prec
+            XtX, Xtz = e.update_q()
+            assert rel(XtX, oXq) <= tol and rel(Xtz, ozq) <= tol, prec
+            XtX, Xtz = e.update_p()
+            assert rel(XtX, oXp) <= tol and rel(Xtz, ozp) <= tol, prec
+            assert np.array_equal(e.double_missing(), oracle.count_double_missing(s.bed, I, J))
+
+
 def test_fp32_accumulation_mode(oracle):
     """The reference's T2 = Float32 semantics (HK_PREC_FP32): FP32 products and FP32 gradient / curvature
     sums, each over the whole i or j range.  Its passes stay within FP32-accumulation error of the FP64
