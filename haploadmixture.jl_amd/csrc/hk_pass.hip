@@ -154,15 +154,28 @@ struct NAcc {
 // likelihoods (>= ~1e-20) are multiplied into an FP64 mantissa that is renormalised once per genotype
 // word (8 pairs, >= ~1e-160: no underflow) with an integer exponent; one log per lane at the end, so
 // rounding grows like sqrt(J) ulps instead of J logs' worth.  MODE_GRAD: Xtz_q[k, i] = sum_{j,h} w_h p_kh.
+// In FP64 at K <= 8 the likelihood mode takes the factored pair likelihood at 8 waves per SIMD (the pass
+// waits on its scalar p loads; left to itself the compiler holds more loads in flight at half the
+// occupancy, which is slower); above K = 8, and in FP32 (which would spill at that occupancy), the general
+// form at the compiler's occupancy is faster.
 template <int K, typename TP, int MODE>
-__global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
+struct QsideLik {
+  static constexpr bool factored = MODE == MODE_LL && K <= 8 && sizeof(TP) == 8;
+};
+
+template <int K, typename TP, int MODE>
+__global__ __launch_bounds__(256, (QsideLik<K, TP, MODE>::factored ? 8 : 1)) void k_qside(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
                                                int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
                                                double* __restrict__ part, int flush, const int* __restrict__ gate) {
   if (gate_off(gate)) return;
   constexpr int NT = NAcc<K, MODE>::v;
   constexpr bool F32 = sizeof(TP) == 4;
-  __shared__ __attribute__((aligned(16))) TP lut[128];
-  fill_pair_lut<TP>(lut);
+  constexpr bool FACT = QsideLik<K, TP, MODE>::factored;
+  __shared__ __attribute__((aligned(16))) TP lut[FACT ? 160 : 128];
+  if constexpr (FACT)
+    fill_lik_lut<TP>(lut);
+  else
+    fill_pair_lut<TP>(lut);
   __syncthreads();
   unsigned bx, by;
   xcd_grouped_block(bx, by);  // the workgroups of a chunk read its p rows through one L2
@@ -204,7 +217,8 @@ __global__ __launch_bounds__(256) void k_qside(const uint32_t* __restrict__ gq, 
       }
       const unsigned nib = (word >> (4 * n)) & 15u;
       if constexpr (MODE == MODE_LL) {
-        llm *= (double)pair_lik<TP>(X, lut, nib);  // FP64 running product; renormalised by frexp per word
+        // FP64 running product; renormalised by frexp per word
+        llm *= (double)(FACT ? lik_factored<TP>(X, lut, nib) : pair_lik<TP>(X, lut, nib));
       } else {
         const PairForm<TP> f(X, lut, nib);
         const TP rl = hk_rcp(f.L);
