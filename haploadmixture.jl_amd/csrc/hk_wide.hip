@@ -40,6 +40,13 @@ constexpr int wide_tiles() {
   return (SIDE == 1 && (sizeof(TP) == 8 || K > 12)) || WideShape<K>::NTILE == 4 ? 2 : 3;
 }
 
+// Unit index of a staging copy, made opaque to the optimiser: its address arithmetic is then redone at every
+// stage instead of being hoisted out of the genotype loop into registers held across it (which cost occupancy).
+__device__ __forceinline__ unsigned stage_unit(unsigned e) {
+  __asm__ volatile("" : "+v"(e));
+  return e;
+}
+
 // ---------------------------------------------------------------------------------------------------------
 // Q side: a wave owns 2 x 16 individuals; each MFMA step takes 4 blocks.  A stage holds 16 blocks (2 words):
 // per block the window's 4 x 16 NTW table columns and the 4K p values.
@@ -53,7 +60,9 @@ __global__ __launch_bounds__(256) void k_qside_mmw(const uint32_t* __restrict__ 
   using S = WideShape<K>;
   constexpr int NKK = S::NKK, NT = K + NKK, QS = S::QS, RG = S::RGQ;
   constexpr int WPS = 2, R = 2;
-  constexpr int CW = 16 * NTW, RW = 4 * CW + 4 * K, RL = wide_pitch(RW), NST = 8 * WPS * RW;
+  constexpr int CW = 16 * NTW, RW = 4 * CW + 4 * K, RL = wide_pitch(RW);
+  constexpr int U = 16 / (int)sizeof(TP), UR = RW / U, NU = 8 * WPS * UR, UPT = (NU + 255) / 256;
+  static_assert(RW % U == 0 && CW % U == 0 && RG % U == 0 && QS % U == 0, "staged rows are whole 16-byte units");
   constexpr bool F32 = sizeof(TP) == 4;
   using M = Mfma<TP>;
   using V4 = typename M::v4;
@@ -89,18 +98,23 @@ __global__ __launch_bounds__(256) void k_qside_mmw(const uint32_t* __restrict__ 
   int nword = 0;
   for (int jws = jw0; jws < jw1; jws += WPS) {
     const int nw = min(WPS, jw1 - jws);
-    __syncthreads();
+    // stage in 16-byte units (each thread's loads unrolled, so they are in flight together)
     const TP* base = pp + (int64_t)jws * 8 * RG;  // rows of blocks 8 jws .. 8 jws + 15 (the table is padded)
-    for (int e = threadIdx.x; e < NST; e += 256) {
-      const int b = e / RW, f = e - b * RW;
-      TP v;
-      if (f < 4 * CW) {
-        const int h = f / CW, col = c0 + (f - h * CW);
-        v = col < QS ? base[(int64_t)b * RG + h * QS + col] : (TP)0;
-      } else {
-        v = base[(int64_t)b * RG + 4 * QS + (f - 4 * CW)];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < UPT; ++u) {
+      const unsigned e = stage_unit(threadIdx.x + 256u * u);
+      if (e < (unsigned)NU) {
+        const unsigned b = e / UR, f = (e - b * UR) * U;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (f < 4u * CW) {
+          const unsigned h = f / CW, col = c0 + (f - h * CW);
+          if (col < (unsigned)QS) v = *reinterpret_cast<const uint4*>(base + (int64_t)b * RG + h * QS + col);
+        } else {
+          v = *reinterpret_cast<const uint4*>(base + (int64_t)b * RG + 4 * QS + (f - 4 * CW));
+        }
+        *reinterpret_cast<uint4*>(prow + b * RL + f) = v;  // columns past QS: zero padding
       }
-      prow[b * RL + f] = v;
     }
     __syncthreads();
     for (int w = 0; w < nw; ++w) {
@@ -178,7 +192,10 @@ __global__ __launch_bounds__(256) void k_pside_mmw(const uint32_t* __restrict__ 
   constexpr bool CURV = MODE == MODE_CURV;
   constexpr int NKK = S::NKK, NT = CURV ? K + NKK : K, QS = S::QS, RG = S::RGP;
   constexpr int WPS = 4;
-  constexpr int CW = CURV ? 16 * NTW : 0, RW = CURV ? CW + K : 16, RL = wide_pitch(RW), NST = 8 * WPS * RW;
+  constexpr int U = 16 / (int)sizeof(TP), KU = (K + U - 1) / U * U;  // q values staged in whole 16-byte units
+  constexpr int CW = CURV ? 16 * NTW : 0, RW = CURV ? CW + KU : 16, RL = wide_pitch(RW), NST = 8 * WPS * RW;
+  constexpr int UR = RW / U, NU = 8 * WPS * UR, UPT = (NU + 255) / 256;
+  static_assert(!CURV || (RW % U == 0 && RG % U == 0 && QS % U == 0 && QS + KU <= RG), "whole 16-byte units");
   constexpr bool F32 = sizeof(TP) == 4;
   using M = Mfma<TP>;
   using V4 = typename M::v4;
@@ -210,24 +227,33 @@ __global__ __launch_bounds__(256) void k_pside_mmw(const uint32_t* __restrict__ 
   int nword = 0;
   for (int iws = iw0; iws < iw1; iws += WPS) {
     const int nw = min(WPS, iw1 - iws);
-    __syncthreads();
-    for (int e = threadIdx.x; e < NST; e += 256) {
-      const int b = e / RW, f = e - b * RW;
-      const int64_t ind = (int64_t)iws * 8 + b;  // < Ip + 32: the q table is padded, q is clamped
-      TP v;
-      if constexpr (CURV) {
-        const TP* src = qq + ind * RG;
-        if (f < CW) {
-          const int col = c0 + f;
-          v = col < QS ? src[col] : (TP)0;
-        } else {
-          v = src[QS + (f - CW)];
+    if constexpr (CURV) {
+      // 16-byte units of the window's columns and the individual's q values (the table row is padded past
+      // them; each thread's loads unrolled, so they are in flight together)
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < UPT; ++u) {
+        const unsigned e = stage_unit(threadIdx.x + 256u * u);
+        if (e < (unsigned)NU) {
+          const unsigned b = e / UR, f = (e - b * UR) * U;
+          const TP* src = qq + ((int64_t)iws * 8 + b) * RG;  // < Ip + 32 rows: the q table is padded
+          uint4 v = make_uint4(0u, 0u, 0u, 0u);
+          if (f < (unsigned)CW) {
+            if (c0 + f < (unsigned)QS) v = *reinterpret_cast<const uint4*>(src + c0 + f);  // else zero
+          } else {
+            v = *reinterpret_cast<const uint4*>(src + QS + (f - CW));
+          }
+          *reinterpret_cast<uint4*>(qrow + b * RL + f) = v;
         }
-      } else {
-        const int64_t ic = ind < I ? ind : I - 1;
-        v = f < K ? q[f + (int64_t)K * ic] : (TP)0;
       }
-      qrow[b * RL + f] = v;
+    } else {
+      __syncthreads();
+      for (int e = threadIdx.x; e < NST; e += 256) {
+        const int b = e / RW, f = e - b * RW;
+        const int64_t ind = (int64_t)iws * 8 + b;
+        const int64_t ic = ind < I ? ind : I - 1;
+        qrow[b * RL + f] = f < K ? q[f + (int64_t)K * ic] : (TP)0;
+      }
     }
     __syncthreads();
     for (int w = 0; w < nw; ++w) {
