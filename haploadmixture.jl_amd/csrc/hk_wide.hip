@@ -3,9 +3,9 @@
 //
 // The passes for K <= 8 (hk_pass.hip) keep every packed curvature column of a pair's K x K product in the
 // accumulators of one workgroup.  At K = 16 that is 136 columns, more than the registers hold, so here the
-// packed columns are split into windows of NTW tiles of 16 (grid.z = window): each workgroup recomputes the
-// pair weights of its pairs and accumulates only its window's columns, reading the window's slice of the
-// staging table.  The arithmetic per pair is the one of the K <= 8 kernels (pair_v / PairForm, MFMA with the
+// packed columns are split into windows of NTW tiles of 16 (grid.z = window; the tiles NTW does not divide go
+// to one more launch with a smaller window): each workgroup recomputes the pair weights of its pairs and
+// accumulates only its window's columns, reading the window's slice of the staging table.  The arithmetic per pair is the one of the K <= 8 kernels (pair_v / PairForm, MFMA with the
 // weights as A and the table as B, MIXED: FP32 partials flushed into FP64); there is no VALU column tail, the
 // last tile is zero-padded.  The P-side EM sums use the same kernel with the q values of each individual as
 // the single B tile (Xtz_p[k, jh] = sum_i w_h q_ki).
@@ -55,7 +55,8 @@ __device__ __forceinline__ unsigned stage_unit(unsigned e) {
 template <int K, typename TP, int NTW>
 __global__ __launch_bounds__(256) void k_qside_mmw(const uint32_t* __restrict__ gq, int64_t Ip, int64_t I, int J8,
                                                    int wpc, const TP* __restrict__ q, const TP* __restrict__ pp,
-                                                   double* __restrict__ part, int flush, const int* __restrict__ gate) {
+                                                   double* __restrict__ part, int flush, int cbase,
+                                                   const int* __restrict__ gate) {
   if (gate_off(gate)) return;
   using S = WideShape<K>;
   constexpr int NKK = S::NKK, NT = K + NKK, QS = S::QS, RG = S::RGQ;
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(256) void k_qside_mmw(const uint32_t* __restrict__ 
   xcd_grouped_block(bx, by);
   const int64_t ibase = ((int64_t)bx * 4 + wv) * 16 * R;
   const int chunk = by;
-  const int c0 = 16 * NTW * (int)blockIdx.z;  // first packed column of this window
+  const int c0 = cbase + 16 * NTW * (int)blockIdx.z;  // first packed column of this window
   const int jw0 = chunk * wpc, jw1 = min(J8, jw0 + wpc);
   fill_pair_lut<TP>(lut);
   TP qi[R][K];
@@ -186,7 +187,7 @@ template <int K, typename TP, int NTW, int MODE>
 __global__ __launch_bounds__(256) void k_pside_mmw(const uint32_t* __restrict__ gp, int64_t Jp, int64_t I, int I8,
                                                    int wpc, const TP* __restrict__ q, const TP* __restrict__ p,
                                                    const TP* __restrict__ qq, double* __restrict__ out, int flush,
-                                                   const int* __restrict__ gate) {
+                                                   int cbase, const int* __restrict__ gate) {
   if (gate_off(gate)) return;
   using S = WideShape<K>;
   constexpr bool CURV = MODE == MODE_CURV;
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(256) void k_pside_mmw(const uint32_t* __restrict__ 
   const int64_t jbase = ((int64_t)blockIdx.x * 4 + wv) * 16;
   const int64_t j = jbase + m;
   const int chunk = blockIdx.y;
-  const int c0 = 16 * NTW * (int)blockIdx.z;
+  const int c0 = cbase + 16 * NTW * (int)blockIdx.z;
   const int iw0 = chunk * wpc, iw1 = min(I8, iw0 + wpc);
   fill_pair_lut<TP>(lut);
   TP P[4][K];
@@ -302,7 +303,7 @@ __global__ __launch_bounds__(256) void k_pside_mmw(const uint32_t* __restrict__ 
       }
     }
   }
-  if (CURV && blockIdx.z == 0) {  // gradient slots of (block j, haplotype g), formed by the finishing kernel
+  if (CURV && c0 == 0) {  // gradient slots of (block j, haplotype g), formed by the finishing kernel
     double* o = out + ((int64_t)chunk * 4 * Jp + 4 * j + g) * NT;
 #pragma unroll
     for (int k = 0; k < K; ++k) o[k] = 0.0;
@@ -348,10 +349,6 @@ __global__ void k_finish_pw(int K, int64_t J, const double* __restrict__ p, doub
 // ---------------------------------------------------------------------------------------------------------
 // Launchers (9 <= K <= 16)
 // ---------------------------------------------------------------------------------------------------------
-template <int K, typename TP, int SIDE>
-static int wide_windows() {
-  return (WideShape<K>::NTILE + wide_tiles<K, TP, SIDE>() - 1) / wide_tiles<K, TP, SIDE>();
-}
 
 template <int K, typename TP>
 static hipError_t wide_tables_k(int side, const PassGeom& g, const void* x, hipStream_t s) {
@@ -368,11 +365,19 @@ static hipError_t wide_tables_k(int side, const PassGeom& g, const void* x, hipS
   return hipGetLastError();
 }
 
+// The packed columns' NTILE tiles go to NTILE / NTW windows of NTW tiles (one launch, grid.z = window) and, when
+// NTW does not divide NTILE, one more launch for a window of the remaining tiles, so that no window carries
+// all-padding tiles (each tile costs four matrix instructions per pair step).
 template <int K, typename TP>
 static hipError_t wide_qcurv_k(const PassGeom& g, const void* q, double* part, hipStream_t s) {
-  const dim3 grid((unsigned)(g.Ip / 128), (unsigned)g.qchunks, (unsigned)wide_windows<K, TP, 0>());
-  hipLaunchKernelGGL((k_qside_mmw<K, TP, wide_tiles<K, TP, 0>()>), grid, dim3(256), 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc,
-                     (const TP*)q, (const TP*)g.pp, part, g.flush_curv, g.gate);
+  constexpr int NTW = wide_tiles<K, TP, 0>(), NF = WideShape<K>::NTILE / NTW, REM = WideShape<K>::NTILE % NTW;
+  hipLaunchKernelGGL((k_qside_mmw<K, TP, NTW>), dim3((unsigned)(g.Ip / 128), (unsigned)g.qchunks, (unsigned)NF),
+                     dim3(256), 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, (const TP*)q, (const TP*)g.pp, part,
+                     g.flush_curv, 0, g.gate);
+  if constexpr (REM > 0)
+    hipLaunchKernelGGL((k_qside_mmw<K, TP, REM>), dim3((unsigned)(g.Ip / 128), (unsigned)g.qchunks, 1u), dim3(256), 0,
+                       s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, (const TP*)q, (const TP*)g.pp, part, g.flush_curv,
+                       16 * NTW * NF, g.gate);
   return hipGetLastError();
 }
 
@@ -380,13 +385,18 @@ template <int K, typename TP>
 static hipError_t wide_pside_k(int mode, const PassGeom& g, const void* q, const void* p, double* out,
                                hipStream_t s) {
   if (mode == MODE_CURV) {
-    const dim3 grid((unsigned)(g.Jp / 64), (unsigned)g.pchunks, (unsigned)wide_windows<K, TP, 1>());
-    hipLaunchKernelGGL((k_pside_mmw<K, TP, wide_tiles<K, TP, 1>(), MODE_CURV>), grid, dim3(256), 0, s, g.gp, g.Jp, g.I,
-                       (int)g.I8, g.pwpc, (const TP*)q, (const TP*)p, (const TP*)g.qq, out, g.flush_curv, g.gate);
+    constexpr int NTW = wide_tiles<K, TP, 1>(), NF = WideShape<K>::NTILE / NTW, REM = WideShape<K>::NTILE % NTW;
+    hipLaunchKernelGGL((k_pside_mmw<K, TP, NTW, MODE_CURV>), dim3((unsigned)(g.Jp / 64), (unsigned)g.pchunks, (unsigned)NF),
+                       dim3(256), 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, (const TP*)q, (const TP*)p,
+                       (const TP*)g.qq, out, g.flush_curv, 0, g.gate);
+    if constexpr (REM > 0)
+      hipLaunchKernelGGL((k_pside_mmw<K, TP, REM, MODE_CURV>), dim3((unsigned)(g.Jp / 64), (unsigned)g.pchunks, 1u),
+                         dim3(256), 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, (const TP*)q, (const TP*)p,
+                         (const TP*)g.qq, out, g.flush_curv, 16 * NTW * NF, g.gate);
   } else if (mode == MODE_GRAD) {
     const dim3 grid((unsigned)(g.Jp / 64), (unsigned)g.pchunks, 1);
     hipLaunchKernelGGL((k_pside_mmw<K, TP, 1, MODE_GRAD>), grid, dim3(256), 0, s, g.gp, g.Jp, g.I, (int)g.I8,
-                       g.pwpc, (const TP*)q, (const TP*)p, (const TP*)nullptr, out, g.flush_grad, g.gate);
+                       g.pwpc, (const TP*)q, (const TP*)p, (const TP*)nullptr, out, g.flush_grad, 0, g.gate);
   } else {
     return hipErrorInvalidValue;
   }
