@@ -105,6 +105,36 @@ def test_degenerate_shapes_match_oracle(I, J, K, oracle):
             assert np.array_equal(e.double_missing(), oracle.count_double_missing(s.bed, I, J))
 
 
+def _random_shape(seed):
+    r = np.random.default_rng(1000 + seed)
+    return (int(r.integers(1, 701)), int(r.integers(1, 301)), int(r.integers(1, 17)),
+            float(r.choice([0.0, 0.05, 0.3])), float(r.choice([0.02, 0.1, 0.3])))
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_shapes_match_oracle(seed, oracle):
+    """Seeded random shapes (I up to 700, J up to 300, K 1..16, missingness 0 / 5 / 30 %, Fst 0.02-0.3):
+    log-likelihood rows, both curvature passes and EM of both engines against the oracle, element by element."""
+    I, J, K, miss, fst = _random_shape(seed)
+    print(f"random shape seed={seed} I={I} J={J} K={K} miss={miss} fst={fst}", flush=True)
+    s, q, p = case(I, J, K, seed=50 + seed, miss=miss, fst=fst, init_seed=70 + seed)
+    olr = oracle.loglik_rows(s.bed, I, J, q, p)
+    oXq, ozq = oracle.update_q(s.bed, I, J, q, p)
+    oXp, ozp = oracle.update_p(s.bed, I, J, q, p)
+    oq, op = oracle.em(s.bed, I, J, q, p)
+    for prec, tol, ltol in (("fp64", 1e-12, 1e-12), ("mixed", 2e-5, 1e-6)):
+        with _eng(s, q, p, prec) as e:
+            ll, rows = e.loglikelihood(rows=True)
+            assert rel(rows, olr) <= ltol and abs(ll - olr.sum()) <= ltol * abs(olr.sum()), prec
+            XtX, Xtz = e.update_q()
+            assert rel(XtX, oXq) <= tol and rel(Xtz, ozq) <= tol, prec
+            XtX, Xtz = e.update_p()
+            assert rel(XtX, oXp) <= tol and rel(Xtz, ozp) <= tol, prec
+            if prec == "fp64":
+                qn, pn = e.em()
+                assert rel(qn, oq) <= 1e-12 and rel(pn, op) <= 1e-12
+
+
 def test_fp32_accumulation_mode(oracle):
     """The reference's T2 = Float32 semantics (HK_PREC_FP32): FP32 products and FP32 gradient / curvature
     sums, each over the whole i or j range.  Its passes stay within FP32-accumulation error of the FP64
