@@ -270,6 +270,22 @@ def test_fit_first_iterations_track_oracle(oracle):
     assert rel(r["trace"], otr) <= 1e-10
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_random_fits_track_oracle(seed, oracle):
+    """Seeded random shapes (I up to 300, J up to 150, K 2..12): the FP64 device loop's log-likelihood trace over
+    the EM warm-up and 8 QN iterations against the CPU loop, iteration by iteration."""
+    r = np.random.default_rng(2000 + seed)
+    I, J, K = int(r.integers(20, 301)), int(r.integers(10, 151)), int(r.integers(2, 13))
+    fst, miss = float(r.choice([0.05, 0.1, 0.2])), float(r.choice([0.0, 0.02, 0.1]))
+    print(f"random fit seed={seed} I={I} J={J} K={K} fst={fst} miss={miss}", flush=True)
+    s, q, p = case(I, J, K, seed=90 + seed, fst=fst, miss=miss, init_seed=110 + seed)
+    _, _, _, _, otr = oracle.fit(s.bed, I, J, q, p, n_iter=8, rtol=0.0)
+    with _eng(s, q, p) as e:
+        res = e.fit(n_iter=8, rtol=0.0)
+    assert len(res["trace"]) == len(otr) == 8
+    assert rel(res["trace"], otr) <= 1e-10
+
+
 @pytest.mark.parametrize("opts", [dict(fix_q=True), dict(fix_p=True), dict(q_penalty=True),
                                   dict(p_penalty=False), dict(guarantee_increase=True), dict(qn_mode=1)],
                          ids=["fix_q", "fix_p", "q_penalty", "p_equality", "guarantee_increase", "lbqn"])
