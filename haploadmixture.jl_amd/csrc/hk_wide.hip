@@ -371,6 +371,7 @@ static hipError_t wide_tables_k(int side, const PassGeom& g, const void* x, hipS
 template <int K, typename TP>
 static hipError_t wide_qcurv_k(const PassGeom& g, const void* q, double* part, hipStream_t s) {
   constexpr int NTW = wide_tiles<K, TP, 0>(), NF = WideShape<K>::NTILE / NTW, REM = WideShape<K>::NTILE % NTW;
+  static_assert(NF >= 1, "at least one full window (NTILE >= 3 for K >= 9)");
   hipLaunchKernelGGL((k_qside_mmw<K, TP, NTW>), dim3((unsigned)(g.Ip / 128), (unsigned)g.qchunks, (unsigned)NF),
                      dim3(256), 0, s, g.gq, g.Ip, g.I, (int)g.J8, g.qwpc, (const TP*)q, (const TP*)g.pp, part,
                      g.flush_curv, 0, g.gate);
@@ -386,6 +387,7 @@ static hipError_t wide_pside_k(int mode, const PassGeom& g, const void* q, const
                                hipStream_t s) {
   if (mode == MODE_CURV) {
     constexpr int NTW = wide_tiles<K, TP, 1>(), NF = WideShape<K>::NTILE / NTW, REM = WideShape<K>::NTILE % NTW;
+    static_assert(NF >= 1, "at least one full window (NTILE >= 3 for K >= 9)");
     hipLaunchKernelGGL((k_pside_mmw<K, TP, NTW, MODE_CURV>), dim3((unsigned)(g.Jp / 64), (unsigned)g.pchunks, (unsigned)NF),
                        dim3(256), 0, s, g.gp, g.Jp, g.I, (int)g.I8, g.pwpc, (const TP*)q, (const TP*)p,
                        (const TP*)g.qq, out, g.flush_curv, 0, g.gate);
