@@ -31,12 +31,18 @@ struct WideShape {
 // of four different rows, which then fall on distinct banks.
 __host__ __device__ constexpr int wide_pitch(int n) { return n <= 16 ? 16 : ((n - 16 + 63) / 64) * 64 + 16; }
 
-// Packed-column tiles per window.  Fewer windows mean less repeated per-pair algebra, more tiles per window
-// more accumulators: three keep two waves per SIMD on the Q side and, for FP32 up to K = 12, on the P side
-// (whose lanes also hold a block's 4K p values), two otherwise; with four tiles in all (K = 10) two windows of
-// two beat two windows of three, one of them half empty.
+// Packed-column tiles per window.  Fewer windows mean less repeated per-pair algebra (X and the pair weights are
+// recomputed per window), more tiles per window more accumulators.  FP32 Q side: up to five tiles in one window
+// (K <= 12), windows of four for seven or eight tiles (K = 14, 15) and of three otherwise (K = 13, 16), all at two
+// waves per SIMD (six tiles, or five beside four, would drop to one).  FP64 Q side: three.  P side (whose lanes
+// also hold a block's 4K p values): three for FP32 up to K = 12, two otherwise, and two for four tiles in all
+// (K = 10), where two windows of two beat two of three, one of them half empty.
 template <int K, typename TP, int SIDE>
 constexpr int wide_tiles() {
+  if constexpr (SIDE == 0 && sizeof(TP) == 4) {
+    constexpr int NT = WideShape<K>::NTILE;
+    return NT <= 5 ? NT : (NT == 7 || NT == 8 ? 4 : 3);
+  }
   return (SIDE == 1 && (sizeof(TP) == 8 || K > 12)) || WideShape<K>::NTILE == 4 ? 2 : 3;
 }
 
