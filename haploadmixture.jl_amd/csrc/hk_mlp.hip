@@ -327,8 +327,9 @@ __global__ void k_softmax(const float* __restrict__ Z, int64_t n, int K, float* 
   for (int k = 0; k < K; ++k) P[i * K + k] = expf(z[k] - mx) / s;
 }
 
-// Mx[k * K + k2] = sum_i T[i, k2] log P[i, k], in two fixed-order stages: workgroup b sums rows b, b + nb, ...
-// (four row groups of 64 threads, thread = entry (k, k2)) into part[b][64]; k_match_sum adds the nb partials.
+// Mx[k * K + k2] = sum_i T[i, k2] log P[i, k], in two fixed-order stages.  K <= 8: workgroup b sums rows b,
+// b + nb, ... (four row groups of 64 threads, thread = entry (k, k2)) into part[b][64]; k_match_sum adds the nb
+// partials.
 constexpr int MATCH_BLOCKS = 64;
 __global__ __launch_bounds__(256) void k_match(const float* __restrict__ P, const float* __restrict__ T, int64_t n,
                                                int K, double* __restrict__ part) {
@@ -345,11 +346,65 @@ __global__ __launch_bounds__(256) void k_match(const float* __restrict__ P, cons
   if (rg == 0) part[blockIdx.x * 64 + e] = ((sh[0][e] + sh[1][e]) + sh[2][e]) + sh[3][e];
 }
 
-__global__ void k_match_sum(const double* __restrict__ part, int nb, double* __restrict__ Mx) {
+// K = 9..16: one thread per entry (K * K <= 256), workgroup b sums rows b, b + nb, ... into part[b][256].
+__global__ __launch_bounds__(256) void k_match_w(const float* __restrict__ P, const float* __restrict__ T, int64_t n,
+                                                 int K, double* __restrict__ part) {
   const int e = threadIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[b * 64 + e];
+  if (e < K * K) {
+    const int k = e / K, k2 = e - k * K;
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x)
+      s += (double)T[i * K + k2] * log(fmaxf(P[i * K + k], 1e-30f));
+  }
+  part[blockIdx.x * 256 + e] = s;
+}
+
+// Mx[e] = sum over b of part[b][e] (row stride 64 or 256), b ascending.
+__global__ void k_match_sum(const double* __restrict__ part, int nb, int stride, double* __restrict__ Mx) {
+  const int e = threadIdx.x;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[b * stride + e];
   Mx[e] = s;
+}
+
+// Minimum-cost assignment of rows to columns of the K x K matrix C (row-major): perm[k] = column of row k.
+// Kuhn-Munkres with row/column potentials and shortest augmenting paths (O(K^3)): row r is added to the
+// matching by a Dijkstra-like search over reduced costs C[r][c] - u[r] - v[c] >= 0, then the potentials are
+// moved so the reduced costs stay non-negative and those on the matching stay zero.
+static void assign_min_cost(const double* C, int K, int* perm) {
+  constexpr double INF = 1e300;
+  std::vector<double> u(K + 1, 0.0), v(K + 1, 0.0), minv(K + 1);
+  std::vector<int> row_of(K + 1, 0), way(K + 1, 0);  // 1-based; column 0 is the virtual root
+  std::vector<char> used(K + 1);
+  for (int r = 1; r <= K; ++r) {
+    row_of[0] = r;
+    int c0 = 0;
+    std::fill(minv.begin(), minv.end(), INF);
+    std::fill(used.begin(), used.end(), 0);
+    do {
+      used[c0] = 1;
+      const int r0 = row_of[c0];
+      double delta = INF;
+      int c1 = 0;
+      for (int c = 1; c <= K; ++c) {
+        if (used[c]) continue;
+        const double cur = C[(r0 - 1) * K + (c - 1)] - u[r0] - v[c];
+        if (cur < minv[c]) { minv[c] = cur; way[c] = c0; }
+        if (minv[c] < delta) { delta = minv[c]; c1 = c; }
+      }
+      for (int c = 0; c <= K; ++c) {
+        if (used[c]) { u[row_of[c]] += delta; v[c] -= delta; }
+        else minv[c] -= delta;
+      }
+      c0 = c1;
+    } while (row_of[c0] != 0);
+    do {  // flip the augmenting path back to the root
+      const int c1 = way[c0];
+      row_of[c0] = row_of[c1];
+      c0 = c1;
+    } while (c0);
+  }
+  for (int c = 1; c <= K; ++c) perm[row_of[c] - 1] = c - 1;
 }
 
 // dZ[i, k] = (P[i, k] - T[i, perm[k]]) / n
@@ -522,7 +577,7 @@ int hk_mlp_destroy(hk_mlp* m) {
 const char* hk_mlp_last_error(const hk_mlp* m) { return m ? m->err : ""; }
 
 int hk_mlp_create(hk_mlp** out, int din, int width, int dout, uint64_t seed) {
-  if (!out || din < 1 || width < 1 || dout < 2 || dout > 8) return HK_EINVAL;
+  if (!out || din < 1 || width < 1 || dout < 2 || dout > HK_MAX_K) return HK_EINVAL;
   hk_mlp* m = new hk_mlp();
   m->din = din;
   m->width = width;
@@ -544,8 +599,8 @@ int hk_mlp_create(hk_mlp** out, int din, int width, int dout, uint64_t seed) {
   H(hipMalloc(&m->m, sizeof(float) * o));
   H(hipMalloc(&m->v, sizeof(float) * o));
   H(hipMalloc(&m->wb, sizeof(uint16_t) * o));
-  H(hipMalloc(&m->Mx, sizeof(double) * 64 * (1 + MATCH_BLOCKS)));  // result, then the stage-1 partials
-  H(hipMalloc(&m->perm, sizeof(int) * 8));
+  H(hipMalloc(&m->Mx, sizeof(double) * 256 * (1 + MATCH_BLOCKS)));  // result, then the stage-1 partials
+  H(hipMalloc(&m->perm, sizeof(int) * HK_MAX_K));
   if (rc == HK_OK) {
     // He-uniform initialisation on the host (seeded), biases zero.
     std::vector<float> w(o, 0.0f);
@@ -654,16 +709,32 @@ int hk_mlp_grad(hk_mlp* m, int64_t row0, int64_t n, int match, float* loss, int*
   rc = mlp_forward_dev(m, X, n);
   if (rc != HK_OK) return rc;
   const int nbm = (int)std::min<int64_t>(MATCH_BLOCKS, (n + 255) / 256);
-  hipLaunchKernelGGL(k_match, dim3(nbm), dim3(256), 0, m->s, m->P, T, n, K, m->Mx + 64);
-  hipLaunchKernelGGL(k_match_sum, dim3(1), dim3(64), 0, m->s, m->Mx + 64, nbm, m->Mx);
-  double Mx[64];
+  if (K <= 8) {
+    hipLaunchKernelGGL(k_match, dim3(nbm), dim3(256), 0, m->s, m->P, T, n, K, m->Mx + 256);
+    hipLaunchKernelGGL(k_match_sum, dim3(1), dim3(64), 0, m->s, m->Mx + 256, nbm, 64, m->Mx);
+  } else {
+    hipLaunchKernelGGL(k_match_w, dim3(nbm), dim3(256), 0, m->s, m->P, T, n, K, m->Mx + 256);
+    hipLaunchKernelGGL(k_match_sum, dim3(1), dim3(256), 0, m->s, m->Mx + 256, nbm, 256, m->Mx);
+  }
+  double Mx[HK_MAX_K * HK_MAX_K];
   MH(hipMemcpyAsync(Mx, m->Mx, sizeof(double) * K * K, hipMemcpyDeviceToHost, m->s));
   MH(hipStreamSynchronize(m->s));
-  int perm[8], best[8];
+  int perm[HK_MAX_K], best[HK_MAX_K];
   for (int k = 0; k < K; ++k) perm[k] = best[k] = k;
   double best_v = 0.0;
   for (int k = 0; k < K; ++k) best_v -= Mx[k * K + k];
-  if (match) {
+  if (match && K > 8) {
+    // K! is out of reach: the assignment problem on the cost -Mx (exact; ties may resolve differently from
+    // the enumeration's first minimum, which needs exactly equal sums)
+    double C[HK_MAX_K * HK_MAX_K];
+    for (int e = 0; e < K * K; ++e) C[e] = -Mx[e];
+    assign_min_cost(C, K, best);
+    double v = 0.0;
+    for (int k = 0; k < K; ++k) v -= Mx[k * K + best[k]];
+    if (v < best_v) best_v = v;
+    else
+      for (int k = 0; k < K; ++k) best[k] = k;  // the identity is optimal too: keep it, as the enumeration does
+  } else if (match) {
     // enumerate permutations (Heap's algorithm order is irrelevant: keep the first minimum)
     std::vector<int> c(K, 0);
     int i = 1;

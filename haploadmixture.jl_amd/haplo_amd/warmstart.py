@@ -118,9 +118,9 @@ class MLP:
 
     def grad(self, row0, n, match=True):
         loss = ctypes.c_float()
-        perm = np.zeros(8, dtype=np.int32)
+        perm = np.zeros(self.K, dtype=np.int32)
         self._check(self._L.hk_mlp_grad(self._h, int(row0), int(n), int(match), ctypes.byref(loss), ptr(perm)))
-        return loss.value, perm[: self.K].copy()
+        return loss.value, perm
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         self.step += 1

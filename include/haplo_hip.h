@@ -242,7 +242,7 @@ int hk_time_pass(hk_ctx* ctx, int kind, int reps, double* ms_per_launch);
  * Gaussian S x 256 matrix, mu_s the observed mean dosage; F is I x 256 row-major (host). */
 int hk_features(hk_ctx* ctx, int n_blocks, uint64_t seed, float* F);
 
-/* MLP din -> width -> width -> dout (ReLU, softmax), BF16 weights, FP32 accumulation, Adam. */
+/* MLP din -> width -> width -> dout (ReLU, softmax), BF16 weights, FP32 accumulation, Adam; 2 <= dout <= HK_MAX_K. */
 typedef struct hk_mlp hk_mlp;
 int hk_mlp_create(hk_mlp** out, int din, int width, int dout, uint64_t seed);
 int hk_mlp_destroy(hk_mlp* m);
@@ -261,7 +261,8 @@ int hk_mlp_forward(hk_mlp* m, const float* X, int64_t n, float* P);
  * with the network for the next call (din must be 256).  Errors are reported on ctx. */
 int hk_mlp_predict(hk_mlp* m, hk_ctx* ctx, int n_blocks, uint64_t seed, float* P);
 /* Forward + backward on resident rows [row0, row0+n): gradients stay on the device (hk_mlp_get_params);
- * match != 0 relabels the target columns by the cross-entropy-minimising permutation. */
+ * match != 0 relabels the target columns by the cross-entropy-minimising permutation (every permutation
+ * enumerated for dout <= 8, the first minimum kept; a minimum-cost assignment above). */
 int hk_mlp_grad(hk_mlp* m, int64_t row0, int64_t n, int match, float* loss, int* perm);
 /* One Adam step (bias-corrected, step >= 1) on the stored gradients; refreshes the BF16 weights. */
 int hk_mlp_adam(hk_mlp* m, float lr, float beta1, float beta2, float eps, int step);

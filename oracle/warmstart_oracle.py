@@ -87,9 +87,18 @@ def forward(w, offsets, D, H, K, X, round_bf16=True):
 
 
 def best_perm(P, T):
-    """perm minimising -sum_i sum_k T[i, perm[k]] log P[i, k] (first minimum in lexicographic order)."""
+    """perm minimising -sum_i sum_k T[i, perm[k]] log P[i, k]: K <= 8 enumerates the permutations (first
+    minimum in lexicographic order); larger K solves the assignment problem on the K x K cost matrix with
+    scipy.optimize.linear_sum_assignment, keeping the identity when it is as good."""
     K = P.shape[1]
     L = np.log(np.maximum(P, 1e-30))
+    if K > 8:
+        from scipy.optimize import linear_sum_assignment
+
+        C = -(L.T @ np.asarray(T, dtype=np.float64))  # C[k, k2] = -sum_i T[i, k2] log P[i, k]
+        _, cols = linear_sum_assignment(C)
+        v, v_id = float(C[np.arange(K), cols].sum()), float(np.trace(C))
+        return (cols, v) if v < v_id else (np.arange(K), v_id)
     best, best_v = None, np.inf
     for perm in itertools.permutations(range(K)):
         v = -float(np.sum(T[:, list(perm)] * L))

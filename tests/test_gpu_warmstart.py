@@ -75,7 +75,8 @@ def _random_mlp(D, H, K, seed):
     return m, w
 
 
-DIMS = [(256, 512, 5, 300), (256, 48, 3, 37), (37, 40, 8, 19), (256, 512, 2, 500)]
+DIMS = [(256, 512, 5, 300), (256, 48, 3, 37), (37, 40, 8, 19), (256, 512, 2, 500), (256, 64, 12, 300),
+        (40, 48, 16, 70)]
 
 
 @pytest.mark.parametrize("D,H,K,n", DIMS)
@@ -121,12 +122,15 @@ def test_mlp_gradient_matches_numpy(D, H, K, n, match):
     m.close()
 
 
-def test_mlp_matching_recovers_relabelled_targets():
-    D, H, K, n = 256, 64, 5, 200
+@pytest.mark.parametrize("K", [5, 13])
+def test_mlp_matching_recovers_relabelled_targets(K):
+    """Targets whose columns are the network's own output relabelled: the matching (the enumeration at K = 5,
+    the assignment solver at K = 13) returns the relabelling."""
+    D, H, n = 256, 64, 200
     m, w = _random_mlp(D, H, K, 7)
     X = np.random.default_rng(8).normal(size=(n, D)).astype(np.float32)
     P = m.forward(X)
-    sigma = np.array([3, 0, 4, 1, 2])
+    sigma = np.array([3, 0, 4, 1, 2]) if K == 5 else np.random.default_rng(9).permutation(K)
     T = (0.8 * P[:, np.argsort(sigma)] + 0.2 / K).astype(np.float32)
     m.load_data(X, T)
     _, perm = m.grad(0, n, match=True)
@@ -163,8 +167,9 @@ def test_mlp_rejects_bad_arguments():
     with pytest.raises(ValueError):
         m.set_params(np.zeros(3, np.float32))
     m.close()
-    with pytest.raises(HipError):
-        ws.MLP(9, din=16, width=16)
+    for K in (1, 17):  # 2 <= K <= 16
+        with pytest.raises(HipError):
+            ws.MLP(K, din=16, width=16)
 
 
 def test_em_with_q_held_matches_oracle(oracle):
@@ -189,7 +194,7 @@ def test_em_with_q_held_matches_oracle(oracle):
     assert r["ll_init"] == pytest.approx(ll, rel=1e-12)
 
 
-@pytest.mark.parametrize("K", range(2, 9))
+@pytest.mark.parametrize("K", range(2, 17))
 def test_warm_start_reaches_plain_optimum_in_fewer_iterations(K):
     """The shipped network of each K (held-out data of its population structure): the warm-started fit ends
     at least as high as the plain one and reaches 1e-6 of the better optimum in fewer iterations."""
@@ -216,9 +221,10 @@ def test_warm_start_reaches_plain_optimum_in_fewer_iterations(K):
     mlp.close()
 
 
-def test_driver_warm_start_and_fallback(tmp_path, capsys):
-    """run_admixture(warm_start=True): the shipped K=5 network fills init_q; for K=9 there is none (networks
-    stop at K=8) and the random start runs.  Both fits reach the same kind of optimum as the plain driver."""
+def test_driver_warm_start_and_fallback(tmp_path, capsys, monkeypatch):
+    """run_admixture(warm_start=True): the shipped K=5 network fills init_q; with the weights directory
+    pointed at an empty folder there is no network for K=9 and the random start runs.  Both fits reach the
+    same kind of optimum as the plain driver."""
     import os
 
     import haplo_amd
@@ -235,6 +241,7 @@ def test_driver_warm_start_and_fallback(tmp_path, capsys):
     assert warm.ll_new >= plain.ll_new - 1e-5 * abs(plain.ll_new)
     assert np.allclose(warm.q.sum(axis=0), 1.0)
     capsys.readouterr()
+    monkeypatch.setattr(ws, "DATA_DIR", str(tmp_path / "no_networks"))
     fb, _, _ = haplo_amd.run_admixture(f, I, J, 9, T2=np.float32, warm_start=True)
     assert "No warm-start network for K=9" in capsys.readouterr().out
     assert np.isfinite(fb.ll_new)

@@ -58,6 +58,30 @@ def test_permutation_search_is_exhaustive_minimum():
     assert np.array_equal(perm, sigma)
 
 
+@pytest.mark.parametrize("K", [9, 10])
+def test_assignment_for_large_K_is_exhaustive_minimum(K):
+    """Above K = 8 best_perm solves the assignment problem instead of enumerating: its permutation and value
+    equal the minimum over all K! permutations (vectorised brute force), and relabelled targets are undone."""
+    rng = np.random.default_rng(K)
+    n = 60
+    P = rng.dirichlet(np.ones(K), size=n)
+    T = rng.dirichlet(np.ones(K) * 0.3, size=n)
+    perm, v = ow.best_perm(P, T)
+    C = -(np.log(P).T @ T)  # C[k, k2]: cost of giving P's column k the target column k2
+    allp = np.array(list(itertools.permutations(range(K))), dtype=np.int8)
+    vals = C[np.arange(K), allp].sum(axis=1)
+    i = int(np.argmin(vals))
+    assert np.array_equal(perm, allp[i]) and np.sum(vals <= vals[i] + 1e-9 * abs(vals[i])) == 1
+    assert v == pytest.approx(vals[i], rel=1e-13)
+    assert v == pytest.approx(-np.sum(T[:, perm] * np.log(P)), rel=1e-13)
+    sigma = rng.permutation(K)
+    perm2, _ = ow.best_perm(P, P[:, np.argsort(sigma)] * 0.9 + 0.1 / K)
+    assert np.array_equal(perm2, sigma)
+    # the identity is kept when it is optimal
+    perm3, _ = ow.best_perm(P, P * 0.9 + 0.1 / K)
+    assert np.array_equal(perm3, np.arange(K))
+
+
 def test_bf16_rounding_is_nearest_even():
     x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -7 + 2 ** -8, 1.0 + 2 ** -9, -3.0 - 2 ** -7 - 2 ** -9], np.float32)
     got = ow.bf16(x)
@@ -144,9 +168,9 @@ def test_adam_matches_torch_optimizer():
         assert np.max(np.abs(t.detach().numpy() - w)) <= 1e-15 * max(1.0, np.max(np.abs(w)))
 
 
-@pytest.mark.parametrize("K", range(2, 9))
+@pytest.mark.parametrize("K", range(2, 17))
 def test_shipped_network_has_its_training_record(K):
-    """A network ships for every K from 2 to 8, each with the record of the run that produced it
+    """A network ships for every K from 2 to 16, each with the record of the run that produced it
     (haplo_amd.warmstart.train_default): 200 plain-method fits, the loss per epoch, the device time of each
     stage."""
     import json
