@@ -70,7 +70,6 @@ hipError_t launch_curv_tables_wide(int K, int prec, int side, const PassGeom& g,
 hipError_t launch_qside_curv_wide(int K, int prec, const PassGeom& g, const void* q, double* part, hipStream_t s);
 hipError_t launch_pside_wide(int K, int prec, int mode, const PassGeom& g, const void* q, const void* p,
                              double* out, hipStream_t s);
-hipError_t launch_finish_p_wide(int K, const PassGeom& g, const double* p, double* res, hipStream_t s);
 void stage_sizes_wide(int K, int64_t Ip, int64_t Jp, int64_t* nq, int64_t* np);
 hipError_t launch_reduce_q(const double* part, int nchunk, int NT, const PassGeom& g, double* res, hipStream_t s);
 hipError_t launch_reduce_q_curv(const double* part, int nchunk, int K, const PassGeom& g, const double* q, double* res,
@@ -109,9 +108,11 @@ struct PStepLists {
 hipError_t prime_p_step_caps(int K);
 hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
                          int* fails, const PStepLists& lists, hipStream_t s, double* dp = nullptr);
-// 9 <= K <= 16 (hk_step_wide.hip): every block's P QP on the dense one-wave solver (4K <= 64 variables).
+// 9 <= K <= 16 (hk_step_wide.hip): the four-lane kernels of those K, grid capped at `cap` resident workgroups
+// (p_step_cap_wide at set-up), the first `ndb` workgroups on the early dense list; the late list after.
+unsigned p_step_cap_wide(int K, bool penalty);
 hipError_t launch_p_step_wide(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
-                              int* fails, hipStream_t s, double* dp);
+                              int* fails, const PStepLists& lists, unsigned cap, int ndb, hipStream_t s, double* dp);
 hipError_t launch_project_q(int K, int64_t I, double* q, double eps, hipStream_t s, const int* gate = nullptr);
 hipError_t launch_project_p(int K, int64_t J, double* p, double eps, hipStream_t s, const int* gate = nullptr);
 hipError_t launch_em_q(int K, int64_t I, int64_t J, const double* res_q, int NT, const int64_t* dm, const double* q,

@@ -1195,7 +1195,6 @@ hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* re
     const hipError_t e = launch_set_int(dense, 0, s, g.gate);
     if (e != hipSuccess) return e;
   }
-  if (K > HK_MAXK_REG) return launch_finish_p_wide(K, g, p, res, s);  // no early dense list: every P QP is dense
   const dim3 grid((unsigned)((4 * g.J + 255) / 256));
   switch (K) {
     case 1: hipLaunchKernelGGL(k_finish_p<1>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
@@ -1205,7 +1204,16 @@ hipError_t launch_finish_p(int K, const PassGeom& g, const double* p, double* re
     case 5: hipLaunchKernelGGL(k_finish_p<5>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
     case 6: hipLaunchKernelGGL(k_finish_p<6>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
     case 7: hipLaunchKernelGGL(k_finish_p<7>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
-    default: hipLaunchKernelGGL(k_finish_p<8>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 8: hipLaunchKernelGGL(k_finish_p<8>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 9: hipLaunchKernelGGL(k_finish_p<9>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 10: hipLaunchKernelGGL(k_finish_p<10>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 11: hipLaunchKernelGGL(k_finish_p<11>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 12: hipLaunchKernelGGL(k_finish_p<12>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 13: hipLaunchKernelGGL(k_finish_p<13>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 14: hipLaunchKernelGGL(k_finish_p<14>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 15: hipLaunchKernelGGL(k_finish_p<15>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    case 16: hipLaunchKernelGGL(k_finish_p<16>, grid, dim3(256), 0, s, g.J, p, res, flags, dense, g.gate); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

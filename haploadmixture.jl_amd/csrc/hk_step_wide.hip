@@ -1,54 +1,68 @@
 // hk_step_wide.hip — update_p! row step for 9 <= K <= 16 (algorithms_inner.jl:259-312 assembly and QP,
 // :344-351 step and projection).
 //
-// The four-lane P solver (hk_step.hip) keeps a haplotype block's K x K curvature and its Cholesky factor in one
-// lane's registers, which stops at K = 8.  Here every block's QP (4K <= 64 variables) goes to the dense one-wave
-// active-set solver, the same one the K <= 8 path uses for blocks with a singular haplotype curvature: lane r
-// owns row r of the curvature, so the QP is solved with the same rule, ratio test and tie-breaks at any K.
-// The penalty form stays in registers (accumulation registers take the overflow); the equality form's Schur
-// complement over K groups spills to scratch at K >= 12.
+// The same one-launch scheme as K <= 8 (hk_step.hip): the four-lane block-structured solver (hk_pstep4.h) on
+// every block the P-side finishing kernel did not list, the dense one-wave solver on the listed blocks (a free
+// haplotype block with a vanishing Cholesky pivot) in the first workgroups of the same launch, and the late
+// list after it.  From K = 9 on, a lane's K x K block curvature, factor and inverse exceed the register file:
+// the four-lane kernels run one wave per SIMD and spill to scratch.  They still do about a sixteenth of the
+// dense solver's arithmetic per QP (four K x K factorisations against one 4K x 4K) and hold sixteen QPs per
+// wavefront instead of one.
 #include "hk_device.h"
 #include "hk_internal.h"
-#include "hk_qp.h"
+#include "hk_pstep4.h"
 
 namespace hk {
 
-// One wavefront per block, grid-stride over the J blocks.
+template <int K>
+static unsigned p_cap_k(bool pen) {
+  return pen ? resident_blocks(k_p_step4<K, true, true>, 256) : resident_blocks(k_p_step4<K, false, true>, 256);
+}
+
+unsigned p_step_cap_wide(int K, bool pen) {
+  switch (K) {
+    case 9: return p_cap_k<9>(pen);
+    case 10: return p_cap_k<10>(pen);
+    case 11: return p_cap_k<11>(pen);
+    case 12: return p_cap_k<12>(pen);
+    case 13: return p_cap_k<13>(pen);
+    case 14: return p_cap_k<14>(pen);
+    case 15: return p_cap_k<15>(pen);
+    case 16: return p_cap_k<16>(pen);
+    default: return 0;
+  }
+}
+
+// Slots hold several quads in turn (the striding form) whether or not J exceeds the resident slots.
 template <int K, bool PEN>
-__global__ __launch_bounds__(64) void k_p_step_all(int64_t J, const double* __restrict__ res,
-                                                   const double* __restrict__ p, double* __restrict__ p_next,
-                                                   double rho, double eps, int* __restrict__ fails,
-                                                   double* __restrict__ dp, const int* __restrict__ gate) {
-  if (gate_off(gate)) return;
-  __shared__ double lds[4 * K * 4 * K];
-  for (int64_t j = blockIdx.x; j < J; j += gridDim.x)
-    p_step_dense<K, PEN>(j, res, p, p_next, rho, eps, fails, dp, lds);
+static void p_step_wide_k(int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
+                          int* fails, const PStepLists& lists, unsigned cap, int ndb, hipStream_t s, double* dp) {
+  const unsigned nq = (unsigned)((4 * J + 255) / 256), nd = (unsigned)(J < 1024 ? J : 1024);
+  const unsigned grid = nq < cap ? nq : cap;
+  hipLaunchKernelGGL((k_p_step4<K, PEN, true>), dim3(ndb + grid), dim3(256), 0, s, J, res_p, p, p_next, o.rho, o.eps,
+                     dp, lists.flags, lists.dense, ndb, fails, lists.late, o.gate);
+  hipLaunchKernelGGL((k_p_step_list<K, PEN>), dim3(nd), dim3(64), 0, s, res_p, p, p_next, o.rho, o.eps, fails, dp,
+                     lists.late, o.gate);
 }
 
 template <int K>
-static void p_step_all_k(int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
-                         int* fails, hipStream_t s, double* dp) {
-  const unsigned grid = (unsigned)(J < 8192 ? J : 8192);
-  if (o.penalty)
-    hipLaunchKernelGGL((k_p_step_all<K, true>), dim3(grid), dim3(64), 0, s, J, res_p, p, p_next, o.rho, o.eps, fails,
-                       dp, o.gate);
-  else
-    hipLaunchKernelGGL((k_p_step_all<K, false>), dim3(grid), dim3(64), 0, s, J, res_p, p, p_next, o.rho, o.eps,
-                       fails, dp, o.gate);
+static void p_step_wide_pen(int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
+                            int* fails, const PStepLists& lists, unsigned cap, int ndb, hipStream_t s, double* dp) {
+  if (o.penalty) p_step_wide_k<K, true>(J, res_p, p, p_next, o, fails, lists, cap, ndb, s, dp);
+  else p_step_wide_k<K, false>(J, res_p, p, p_next, o, fails, lists, cap, ndb, s, dp);
 }
 
 hipError_t launch_p_step_wide(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
-                              int* fails, hipStream_t s, double* dp) {
-  if (J < 1) return hipSuccess;
+                              int* fails, const PStepLists& lists, unsigned cap, int ndb, hipStream_t s, double* dp) {
   switch (K) {
-    case 9: p_step_all_k<9>(J, res_p, p, p_next, o, fails, s, dp); break;
-    case 10: p_step_all_k<10>(J, res_p, p, p_next, o, fails, s, dp); break;
-    case 11: p_step_all_k<11>(J, res_p, p, p_next, o, fails, s, dp); break;
-    case 12: p_step_all_k<12>(J, res_p, p, p_next, o, fails, s, dp); break;
-    case 13: p_step_all_k<13>(J, res_p, p, p_next, o, fails, s, dp); break;
-    case 14: p_step_all_k<14>(J, res_p, p, p_next, o, fails, s, dp); break;
-    case 15: p_step_all_k<15>(J, res_p, p, p_next, o, fails, s, dp); break;
-    case 16: p_step_all_k<16>(J, res_p, p, p_next, o, fails, s, dp); break;
+    case 9: p_step_wide_pen<9>(J, res_p, p, p_next, o, fails, lists, cap, ndb, s, dp); break;
+    case 10: p_step_wide_pen<10>(J, res_p, p, p_next, o, fails, lists, cap, ndb, s, dp); break;
+    case 11: p_step_wide_pen<11>(J, res_p, p, p_next, o, fails, lists, cap, ndb, s, dp); break;
+    case 12: p_step_wide_pen<12>(J, res_p, p, p_next, o, fails, lists, cap, ndb, s, dp); break;
+    case 13: p_step_wide_pen<13>(J, res_p, p, p_next, o, fails, lists, cap, ndb, s, dp); break;
+    case 14: p_step_wide_pen<14>(J, res_p, p, p_next, o, fails, lists, cap, ndb, s, dp); break;
+    case 15: p_step_wide_pen<15>(J, res_p, p, p_next, o, fails, lists, cap, ndb, s, dp); break;
+    case 16: p_step_wide_pen<16>(J, res_p, p, p_next, o, fails, lists, cap, ndb, s, dp); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

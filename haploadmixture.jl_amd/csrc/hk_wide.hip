@@ -331,27 +331,6 @@ __global__ __launch_bounds__(256) void k_pside_mmw(const uint32_t* __restrict__ 
       }
 }
 
-// Xtz_p[:, jh] = XtX_p[:, :, jh] p[:, jh] into slots 0..K-1 of res[jh][NT]; thread per (jh, k), so the K threads
-// of a column read its packed row together.
-__global__ void k_finish_pw(int K, int64_t J, const double* __restrict__ p, double* __restrict__ res,
-                            const int* __restrict__ gate) {
-  if (gate_off(gate)) return;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < 4 * J * K;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t jh = idx / K;
-    const int k = (int)(idx - jh * K);
-    const int NT = K + K * (K + 1) / 2;
-    const double* r = res + jh * NT + K;
-    const double* pv = p + (int64_t)K * jh;
-    double s = 0.0;
-    for (int k2 = 0; k2 < K; ++k2) {
-      const int hi = k > k2 ? k : k2, lo = k > k2 ? k2 : k;
-      s += r[hi * (hi + 1) / 2 + lo] * pv[k2];
-    }
-    res[jh * NT + k] = s;  // the slots written here are read by no other thread
-  }
-}
-
 // ---------------------------------------------------------------------------------------------------------
 // Launchers (9 <= K <= 16)
 // ---------------------------------------------------------------------------------------------------------
@@ -438,12 +417,6 @@ hipError_t launch_pside_wide(int K, int prec, int mode, const PassGeom& g, const
                              double* out, hipStream_t s) {
   if (prec == PREC_FP64) { HK_WIDE_SWITCH(wide_pside_k, double, mode, g, q, p, out, s) }
   HK_WIDE_SWITCH(wide_pside_k, float, mode, g, q, p, out, s)
-}
-
-hipError_t launch_finish_p_wide(int K, const PassGeom& g, const double* p, double* res, hipStream_t s) {
-  if (K < 9 || K > HK_MAXK) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_finish_pw, dim3(grid_1d(4 * g.J * K)), dim3(256), 0, s, K, g.J, p, res, g.gate);
-  return hipGetLastError();
 }
 
 template <int K>

@@ -45,8 +45,8 @@ enum hk_precision {
                         sum, QP and QN in FP64.  A comparison point for HK_PREC_MIXED. */
 };
 
-/* Largest K.  K <= 8 runs the register-resident pass and four-lane P-step kernels; 9 <= K <= 16 runs passes over
- * windows of the packed K x K columns and the dense one-wave solver for every P row QP (slower per pair). */
+/* Largest K.  K <= 8 runs the register-resident pass kernels; 9 <= K <= 16 runs passes over windows of the packed
+ * K x K columns (slower per pair).  The four-lane P row QPs serve every K (spilling to scratch above 8). */
 #define HK_MAX_K 16
 #define HK_SKFR_MAX_K 16 /* largest SKFR cluster count */
 
