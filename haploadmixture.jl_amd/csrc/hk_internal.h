@@ -108,11 +108,11 @@ struct PStepLists {
 hipError_t prime_p_step_caps(int K);
 hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
                          int* fails, const PStepLists& lists, hipStream_t s, double* dp = nullptr);
-// 9 <= K <= 16 (hk_step_wide.hip): the four-lane kernels of those K, grid capped at `cap` resident workgroups
-// (p_step_cap_wide at set-up), the first `ndb` workgroups on the early dense list; the late list after.
+// 9 <= K <= 16 (hk_step_wide.hip): the four-lane kernels of those K, one quad per slot, the first `ndb`
+// workgroups on the early dense list; the late list after.  p_step_cap_wide: resident workgroups (set-up).
 unsigned p_step_cap_wide(int K, bool penalty);
 hipError_t launch_p_step_wide(int K, int64_t J, const double* res_p, const double* p, double* p_next, const StepOpts& o,
-                              int* fails, const PStepLists& lists, unsigned cap, int ndb, hipStream_t s, double* dp);
+                              int* fails, const PStepLists& lists, int ndb, hipStream_t s, double* dp);
 hipError_t launch_project_q(int K, int64_t I, double* q, double eps, hipStream_t s, const int* gate = nullptr);
 hipError_t launch_project_p(int K, int64_t J, double* p, double eps, hipStream_t s, const int* gate = nullptr);
 hipError_t launch_em_q(int K, int64_t I, int64_t J, const double* res_q, int NT, const int64_t* dm, const double* q,

@@ -445,7 +445,7 @@ hipError_t launch_p_step(int K, int64_t J, const double* res_p, const double* p,
   hipError_t e = launch_set_int(lists.late, 0, s, o.gate);
   if (e != hipSuccess) return e;
   constexpr int NDB = 64;  // 256 wavefronts for the early list (it holds a few blocks in the usual case)
-  if (K > HK_MAXK_REG) return launch_p_step_wide(K, J, res_p, p, p_next, o, fails, lists, cap, NDB, s, dp);
+  if (K > HK_MAXK_REG) return launch_p_step_wide(K, J, res_p, p, p_next, o, fails, lists, NDB, s, dp);
   const unsigned nq = (unsigned)((4 * J + 255) / 256), nd = (unsigned)(J < 1024 ? J : 1024);
   return with_k(K, [&](auto kc) {
     constexpr int KT = decltype(kc)::value;
