@@ -86,8 +86,9 @@ def _well_conditioned_blocks(XtX, K, J, ratio=1e-6):
 
 @pytest.mark.parametrize("I,J,K", [(400, 30, 9), (1000, 24, 12), (1500, 16, 16)])
 def test_wide_row_steps_match_oracle(I, J, K, oracle):
-    """update_q!/update_p! QP + step + projection: K-variable Q QPs and 4K-variable P QPs on the dense
-    one-wave solver, penalty and equality forms.  P results are compared on the blocks with well-conditioned
+    """update_q!/update_p! QP + step + projection: K-variable Q QPs, and 4K-variable P QPs on the four-lane
+    block-structured solver (blocks with a singular haplotype curvature on the dense one-wave solver in the
+    same launch), penalty and equality forms.  P results are compared on the blocks with well-conditioned
     haplotype curvature (at least 90 % of them here) and checked to be simplex points everywhere."""
     print(f"wide row steps I={I} J={J} K={K}", flush=True)
     s, q, p = case(I, J, K)
