@@ -36,7 +36,7 @@ WORKLOADS = {
     "mixed_I20k_J200k_K6": dict(I=20000, J=200000, K=6, fst=0.05, miss=0.02, seed=3, prec="mixed"),
     "fp64_I5k_J50k_K4": dict(I=5000, J=50000, K=4, fst=0.05, miss=0.01, seed=2, prec="fp64"),
     "cpu_I500_J2k_K3": dict(I=500, J=2000, K=3, fst=0.10, miss=0.01, seed=1, prec="fp64"),
-    # K above 8: windowed curvature passes and dense P row QPs
+    # K above 8: windowed curvature passes; four-lane P row QPs spilling to scratch
     "mixed_I20k_J50k_K12": dict(I=20000, J=50000, K=12, fst=0.05, miss=0.02, seed=5, prec="mixed"),
     # batched model selection: K = 2..8 x 4 restarts on data simulated with K = 5
     "select_I3k_J20k_K2-8x4": dict(I=3000, J=20000, K=5, fst=0.05, miss=0.01, seed=4, prec="mixed"),
